@@ -1,0 +1,361 @@
+"""CPU oracle for the tensor-completion hot path — TEST INFRASTRUCTURE ONLY.
+
+This package restates the reference's update rules on the CPU (FP64, plus an
+FP32 twin used as the timed CPU baseline).  It may be imported by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` only;
+the product (``cutc_amd``) never imports it.
+
+Reference files restated (SG = ``software/single GPU/sgpu``):
+  SG/io.c:58-106,234-308     .tns reader           -> tns_read
+  SG/ciss.c:68-195           CISS layout           -> ciss_reference_layout
+  SG/als.cu:186-317,467-491  ALS mode update       -> als_mode / als
+  SG/sgd.cu:21-143,264-324   SGD epoch             -> sgd_epoch / sgd
+  SG/ccd.cu:532-554, NC/ccd.cu:28-180  CCD++       -> ccd_epoch / ccd
+  SG/completion.c:123-214,296-362      loss/RMSE   -> loss_sq / rmse / converge
+
+Pins (each function is checked against something it is not a retyping of):
+  tns_read                 hand-written files (tests/golden/tns/*)
+  ciss_reference_layout    hand-worked layouts (tests/golden/ciss_*.json) and a
+                           brute-force decode of every bitmap
+  mttkrp / gram_rows       dense numpy einsum over the densified tensor
+  cholesky / solve         scipy.linalg.cho_factor / cho_solve
+  als_mode                 per-row ridge regression via numpy.linalg.lstsq on the
+                           stacked [H; sqrt(lambda) I] system; exact rank-1 recovery
+  sgd_epoch                single-nonzero closed form; two-nonzero sequential form
+  ccd_epoch                per-column 1-D ridge optimality (zero partial gradient)
+                           and monotone decrease of the objective
+  grad                     central finite differences in FP64
+  loss_sq / converge       closed forms
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+MAX_MODES = 4
+
+
+class OrcCoo(C.Structure):
+    _fields_ = [
+        ("nmodes", C.c_uint32),
+        ("nnz", C.c_uint64),
+        ("dims", C.c_uint64 * MAX_MODES),
+        ("ind", C.POINTER(C.c_uint32) * MAX_MODES),
+        ("vals", C.POINTER(C.c_double)),
+        ("offset", C.c_uint64 * MAX_MODES),
+    ]
+
+
+class OrcConv(C.Structure):
+    _fields_ = [
+        ("best_rmse", C.c_double),
+        ("tolerance", C.c_double),
+        ("nbadepochs", C.c_uint64),
+        ("bestepoch", C.c_uint64),
+        ("max_badepochs", C.c_uint64),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: run `make -C {_HERE}`")
+        _LIB = C.CDLL(path)
+    return _LIB
+
+
+@dataclass
+class Coo:
+    """COO tensor with 0-based uint32 indices and float64 values."""
+
+    ind: list
+    vals: np.ndarray
+    dims: tuple
+    offset: tuple = field(default=None)
+
+    @property
+    def nmodes(self):
+        return len(self.ind)
+
+    @property
+    def nnz(self):
+        return int(self.vals.shape[0])
+
+    def __post_init__(self):
+        self.ind = [np.ascontiguousarray(a, dtype=np.uint32) for a in self.ind]
+        self.vals = np.ascontiguousarray(self.vals, dtype=np.float64)
+        self.dims = tuple(int(d) for d in self.dims)
+        if self.offset is None:
+            self.offset = tuple(0 for _ in self.ind)
+        self._c = None
+
+    def c(self):
+        s = OrcCoo()
+        s.nmodes = self.nmodes
+        s.nnz = self.nnz
+        for m in range(self.nmodes):
+            s.dims[m] = self.dims[m]
+            s.ind[m] = self.ind[m].ctypes.data_as(C.POINTER(C.c_uint32))
+            s.offset[m] = self.offset[m]
+        s.vals = self.vals.ctypes.data_as(C.POINTER(C.c_double))
+        self._c = s
+        return C.byref(s)
+
+    def permuted(self, perm):
+        return Coo([a[perm] for a in self.ind], self.vals[perm], self.dims, self.offset)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _ptrs(arrs, kind):
+    T = C.POINTER(C.c_double) if kind == "f64" else C.POINTER(C.c_float)
+    arr = (T * MAX_MODES)()
+    for i, a in enumerate(arrs):
+        arr[i] = a.ctypes.data_as(T)
+    return arr
+
+
+def _prec(F):
+    return "f64" if F[0].dtype == np.float64 else "f32"
+
+
+def tns_read(path):
+    s = OrcCoo()
+    st = lib().orc_tns_read(path.encode(), C.byref(s))
+    if st != 0:
+        raise OSError(st, f"orc_tns_read({path}) failed with status {st}")
+    try:
+        n = s.nnz
+        ind = [np.ctypeslib.as_array(s.ind[m], shape=(n,)).copy() for m in range(s.nmodes)]
+        vals = np.ctypeslib.as_array(s.vals, shape=(n,)).copy()
+        return Coo(ind, vals, tuple(s.dims[m] for m in range(s.nmodes)),
+                   tuple(s.offset[m] for m in range(s.nmodes)))
+    finally:
+        lib().orc_coo_free(C.byref(s))
+
+
+def mode_order(t: Coo, mode: int):
+    perm = np.empty(t.nnz, np.uint32)
+    rowptr = np.empty(t.dims[mode] + 1, np.uint64)
+    st = lib().orc_mode_order(t.c(), C.c_uint32(mode), _u32p(perm), _u64p(rowptr))
+    assert st == 0
+    return perm, rowptr
+
+
+def sort_by_mode(t: Coo, mode: int) -> Coo:
+    perm, _ = mode_order(t, mode)
+    return t.permuted(perm)
+
+
+def ciss_reference_layout(t_sorted: Coo, mode: int, tile_len: int = 32, truncate_vals=False):
+    """Reference TB-COO arrays (SG/ciss.c:68-195).  Fields the reference leaves
+    unwritten are NaN."""
+    n = t_sorted.nnz
+    directory = np.zeros(n + 1, np.uint64)
+    dcounter = np.zeros(n + 2, np.uint64)
+    entries = np.full(3 * (n // tile_len + 1 + n + 1), np.nan)
+    dl = C.c_uint64()
+    sz = C.c_uint64()
+    st = lib().orc_ciss_reference_layout(t_sorted.c(), C.c_uint32(mode), C.c_uint32(tile_len),
+                                         C.c_int(1 if truncate_vals else 0), _u64p(directory),
+                                         _u64p(dcounter), _dp(entries), C.byref(dl), C.byref(sz))
+    if st != 0:
+        raise ValueError(f"orc_ciss_reference_layout status {st}")
+    d = dl.value
+    return directory[:d].copy(), dcounter[:d + 1].copy(), entries[: 3 * sz.value].reshape(-1, 3).copy()
+
+
+def _fn(name, F):
+    return getattr(lib(), f"{name}_{_prec(F)}")
+
+
+def loss_sq(t: Coo, F, nthreads=1):
+    f = _fn("orc_loss_sq", F)
+    f.restype = C.c_double
+    return f(t.c(), C.c_uint32(F[0].shape[1]), _ptrs(F, _prec(F)), C.c_int(nthreads))
+
+
+def rmse(t: Coo, F, nthreads=1):
+    return float(np.sqrt(loss_sq(t, F, nthreads) / t.nnz))
+
+
+def frob_sq(dims, F, lam):
+    f = _fn("orc_frob_sq", F)
+    f.restype = C.c_double
+    d = (C.c_uint64 * MAX_MODES)(*dims)
+    return f(C.c_uint32(len(F)), d, C.c_uint32(F[0].shape[1]), _ptrs(F, _prec(F)), C.c_double(lam))
+
+
+def mttkrp(t: Coo, mode, F, order=None, nthreads=1):
+    perm, rowptr = order if order is not None else mode_order(t, mode)
+    R = F[0].shape[1]
+    out = np.zeros((t.dims[mode], R), F[0].dtype)
+    p = _fp(out) if out.dtype == np.float32 else _dp(out)
+    st = _fn("orc_mttkrp", F)(t.c(), C.c_uint32(mode), C.c_uint32(R), _ptrs(F, _prec(F)),
+                              _u32p(perm), _u64p(rowptr), p, C.c_int(nthreads))
+    assert st == 0
+    return out
+
+
+def gram_rows(t: Coo, mode, F, row0=0, nrows=None, order=None):
+    perm, rowptr = order if order is not None else mode_order(t, mode)
+    R = F[0].shape[1]
+    if nrows is None:
+        nrows = t.dims[mode] - row0
+    G = np.zeros((nrows, R, R), np.float64)
+    st = _fn("orc_gram_rows", F)(t.c(), C.c_uint32(mode), C.c_uint32(R), _ptrs(F, _prec(F)),
+                                 _u32p(perm), _u64p(rowptr), C.c_uint64(row0), C.c_uint64(nrows),
+                                 _dp(G))
+    assert st == 0
+    return G
+
+
+def als_mode(t: Coo, mode, F, lam, order=None, nthreads=1):
+    """In-place ALS update of F[mode].  Returns the status (0 ok, 7 not SPD)."""
+    perm, rowptr = order if order is not None else mode_order(t, mode)
+    R = F[0].shape[1]
+    return _fn("orc_als_mode", F)(t.c(), C.c_uint32(mode), C.c_uint32(R), _ptrs(F, _prec(F)),
+                                  C.c_double(lam), _u32p(perm), _u64p(rowptr), C.c_int(nthreads))
+
+
+def sgd_epoch(t: Coo, F, lr, lam, scale=1.0, order=None):
+    R = F[0].shape[1]
+    op = _u32p(order) if order is not None else None
+    _fn("orc_sgd_epoch", F)(t.c(), op, C.c_uint32(R), _ptrs(F, _prec(F)), C.c_double(lr),
+                            C.c_double(lam), C.c_double(scale))
+
+
+def residual(t: Coo, F):
+    res = np.empty(t.nnz, F[0].dtype)
+    p = _fp(res) if res.dtype == np.float32 else _dp(res)
+    _fn("orc_residual", F)(t.c(), C.c_uint32(F[0].shape[1]), _ptrs(F, _prec(F)), p)
+    return res
+
+
+def ccd_epoch(t: Coo, F, lam, res, inner=1, orders=None, nthreads=1):
+    R = F[0].shape[1]
+    if orders is None:
+        orders = [mode_order(t, m) for m in range(t.nmodes)]
+    P = (C.POINTER(C.c_uint32) * MAX_MODES)()
+    RP = (C.POINTER(C.c_uint64) * MAX_MODES)()
+    for m, (perm, rowptr) in enumerate(orders):
+        P[m] = _u32p(perm)
+        RP[m] = _u64p(rowptr)
+    p = _fp(res) if res.dtype == np.float32 else _dp(res)
+    st = _fn("orc_ccd_epoch", F)(t.c(), C.c_uint32(R), _ptrs(F, _prec(F)), C.c_double(lam),
+                                 C.c_uint32(inner), p, P, RP, C.c_int(nthreads))
+    assert st == 0
+
+
+def grad(t: Coo, F, lam, reg_per_occurrence=False):
+    G = [np.zeros_like(f) for f in F]
+    f = _fn("orc_grad", F)
+    f.restype = C.c_double
+    loss = f(t.c(), C.c_uint32(F[0].shape[1]), _ptrs(F, _prec(F)), C.c_double(lam),
+             C.c_int(1 if reg_per_occurrence else 0), _ptrs(G, _prec(F)))
+    return loss, G
+
+
+def cholesky(A):
+    A = np.array(A, np.float64, order="C", copy=True)
+    st = lib().orc_cholesky(_dp(A), C.c_uint32(A.shape[0]))
+    if st != 0:
+        raise np.linalg.LinAlgError("not SPD")
+    return A
+
+
+def cholesky_solve(L, b):
+    x = np.array(b, np.float64, copy=True)
+    lib().orc_cholesky_solve(_dp(np.ascontiguousarray(L)), C.c_uint32(L.shape[0]), _dp(x))
+    return x
+
+
+class Converge:
+    """SG/completion.c:296-362 bookkeeping."""
+
+    def __init__(self, tolerance=1e-4, max_badepochs=3, best_rmse=1e7):
+        self.s = OrcConv(best_rmse, tolerance, 0, 0, max_badepochs)
+
+    def step(self, rmse_value, epoch):
+        imp = C.c_int()
+        conv = lib().orc_converge(C.byref(self.s), C.c_double(rmse_value), C.c_uint64(epoch),
+                                  C.byref(imp))
+        return bool(conv), bool(imp.value)
+
+
+# ---------------------------------------------------------------------------
+# whole-solver drivers (the reference's epoch loops, SG/als.cu:676-810,
+# SG/sgd.cu:276-324, SG/ccd.cu:524-626), fixed mode order 0..N-1.
+# ---------------------------------------------------------------------------
+
+def als(t: Coo, F, lam, iters, test: Coo | None = None, nthreads=1):
+    """Runs `iters` ALS sweeps in place.  Returns per-iteration factor copies and
+    test RMSE."""
+    orders = [mode_order(t, m) for m in range(t.nmodes)]
+    hist, rm = [], []
+    for _ in range(iters):
+        for m in range(t.nmodes):
+            st = als_mode(t, m, F, lam, order=orders[m], nthreads=nthreads)
+            if st != 0:
+                raise np.linalg.LinAlgError(f"ALS system not SPD in mode {m}")
+        hist.append([f.copy() for f in F])
+        if test is not None:
+            rm.append(rmse(test, F))
+    return hist, rm
+
+
+def sgd(t: Coo, F, lr, lam, epochs, test: Coo | None = None, scale=1.0, decay=None,
+        bold_driver=False, order=None):
+    """Sequential SGD in mode-0 order.  decay: multiply lr by it after each
+    epoch (exponential schedule).  bold_driver: the reference schedule
+    (SG/sgd.cu:314-323)."""
+    if order is None:
+        order, _ = mode_order(t, 0)
+    rm, lrs = [], []
+    prev_obj = None
+    for e in range(1, epochs + 1):
+        lrs.append(lr)
+        sgd_epoch(t, F, lr, lam, scale, order=order)
+        if test is not None:
+            rm.append(rmse(test, F))
+        if bold_driver:
+            obj = loss_sq(t, F) + frob_sq(t.dims, F, lam)
+            if e > 1:
+                lr = lr * 1.05 if obj < prev_obj else lr * 0.5
+            prev_obj = obj
+        elif decay is not None:
+            lr *= decay
+    return rm, lrs
+
+
+def ccd(t: Coo, F, lam, epochs, test: Coo | None = None, inner=1, nthreads=1):
+    orders = [mode_order(t, m) for m in range(t.nmodes)]
+    res = residual(t, F)
+    hist, rm = [], []
+    for _ in range(epochs):
+        ccd_epoch(t, F, lam, res, inner=inner, orders=orders, nthreads=nthreads)
+        hist.append([f.copy() for f in F])
+        if test is not None:
+            rm.append(rmse(test, F))
+    return hist, rm

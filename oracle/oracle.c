@@ -1,0 +1,309 @@
+/*
+ * oracle.c — CPU restatement of abovekumo/cuTC (reference) for the tensor
+ * completion hot path.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Reference files restated here (paths relative to the reference root,
+ * SG = "software/single GPU/sgpu", NC = "software/multiple GPU_NCCL"):
+ *   SG/io.c:58-106, 234-308     .tns text parsing and dims / index base
+ *   SG/ciss.c:68-195            CISS (TB-COO) layout
+ *   SG/als.cu:186-317, 467-491, 676-716   ALS per-mode update
+ *   SG/sgd.cu:21-143, 264-324   SGD update rule
+ *   SG/ccd.cu:532-554, NC/ccd.cu:28-180   canonical CCD++
+ *   SG/completion.c:123-214, 296-362      loss / RMSE / Frobenius / convergence
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* .tns reader                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Lines are data lines when getline returned more than one byte and the first
+ * byte is not '#' (SG/io.c:93, :251, :272). */
+static int is_data_line(const char *line, ssize_t read) {
+  return read > 1 && line[0] != '#';
+}
+
+int orc_tns_read(const char *path, orc_coo *out) {
+  memset(out, 0, sizeof(*out));
+  FILE *fin = fopen(path, "r");
+  if (!fin) return 2; /* SG/io.c:199-202 returns NULL */
+  char *line = NULL;
+  size_t len = 0;
+  ssize_t read;
+  /* mode count = whitespace-separated tokens of the first data line minus one
+   * (SG/io.c:250-261; strtok on " \t"). */
+  uint32_t nmodes = 0;
+  while ((read = getline(&line, &len, fin)) != -1) {
+    if (is_data_line(line, read)) {
+      char *save = NULL;
+      for (char *p = strtok_r(line, " \t", &save); p; p = strtok_r(NULL, " \t", &save))
+        ++nmodes;
+      break;
+    }
+  }
+  if (nmodes < 2) {
+    free(line);
+    fclose(fin);
+    return 3;
+  }
+  --nmodes;
+  if (nmodes > ORC_MAX_MODES) {
+    free(line);
+    fclose(fin);
+    return 3;
+  }
+  uint64_t dims[ORC_MAX_MODES], offset[ORC_MAX_MODES];
+  for (uint32_t m = 0; m < nmodes; ++m) {
+    dims[m] = 0;
+    offset[m] = 1; /* SG/io.c:263-266 */
+  }
+  uint64_t nnz = 0;
+  rewind(fin);
+  while ((read = getline(&line, &len, fin)) != -1) {
+    if (!is_data_line(line, read)) continue;
+    char *ptr = line;
+    for (uint32_t m = 0; m < nmodes; ++m) {
+      uint64_t ind = strtoull(ptr, &ptr, 10);
+      if (ind > dims[m]) dims[m] = ind;     /* dims = max index, :278 */
+      if (ind < offset[m]) offset[m] = ind; /* base = min index, :281 */
+    }
+    strtod(ptr, &ptr);
+    ++nnz;
+  }
+  for (uint32_t m = 0; m < nmodes; ++m) {
+    if (offset[m] != 0 && offset[m] != 1) { /* SG/io.c:291-296 */
+      free(line);
+      fclose(fin);
+      return 4;
+    }
+    if (offset[m] == 0) ++dims[m]; /* SG/io.c:299-304 */
+  }
+  if (nnz == 0) {
+    free(line);
+    fclose(fin);
+    return 3;
+  }
+  out->nmodes = nmodes;
+  out->nnz = nnz;
+  for (uint32_t m = 0; m < nmodes; ++m) {
+    out->dims[m] = dims[m];
+    out->offset[m] = offset[m];
+    out->ind[m] = (uint32_t *)malloc(nnz * sizeof(uint32_t));
+  }
+  out->vals = (double *)malloc(nnz * sizeof(double));
+  rewind(fin);
+  uint64_t n = 0;
+  while ((read = getline(&line, &len, fin)) != -1) {
+    if (!is_data_line(line, read)) continue;
+    char *ptr = line;
+    for (uint32_t m = 0; m < nmodes; ++m) {
+      uint64_t ind = strtoul(ptr, &ptr, 10); /* SG/io.c:96 */
+      out->ind[m][n] = (uint32_t)(ind - offset[m]);
+    }
+    out->vals[n++] = strtod(ptr, &ptr); /* SG/io.c:98 */
+  }
+  free(line);
+  fclose(fin);
+  return 0;
+}
+
+void orc_coo_free(orc_coo *t) {
+  for (uint32_t m = 0; m < ORC_MAX_MODES; ++m) {
+    free(t->ind[m]);
+    t->ind[m] = NULL;
+  }
+  free(t->vals);
+  t->vals = NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ordering                                                                  */
+/* ------------------------------------------------------------------------ */
+
+static void counting_pass(const uint32_t *key, uint64_t dim, uint64_t nnz,
+                          const uint32_t *in, uint32_t *out, uint64_t *count) {
+  memset(count, 0, (dim + 1) * sizeof(uint64_t));
+  for (uint64_t p = 0; p < nnz; ++p) count[key[in[p]] + 1]++;
+  for (uint64_t d = 0; d < dim; ++d) count[d + 1] += count[d];
+  for (uint64_t p = 0; p < nnz; ++p) out[count[key[in[p]]]++] = in[p];
+}
+
+int orc_mode_order(const orc_coo *t, uint32_t mode, uint32_t *perm, uint64_t *rowptr) {
+  if (mode >= t->nmodes) return 1;
+  uint64_t nnz = t->nnz;
+  uint64_t maxdim = 0;
+  for (uint32_t m = 0; m < t->nmodes; ++m)
+    if (t->dims[m] > maxdim) maxdim = t->dims[m];
+  uint32_t *tmp = (uint32_t *)malloc(nnz * sizeof(uint32_t));
+  uint64_t *count = (uint64_t *)malloc((maxdim + 1) * sizeof(uint64_t));
+  for (uint64_t p = 0; p < nnz; ++p) perm[p] = (uint32_t)p;
+  /* least significant key first: modes mode+N-1, ..., mode+1, mode */
+  uint32_t *a = perm, *b = tmp;
+  for (int s = (int)t->nmodes - 1; s >= 0; --s) {
+    uint32_t m = (mode + (uint32_t)s) % t->nmodes;
+    counting_pass(t->ind[m], t->dims[m], nnz, a, b, count);
+    uint32_t *x = a;
+    a = b;
+    b = x;
+  }
+  if (a != perm) memcpy(perm, a, nnz * sizeof(uint32_t));
+  if (rowptr) {
+    uint64_t dim = t->dims[mode];
+    memset(rowptr, 0, (dim + 1) * sizeof(uint64_t));
+    for (uint64_t p = 0; p < nnz; ++p) rowptr[t->ind[mode][p] + 1]++;
+    for (uint64_t d = 0; d < dim; ++d) rowptr[d + 1] += rowptr[d];
+  }
+  free(tmp);
+  free(count);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* reference CISS layout (SG/ciss.c:68-195)                                  */
+/* ------------------------------------------------------------------------ */
+
+int orc_ciss_reference_layout(const orc_coo *t, uint32_t mode, uint32_t tile_len,
+                              int truncate_vals, uint64_t *directory,
+                              uint64_t *dcounter, double *entries,
+                              uint64_t *dlength, uint64_t *size) {
+  if (mode >= t->nmodes || t->nmodes != 3 || tile_len == 0 || t->nnz == 0) return 1;
+  const uint64_t nnz = t->nnz;
+  const uint32_t m1 = (mode + 1) % 3, m2 = (mode + 2) % 3;
+  /* raw ids as the reference holds them: 1-based (the reference's kernels
+   * subtract one, SG/als.cu:256-258). */
+#define RAW(mm, p) ((uint64_t)t->ind[(mm)][(p)] + 1u)
+
+  /* directory + dcounter (SG/ciss.c:89-115).  The reference accumulates
+   * dcounter in a scratch of length nnz indexed up to dlength; the values it
+   * keeps are dcounter[0] = 0, dcounter[c] = index of the last nonzero of
+   * slot c-1, dcounter[dlength] = nnz - 1. */
+  uint64_t counter = 1;
+  uint64_t last = RAW(mode, 0);
+  directory[0] = last;
+  dcounter[0] = 0;
+  for (uint64_t p = 1; p < nnz; ++p) {
+    if (RAW(mode, p) != last) {
+      directory[counter] = RAW(mode, p);
+      dcounter[counter] = p - 1;
+      ++counter;
+      last = RAW(mode, p);
+    }
+  }
+  /* dcounter[1] is the running count after slot 0, which the reference's
+   * accumulation leaves equal to the last index of slot 0 (SG/ciss.c:104-107). */
+  dcounter[counter] = nnz - 1;
+  *dlength = counter;
+
+  /* tile records (SG/ciss.c:121-188) */
+  const uint64_t L = tile_len;
+  const uint64_t tiles = nnz / L + 1;
+  *size = tiles + nnz + 1;
+  uint64_t position = 0, slot_l = 0;
+  uint64_t p = 0;
+  while (p < nnz + 1) {
+    uint64_t bitmap = 1;
+    const uint64_t slot_s = slot_l;
+    uint64_t running = directory[slot_s];
+    double *tile = entries + position;
+    tile[0] = -1.0 * (double)slot_s; /* header[0] = -(first slot); -0.0 for slot 0 */
+    for (uint64_t lc = 1; lc <= L; ++lc) {
+      if (p == nnz) { /* sentinel record, SG/ciss.c:144-151 */
+        tile[lc * 3] = -1;
+        tile[lc * 3 + 1] = -1;
+        tile[lc * 3 + 2] = -1;
+        ++p;
+        break;
+      }
+      const uint64_t ti = RAW(mode, p);
+      if (ti == running) {
+        bitmap = (bitmap << 1) + 1;
+      } else {
+        bitmap = bitmap << 1;
+        running = ti;
+        ++slot_l;
+        tile[1] = -1.0 * (double)slot_l; /* written only on a row change, :171 */
+      }
+      tile[lc * 3] = (double)RAW(m1, p);
+      tile[lc * 3 + 1] = (double)RAW(m2, p);
+      double v = t->vals[p];
+      if (truncate_vals) v = (double)(uint64_t)v; /* idx_t tmpv, quirk Q5 */
+      tile[lc * 3 + 2] = v;
+      tile[2] = (double)bitmap;
+      ++p;
+    }
+    position += (L + 1) * 3;
+  }
+#undef RAW
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dense helpers                                                             */
+/* ------------------------------------------------------------------------ */
+
+int orc_cholesky(double *A, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t j = 0; j <= i; ++j) {
+      double s = A[(size_t)i * n + j];
+      for (uint32_t k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      if (i == j) {
+        if (!(s > 0.0)) return 7;
+        A[(size_t)i * n + i] = sqrt(s);
+      } else {
+        A[(size_t)i * n + j] = s / A[(size_t)j * n + j];
+      }
+    }
+    for (uint32_t j = i + 1; j < n; ++j) A[(size_t)i * n + j] = 0.0;
+  }
+  return 0;
+}
+
+void orc_cholesky_solve(const double *L, uint32_t n, double *b) {
+  for (uint32_t i = 0; i < n; ++i) { /* L y = b */
+    double s = b[i];
+    for (uint32_t k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * b[k];
+    b[i] = s / L[(size_t)i * n + i];
+  }
+  for (int i = (int)n - 1; i >= 0; --i) { /* L^T x = y */
+    double s = b[i];
+    for (uint32_t k = (uint32_t)i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * b[k];
+    b[i] = s / L[(size_t)i * n + i];
+  }
+}
+
+int orc_converge(orc_conv *c, double converge_rmse, uint64_t epoch, int *improved) {
+  *improved = 0;
+  if (converge_rmse - c->best_rmse < -c->tolerance) { /* SG/completion.c:332-341 */
+    c->nbadepochs = 0;
+    c->best_rmse = converge_rmse;
+    c->bestepoch = epoch;
+    *improved = 1;
+    return 0;
+  }
+  c->nbadepochs += 1; /* :343-346 */
+  return c->nbadepochs == c->max_badepochs;
+}
+
+/* ------------------------------------------------------------------------ */
+/* precision-templated kernels                                               */
+/* ------------------------------------------------------------------------ */
+#define REAL double
+#define SFX(name) name##_f64
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX
+
+#define REAL float
+#define SFX(name) name##_f32
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX

@@ -1,0 +1,272 @@
+/*
+ * oracle_impl.h — precision-templated body of the oracle (included twice by
+ * oracle.c with REAL = double and REAL = float).  TEST INFRASTRUCTURE ONLY.
+ */
+
+/* prediction <F_0(i0,:), F_1(i1,:), ...> for nonzero p, accumulated in REAL
+ * (SG/completion.c:29-51 with the 0-based row index the kernels use,
+ * SG/als.cu:256-258; quirk Q6 is not reproduced). */
+static inline REAL SFX(orc_predict)(const orc_coo *t, uint32_t R, REAL *const *F, uint64_t p) {
+  REAL est = 0;
+  for (uint32_t f = 0; f < R; ++f) {
+    REAL v = F[0][(size_t)t->ind[0][p] * R + f];
+    for (uint32_t m = 1; m < t->nmodes; ++m) v *= F[m][(size_t)t->ind[m][p] * R + f];
+    est += v;
+  }
+  return est;
+}
+
+double SFX(orc_loss_sq)(const orc_coo *t, uint32_t R, REAL *const *F, int nthreads) {
+  double loss = 0.0;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : loss) schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t p = 0; p < (int64_t)t->nnz; ++p) {
+    double err = t->vals[p] - (double)SFX(orc_predict)(t, R, F, (uint64_t)p);
+    loss += err * err;
+  }
+  return loss;
+}
+
+double SFX(orc_frob_sq)(uint32_t nmodes, const uint64_t *dims, uint32_t R,
+                        REAL *const *F, double lambda) {
+  double reg = 0.0;
+  for (uint32_t m = 0; m < nmodes; ++m) {
+    double acc = 0.0;
+    for (size_t x = 0; x < (size_t)dims[m] * R; ++x) acc += (double)F[m][x] * (double)F[m][x];
+    reg += lambda * acc;
+  }
+  return reg;
+}
+
+/* h = hadamard product of the rows of every mode except `mode` for nonzero p */
+static inline void SFX(orc_hrow)(const orc_coo *t, uint32_t mode, uint32_t R, REAL *const *F,
+                                 uint64_t p, REAL *h) {
+  int first = 1;
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    if (m == mode) continue;
+    const REAL *row = F[m] + (size_t)t->ind[m][p] * R;
+    if (first) {
+      for (uint32_t f = 0; f < R; ++f) h[f] = row[f];
+      first = 0;
+    } else {
+      for (uint32_t f = 0; f < R; ++f) h[f] *= row[f];
+    }
+  }
+}
+
+int SFX(orc_mttkrp)(const orc_coo *t, uint32_t mode, uint32_t R, REAL *const *F,
+                    const uint32_t *perm, const uint64_t *rowptr, REAL *out, int nthreads) {
+  if (mode >= t->nmodes || R == 0) return 1;
+  const int64_t dim = (int64_t)t->dims[mode];
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t i = 0; i < dim; ++i) {
+    REAL h[512];
+    REAL *o = out + (size_t)i * R;
+    for (uint32_t f = 0; f < R; ++f) o[f] = 0;
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      uint64_t p = perm[q];
+      SFX(orc_hrow)(t, mode, R, F, p, h);
+      REAL x = (REAL)t->vals[p];
+      for (uint32_t f = 0; f < R; ++f) o[f] += x * h[f];
+    }
+  }
+  return 0;
+}
+
+int SFX(orc_gram_rows)(const orc_coo *t, uint32_t mode, uint32_t R, REAL *const *F,
+                       const uint32_t *perm, const uint64_t *rowptr, uint64_t row0,
+                       uint64_t nrows, double *G) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  REAL h[512];
+  for (uint64_t r = 0; r < nrows; ++r) {
+    uint64_t i = row0 + r;
+    double *g = G + (size_t)r * R * R;
+    for (size_t x = 0; x < (size_t)R * R; ++x) g[x] = 0.0;
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      SFX(orc_hrow)(t, mode, R, F, perm[q], h);
+      for (uint32_t a = 0; a < R; ++a)
+        for (uint32_t b = 0; b < R; ++b) g[(size_t)a * R + b] += (double)(h[a] * h[b]);
+    }
+  }
+  return 0;
+}
+
+int SFX(orc_als_mode)(const orc_coo *t, uint32_t mode, uint32_t R, REAL **F, double lambda,
+                      const uint32_t *perm, const uint64_t *rowptr, int nthreads) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  const int64_t dim = (int64_t)t->dims[mode];
+  int status = 0;
+  /* the reference zeroes the factor (SG/als.cu:699) and then solves only the
+   * rows present in the directory (SG/als.cu:481-490, 715-716), so a row
+   * with no nonzero stays 0.  The other modes' factors are read while
+   * F[mode] is rewritten, which is safe because they are different arrays. */
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t i = 0; i < dim; ++i) {
+    REAL *dst = F[mode] + (size_t)i * R;
+    if (rowptr[i] == rowptr[i + 1]) {
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0;
+      continue;
+    }
+    REAL h[512];
+    REAL gacc[512 * 2];
+    double *G = (double *)malloc((size_t)R * R * sizeof(double));
+    double *rhs = (double *)malloc((size_t)R * sizeof(double));
+    REAL *g = (REAL *)malloc((size_t)R * R * sizeof(REAL));
+    for (size_t x = 0; x < (size_t)R * R; ++x) g[x] = 0;
+    for (uint32_t f = 0; f < R; ++f) gacc[f] = 0;
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      uint64_t p = perm[q];
+      SFX(orc_hrow)(t, mode, R, F, p, h);
+      REAL x = (REAL)t->vals[p];
+      /* lower triangle of h h^T (SG/als.cu:271-291) and x * h (:295-312) */
+      for (uint32_t a = 0; a < R; ++a)
+        for (uint32_t b = 0; b <= a; ++b) g[(size_t)a * R + b] += h[a] * h[b];
+      for (uint32_t f = 0; f < R; ++f) gacc[f] += x * h[f];
+    }
+    for (uint32_t a = 0; a < R; ++a) {
+      for (uint32_t b = 0; b <= a; ++b) {
+        G[(size_t)a * R + b] = (double)g[(size_t)a * R + b];
+        G[(size_t)b * R + a] = (double)g[(size_t)a * R + b];
+      }
+      G[(size_t)a * R + a] += lambda; /* p_update_matrice, SG/als.cu:484-487 */
+      rhs[a] = (double)gacc[a];
+    }
+    if (orc_cholesky(G, R) != 0) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+      status = 7;
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0;
+    } else {
+      orc_cholesky_solve(G, R, rhs);
+      for (uint32_t f = 0; f < R; ++f) dst[f] = (REAL)rhs[f];
+    }
+    free(G);
+    free(rhs);
+    free(g);
+  }
+  return status;
+}
+
+void SFX(orc_sgd_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R, REAL **F,
+                        double lr, double lambda, double scale) {
+  const uint32_t N = t->nmodes;
+  const REAL step = (REAL)(lr * scale);
+  const REAL lam = (REAL)lambda;
+  REAL buf[ORC_MAX_MODES][512];
+  for (uint64_t q = 0; q < t->nnz; ++q) {
+    uint64_t p = order ? order[q] : q;
+    REAL *rows[ORC_MAX_MODES];
+    for (uint32_t m = 0; m < N; ++m) {
+      rows[m] = F[m] + (size_t)t->ind[m][p] * R;
+      for (uint32_t f = 0; f < R; ++f) buf[m][f] = rows[m][f]; /* SG/sgd.cu:75-84 */
+    }
+    REAL pred = 0; /* :86-89 */
+    for (uint32_t f = 0; f < R; ++f) {
+      REAL v = buf[0][f];
+      for (uint32_t m = 1; m < N; ++m) v *= buf[m][f];
+      pred += v;
+    }
+    const REAL err = (REAL)t->vals[p] - pred; /* :90 */
+    for (uint32_t f = 0; f < R; ++f) {          /* :92-99 */
+      for (uint32_t m = 0; m < N; ++m) {
+        REAL others = 1;
+        for (uint32_t o = 0; o < N; ++o)
+          if (o != m) others *= buf[o][f];
+        REAL mod = err * others - lam * buf[m][f];
+        rows[m][f] += step * mod;
+      }
+    }
+  }
+}
+
+void SFX(orc_residual)(const orc_coo *t, uint32_t R, REAL *const *F, REAL *res) {
+  for (uint64_t p = 0; p < t->nnz; ++p)
+    res[p] = (REAL)t->vals[p] - SFX(orc_predict)(t, R, F, p);
+}
+
+/* rank-one component f at nonzero p */
+static inline REAL SFX(orc_comp)(const orc_coo *t, uint32_t R, REAL *const *F, uint32_t f,
+                                 uint64_t p) {
+  REAL v = F[0][(size_t)t->ind[0][p] * R + f];
+  for (uint32_t m = 1; m < t->nmodes; ++m) v *= F[m][(size_t)t->ind[m][p] * R + f];
+  return v;
+}
+
+int SFX(orc_ccd_epoch)(const orc_coo *t, uint32_t R, REAL **F, double lambda, uint32_t inner,
+                       REAL *res, uint32_t *const *perms, uint64_t *const *rowptrs,
+                       int nthreads) {
+  const uint32_t N = t->nmodes;
+  (void)nthreads;
+  for (uint32_t f = 0; f < R; ++f) {
+    /* add the current rank-one component back to the residual */
+    for (uint64_t p = 0; p < t->nnz; ++p) res[p] += SFX(orc_comp)(t, R, F, f, p);
+    for (uint32_t it = 0; it < inner; ++it) {
+      for (uint32_t m = 0; m < N; ++m) {
+        const int64_t dim = (int64_t)t->dims[m];
+        const uint32_t *perm = perms[m];
+        const uint64_t *rowptr = rowptrs[m];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+        for (int64_t i = 0; i < dim; ++i) {
+          REAL num = 0, den = 0;
+          for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            uint64_t p = perm[q];
+            REAL h = 1;
+            for (uint32_t o = 0; o < N; ++o)
+              if (o != m) h *= F[o][(size_t)t->ind[o][p] * R + f];
+            num += res[p] * h;
+            den += h * h;
+          }
+          /* a_if <- num / (lambda + den) (SG/ccd.cu:345, p_compute_newcol) */
+          F[m][(size_t)i * R + f] = (REAL)((double)num / (lambda + (double)den));
+        }
+      }
+    }
+    /* subtract the updated component */
+    for (uint64_t p = 0; p < t->nnz; ++p) res[p] -= SFX(orc_comp)(t, R, F, f, p);
+  }
+  return 0;
+}
+
+double SFX(orc_grad)(const orc_coo *t, uint32_t R, REAL *const *F, double lambda,
+                     int reg_per_occurrence, REAL **G) {
+  const uint32_t N = t->nmodes;
+  double loss = 0.0;
+  for (uint32_t m = 0; m < N; ++m)
+    for (size_t x = 0; x < (size_t)t->dims[m] * R; ++x) G[m][x] = 0;
+  REAL h[512];
+  for (uint64_t p = 0; p < t->nnz; ++p) {
+    const REAL e = (REAL)t->vals[p] - SFX(orc_predict)(t, R, F, p);
+    loss += 0.5 * (double)e * (double)e;
+    for (uint32_t m = 0; m < N; ++m) {
+      SFX(orc_hrow)(t, m, R, F, p, h);
+      REAL *g = G[m] + (size_t)t->ind[m][p] * R;
+      const REAL *a = F[m] + (size_t)t->ind[m][p] * R;
+      for (uint32_t f = 0; f < R; ++f) {
+        g[f] -= e * h[f];
+        if (reg_per_occurrence) g[f] += (REAL)lambda * a[f];
+      }
+      if (reg_per_occurrence)
+        for (uint32_t f = 0; f < R; ++f) loss += 0.5 * lambda * (double)a[f] * (double)a[f];
+    }
+  }
+  if (!reg_per_occurrence) {
+    for (uint32_t m = 0; m < N; ++m) {
+      for (size_t x = 0; x < (size_t)t->dims[m] * R; ++x) {
+        G[m][x] += (REAL)lambda * F[m][x];
+        loss += 0.5 * lambda * (double)F[m][x] * (double)F[m][x];
+      }
+    }
+  }
+  return loss;
+}

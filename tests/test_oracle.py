@@ -1,0 +1,486 @@
+"""Pins for the CPU oracle (oracle/): every routine is checked against something
+that is not a retyping of its own formula — hand-worked fixtures, dense numpy /
+scipy library routines, closed forms, invariants and finite differences."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_coo(rng, dims, nnz):
+    tot = int(np.prod(dims))
+    lin = rng.choice(tot, size=nnz, replace=False)
+    ind, rem = [], lin
+    for d in reversed(dims):
+        ind.append(rem % d)
+        rem = rem // d
+    ind = ind[::-1]
+    vals = rng.normal(size=nnz) + 3.0
+    return O.Coo(ind, vals, dims)
+
+
+def dense(t: O.Coo):
+    X = np.zeros(t.dims)
+    M = np.zeros(t.dims, bool)
+    X[tuple(t.ind)] = t.vals
+    M[tuple(t.ind)] = True
+    return X, M
+
+
+def rand_factors(rng, dims, R, dtype=np.float64):
+    return [rng.uniform(-1, 1, size=(d, R)).astype(dtype) for d in dims]
+
+
+# --------------------------------------------------------------------------
+# .tns reader (SG/io.c)
+# --------------------------------------------------------------------------
+
+def test_tns_one_based_with_comments_and_blank_lines():
+    t = O.tns_read(os.path.join(GOLD, "tns", "base1.tns"))
+    assert t.nmodes == 3 and t.nnz == 7
+    assert t.dims == (4, 3, 3)
+    assert t.offset == (1, 1, 1)
+    assert list(t.ind[0]) == [0, 0, 0, 1, 1, 3, 3]
+    assert list(t.ind[2]) == [0, 2, 0, 1, 1, 0, 2]
+    assert list(t.vals) == [1.5, 2.5, 3.5, 4.5, 5.5, 1.25, 2.0]
+
+
+def test_tns_zero_based_matches_one_based():
+    a = O.tns_read(os.path.join(GOLD, "tns", "base1.tns"))
+    b = O.tns_read(os.path.join(GOLD, "tns", "base0.tns"))
+    assert b.offset == (0, 0, 0)
+    assert a.dims == b.dims
+    for m in range(3):
+        assert np.array_equal(a.ind[m], b.ind[m])
+    assert np.array_equal(a.vals, b.vals)
+
+
+def test_tns_four_way():
+    t = O.tns_read(os.path.join(GOLD, "tns", "fourway.tns"))
+    assert t.nmodes == 4 and t.nnz == 3
+    assert t.dims == (3, 3, 2, 4)
+    assert list(t.vals) == [0.5, -1.5, 2.25]
+
+
+def test_tns_base_above_one_reads_as_one_based_and_missing_file():
+    # SG/io.c:263-281 starts the per-mode base at 1 and keeps the minimum, so a
+    # smallest index of 2 leaves the base at 1: the exit(1) branch at :291-296
+    # is unreachable and the file reads as 1-based with an empty first row.
+    t = O.tns_read(os.path.join(GOLD, "tns", "badbase.tns"))
+    assert t.offset == (1, 1, 1)
+    assert t.dims == (3, 2, 2)
+    assert list(t.ind[0]) == [1, 2]
+    with pytest.raises(OSError) as e:
+        O.tns_read(os.path.join(GOLD, "tns", "does_not_exist.tns"))
+    assert e.value.args[0] == 2
+
+
+# --------------------------------------------------------------------------
+# CISS reference layout (SG/ciss.c)
+# --------------------------------------------------------------------------
+
+def _case_coo(c):
+    a = np.array(c["coo"], dtype=np.float64)
+    return O.Coo([a[:, 0].astype(np.int64) - 1, a[:, 1].astype(np.int64) - 1,
+                  a[:, 2].astype(np.int64) - 1], a[:, 3], c["dims"])
+
+
+def _same(expected, got):
+    exp = np.array([[np.nan if v is None else v for v in r] for r in expected], dtype=np.float64)
+    assert exp.shape == got.shape, (exp.shape, got.shape)
+    both_nan = np.isnan(exp) & np.isnan(got)
+    eq = (exp == got) | both_nan
+    assert eq.all(), np.argwhere(~eq)
+    # -0.0 in header[0] of a tile whose first slot is 0 (SG/ciss.c:139)
+    zero = (exp == 0) & ~np.isnan(exp)
+    assert np.array_equal(np.signbit(exp[zero]), np.signbit(got[zero]))
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_ciss_reference_layout_hand_worked(idx):
+    cases = json.load(open(os.path.join(GOLD, "ciss_cases.json")))["cases"]
+    c = cases[idx]
+    t = _case_coo(c)
+    d, dc, ent = O.ciss_reference_layout(t, c["mode"], c["tile_len"])
+    assert list(d) == c["directory"]
+    assert list(dc) == c["dcounter"]
+    _same(c["entries"], ent)
+
+
+def test_ciss_truncation_quirk():
+    t = O.Coo([[0, 0], [0, 1], [0, 1]], [2.7, 4.2], (1, 2, 2))
+    _, _, ent = O.ciss_reference_layout(t, 0, 4, truncate_vals=False)
+    assert ent[1, 2] == 2.7 and ent[2, 2] == 4.2
+    _, _, ent = O.ciss_reference_layout(t, 0, 4, truncate_vals=True)
+    assert ent[1, 2] == 2.0 and ent[2, 2] == 4.0
+
+
+def _decode_reference(bitmap, lane, f_id):
+    """The kernels' decode (SG/als.cu:230-243) written with Python ints."""
+    b = int("{:064b}".format(bitmap)[::-1], 2)  # __brevll
+    while b & 1 == 0:
+        b >>= 1
+    b >>= 1
+    mask = (b << (63 - lane)) & ((1 << 64) - 1)
+    myfid = f_id + lane - bin(mask).count("1") + 1
+    mybit = (b >> lane) & 1
+    return myfid, mybit
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 8, 12])
+def test_ciss_bitmap_decode_exhaustive(L):
+    """Every continuation pattern of a full tile: the reference's popcount
+    decode equals a naive count of row changes."""
+    for pattern in range(1 << L):
+        cont = [(pattern >> e) & 1 for e in range(L)]  # 1 = entry continues the row
+        bitmap = 1
+        for e in range(L):
+            bitmap = (bitmap << 1) | cont[e]
+        f0 = 5
+        slot = f0
+        for e in range(L):
+            if not cont[e]:
+                slot += 1
+            fid, bit = _decode_reference(bitmap, e, f0)
+            assert bit == cont[e]
+            assert fid == slot
+
+
+def test_ciss_bitmap_decode_random_L32():
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        cont = rng.integers(0, 2, 32)
+        bitmap = 1
+        for e in range(32):
+            bitmap = (bitmap << 1) | int(cont[e])
+        slot = 11
+        for e in range(32):
+            slot += 0 if cont[e] else 1
+            assert _decode_reference(bitmap, e, 11) == (slot, int(cont[e]))
+
+
+def test_ciss_layout_random_roundtrip():
+    """Decoding the emitted layout with the reference's own decode returns the
+    sorted COO (directory slot -> row, records -> next indices and values)."""
+    rng = np.random.default_rng(5)
+    t = rand_coo(rng, (13, 7, 5), 150)
+    for mode in range(3):
+        ts = O.sort_by_mode(t, mode)
+        for L in (1, 4, 7, 32):
+            d, dc, ent = O.ciss_reference_layout(ts, mode, L)
+            got = []
+            n = ts.nnz
+            for tile in range(n // L + 1):
+                base = tile * (L + 1)
+                hdr = ent[base]
+                if tile * L >= n:
+                    assert tuple(ent[base + 1]) == (-1, -1, -1)
+                    continue
+                f_id = int(-hdr[0])
+                for e in range(L):
+                    rec = ent[base + 1 + e]
+                    if rec[0] == -1 and rec[1] == -1:
+                        break
+                    fid, _ = _decode_reference(int(hdr[2]), e, f_id)
+                    got.append((int(d[fid]) - 1, int(rec[0]) - 1, int(rec[1]) - 1, rec[2]))
+            m1, m2 = (mode + 1) % 3, (mode + 2) % 3
+            exp = [(int(ts.ind[mode][p]), int(ts.ind[m1][p]), int(ts.ind[m2][p]), ts.vals[p])
+                   for p in range(n)]
+            assert got == exp
+            # dcounter: last nonzero index of each slot
+            rows = ts.ind[mode]
+            last = np.flatnonzero(np.r_[rows[1:] != rows[:-1], True])
+            assert list(dc[1:]) == list(last)
+
+
+def test_mode_order_is_lexicographic():
+    rng = np.random.default_rng(9)
+    t = rand_coo(rng, (6, 5, 4), 60)
+    for mode in range(3):
+        perm, rowptr = O.mode_order(t, mode)
+        keys = [tuple(int(t.ind[(mode + s) % 3][p]) for s in range(3)) for p in perm]
+        assert keys == sorted(keys)
+        counts = np.bincount(t.ind[mode], minlength=t.dims[mode])
+        assert np.array_equal(np.diff(rowptr), counts)
+
+
+# --------------------------------------------------------------------------
+# MTTKRP / Gram vs dense numpy
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nmodes", [3, 4])
+def test_mttkrp_matches_dense_einsum(nmodes):
+    rng = np.random.default_rng(11)
+    dims = (5, 4, 6, 3)[:nmodes]
+    t = rand_coo(rng, dims, 40)
+    F = rand_factors(rng, dims, 7)
+    X, _ = dense(t)
+    letters = "ijkl"[:nmodes]
+    for mode in range(nmodes):
+        others = [m for m in range(nmodes) if m != mode]
+        spec = letters + "," + ",".join(letters[m] + "r" for m in others) + "->" + letters[mode] + "r"
+        exp = np.einsum(spec, X, *[F[m] for m in others])
+        got = O.mttkrp(t, mode, F)
+        np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12)
+
+
+def test_gram_rows_matches_dense():
+    rng = np.random.default_rng(12)
+    dims = (4, 6, 5)
+    t = rand_coo(rng, dims, 50)
+    F = rand_factors(rng, dims, 5)
+    _, M = dense(t)
+    G = O.gram_rows(t, 1, F)
+    for j in range(dims[1]):
+        H = np.einsum("ik,ir,kr->ikr", M[:, j, :], F[0], F[2]).reshape(-1, 5)
+        np.testing.assert_allclose(G[j], H.T @ H, rtol=1e-12, atol=1e-12)
+        w = np.linalg.eigvalsh(G[j])
+        assert w.min() > -1e-10  # symmetric PSD
+        np.testing.assert_array_equal(G[j], G[j].T)
+
+
+# --------------------------------------------------------------------------
+# dense Cholesky vs LAPACK (scipy)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 16, 33])
+def test_cholesky_matches_lapack(n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    A = A @ A.T + n * np.eye(n)
+    L = O.cholesky(A)
+    Ls = scipy.linalg.cholesky(A, lower=True)
+    np.testing.assert_allclose(L, Ls, rtol=1e-12, atol=1e-12)
+    b = rng.normal(size=n)
+    x = O.cholesky_solve(L, b)
+    np.testing.assert_allclose(x, scipy.linalg.cho_solve((Ls, True), b), rtol=1e-10, atol=1e-12)
+
+
+def test_cholesky_rejects_non_spd():
+    A = np.array([[1.0, 2.0], [2.0, 1.0]])
+    with pytest.raises(np.linalg.LinAlgError):
+        O.cholesky(A)
+
+
+# --------------------------------------------------------------------------
+# ALS
+# --------------------------------------------------------------------------
+
+def test_als_mode_equals_per_row_ridge_lstsq():
+    rng = np.random.default_rng(21)
+    dims = (6, 7, 5)
+    t = rand_coo(rng, dims, 90)
+    R, lam = 4, 0.3
+    F = rand_factors(rng, dims, R)
+    X, M = dense(t)
+    for mode in range(3):
+        F0 = [f.copy() for f in F]
+        assert O.als_mode(t, mode, F, lam) == 0
+        others = [m for m in range(3) if m != mode]
+        for i in range(dims[mode]):
+            sel = np.flatnonzero(t.ind[mode] == i)
+            if sel.size == 0:
+                assert np.all(F[mode][i] == 0)
+                continue
+            H = F0[others[0]][t.ind[others[0]][sel]] * F0[others[1]][t.ind[others[1]][sel]]
+            A = np.vstack([H, math.sqrt(lam) * np.eye(R)])
+            b = np.r_[t.vals[sel], np.zeros(R)]
+            x = np.linalg.lstsq(A, b, rcond=None)[0]
+            np.testing.assert_allclose(F[mode][i], x, rtol=1e-9, atol=1e-10)
+        F = [f.copy() for f in F]
+
+
+def test_als_recovers_exact_rank_one():
+    rng = np.random.default_rng(4)
+    dims = (5, 4, 3)
+    a, b, c = (rng.uniform(0.5, 1.5, d) for d in dims)
+    X = np.einsum("i,j,k->ijk", a, b, c)
+    idx = np.nonzero(np.ones(dims))
+    t = O.Coo(list(idx), X[idx], dims)
+    F = [rng.uniform(0.1, 1, (d, 1)) for d in dims]
+    O.als(t, F, 1e-12, 1)
+    P = np.einsum("ir,jr,kr->ijk", *F)
+    np.testing.assert_allclose(P, X, rtol=1e-8)
+
+
+# --------------------------------------------------------------------------
+# SGD
+# --------------------------------------------------------------------------
+
+def test_sgd_single_nonzero_closed_form():
+    R = 3
+    t = O.Coo([[1], [0], [2]], [2.0], (2, 1, 3))
+    rng = np.random.default_rng(2)
+    F = rand_factors(rng, (2, 1, 3), R)
+    a, b, c = F[0][1].copy(), F[1][0].copy(), F[2][2].copy()
+    lr, lam, s = 0.1, 0.01, 0.5
+    e = 2.0 - np.sum(a * b * c)
+    O.sgd_epoch(t, F, lr, lam, s)
+    np.testing.assert_allclose(F[0][1], a + lr * s * (e * b * c - lam * a), rtol=1e-14)
+    np.testing.assert_allclose(F[1][0], b + lr * s * (e * a * c - lam * b), rtol=1e-14)
+    np.testing.assert_allclose(F[2][2], c + lr * s * (e * a * b - lam * c), rtol=1e-14)
+    np.testing.assert_array_equal(F[0][0], rand_factors(np.random.default_rng(2), (2, 1, 3), R)[0][0])
+
+
+def test_sgd_two_nonzeros_are_sequential():
+    """The second update sees the first one's result on a shared row."""
+    R = 2
+    t = O.Coo([[0, 0], [0, 1], [0, 0]], [1.0, -1.0], (1, 2, 1))
+    F = [np.array([[0.5, -0.2]]), np.array([[0.3, 0.1], [0.7, 0.4]]), np.array([[0.9, 0.6]])]
+    G = [f.copy() for f in F]
+    lr, lam = 0.2, 0.05
+    O.sgd_epoch(t, F, lr, lam, 1.0)
+    for p, j in ((0, 0), (1, 1)):
+        a, b, c = G[0][0].copy(), G[1][j].copy(), G[2][0].copy()
+        e = t.vals[p] - np.sum(a * b * c)
+        G[0][0] = a + lr * (e * b * c - lam * a)
+        G[1][j] = b + lr * (e * a * c - lam * b)
+        G[2][0] = c + lr * (e * a * b - lam * c)
+    for f, g in zip(F, G):
+        np.testing.assert_allclose(f, g, rtol=1e-14)
+
+
+def test_sgd_four_way_step():
+    t = O.Coo([[0], [0], [0], [0]], [1.0], (1, 1, 1, 1))
+    F = [np.array([[0.5, 0.2]]), np.array([[0.3, -0.1]]), np.array([[0.9, 0.4]]),
+         np.array([[0.2, 0.8]])]
+    a, b, c, d = (f[0].copy() for f in F)
+    e = 1.0 - np.sum(a * b * c * d)
+    O.sgd_epoch(t, F, 0.1, 0.0, 1.0)
+    np.testing.assert_allclose(F[3][0], d + 0.1 * e * a * b * c, rtol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# CCD++
+# --------------------------------------------------------------------------
+
+def _objective(t, F, lam):
+    return 0.5 * O.loss_sq(t, F) + 0.5 * O.frob_sq(t.dims, F, lam)
+
+
+def test_ccd_last_column_is_stationary_and_objective_decreases():
+    rng = np.random.default_rng(31)
+    dims = (8, 6, 7)
+    t = rand_coo(rng, dims, 200)
+    R, lam = 3, 0.2
+    F = rand_factors(rng, dims, R)
+    res = O.residual(t, F)
+    objs = [_objective(t, F, lam)]
+    orders = [O.mode_order(t, m) for m in range(3)]
+    for _ in range(4):
+        O.ccd_epoch(t, F, lam, res, orders=orders)
+        objs.append(_objective(t, F, lam))
+        # the residual array tracks x - prediction
+        np.testing.assert_allclose(res, O.residual(t, F), rtol=1e-9, atol=1e-9)
+    assert all(b <= a + 1e-9 for a, b in zip(objs, objs[1:])), objs
+    # the last coordinate block updated (mode 2, column R-1) minimises the
+    # objective: its partial gradient is zero.
+    _, G = O.grad(t, F, lam)
+    assert np.max(np.abs(G[2][:, R - 1])) < 1e-9
+
+
+def test_ccd_one_column_single_mode_is_ridge():
+    """R=1, mode 0 update equals the 1-D ridge closed form from the add-back
+    residual (which is x itself for a one-column model)."""
+    rng = np.random.default_rng(8)
+    dims = (4, 5, 3)
+    t = rand_coo(rng, dims, 30)
+    F = rand_factors(rng, dims, 1)
+    F0 = [f.copy() for f in F]
+    lam = 0.7
+    res = O.residual(t, F)
+    O.ccd_epoch(t, F, lam, res)
+    for i in range(dims[0]):
+        sel = np.flatnonzero(t.ind[0] == i)
+        h = F0[1][t.ind[1][sel], 0] * F0[2][t.ind[2][sel], 0]
+        exp = np.sum(t.vals[sel] * h) / (lam + np.sum(h * h))
+        assert F[0][i, 0] == pytest.approx(exp, rel=1e-12, abs=1e-14)
+
+
+# --------------------------------------------------------------------------
+# gradient (adjoint) vs central finite differences
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("per_occ", [False, True])
+@pytest.mark.parametrize("nmodes", [3, 4])
+def test_grad_matches_finite_differences(per_occ, nmodes):
+    rng = np.random.default_rng(40 + nmodes)
+    dims = (5, 4, 6, 3)[:nmodes]
+    t = rand_coo(rng, dims, 45)
+    R, lam, eps = 3, 0.3, 1e-6
+    F = rand_factors(rng, dims, R)
+    loss, G = O.grad(t, F, lam, per_occ)
+
+    def f(FF):
+        return O.grad(t, FF, lam, per_occ)[0]
+
+    for m in range(nmodes):
+        for _ in range(6):
+            i, r = rng.integers(dims[m]), rng.integers(R)
+            Fp = [x.copy() for x in F]
+            Fm = [x.copy() for x in F]
+            Fp[m][i, r] += eps
+            Fm[m][i, r] -= eps
+            fd = (f(Fp) - f(Fm)) / (2 * eps)
+            assert G[m][i, r] == pytest.approx(fd, rel=1e-6, abs=1e-7)
+
+
+def test_grad_per_occurrence_is_summed_sgd_direction():
+    """-grad with per-occurrence L2 equals the sum over nonzeros of the
+    reference SGD step direction computed at the same point."""
+    rng = np.random.default_rng(44)
+    dims = (4, 3, 5)
+    t = rand_coo(rng, dims, 25)
+    R, lam = 2, 0.1
+    F = rand_factors(rng, dims, R)
+    _, G = O.grad(t, F, lam, True)
+    acc = [np.zeros_like(f) for f in F]
+    for p in range(t.nnz):
+        rows = [F[m][t.ind[m][p]] for m in range(3)]
+        e = t.vals[p] - np.sum(rows[0] * rows[1] * rows[2])
+        for m in range(3):
+            o = [rows[q] for q in range(3) if q != m]
+            acc[m][t.ind[m][p]] += e * o[0] * o[1] - lam * rows[m]
+    for m in range(3):
+        np.testing.assert_allclose(-G[m], acc[m], rtol=1e-12, atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# loss / RMSE / convergence
+# --------------------------------------------------------------------------
+
+def test_loss_rmse_closed_form():
+    t = O.Coo([[0, 1], [0, 0], [0, 1]], [3.0, 1.0], (2, 1, 2))
+    F = [np.array([[1.0], [2.0]]), np.array([[1.0]]), np.array([[1.0], [0.25]])]
+    # predictions: 1*1*1 = 1 and 2*1*0.25 = 0.5 -> errors 2 and 0.5
+    assert O.loss_sq(t, F) == pytest.approx(4.25)
+    assert O.rmse(t, F) == pytest.approx(math.sqrt(4.25 / 2))
+    assert O.frob_sq(t.dims, F, 0.5) == pytest.approx(0.5 * (5.0 + 1.0 + 1.0625))
+
+
+def test_converge_sequence():
+    c = O.Converge(tolerance=1e-4, max_badepochs=3)
+    seq = [1.0, 0.9, 0.89995, 0.8, 0.80001, 0.79999, 0.81]
+    out = [c.step(v, e) for e, v in enumerate(seq)]
+    assert [o[1] for o in out] == [True, True, False, True, False, False, False]
+    assert [o[0] for o in out] == [False, False, False, False, False, False, True]
+    assert c.s.bestepoch == 3 and c.s.best_rmse == 0.8
+
+
+def test_f32_twin_tracks_f64():
+    rng = np.random.default_rng(50)
+    dims = (20, 15, 10)
+    t = rand_coo(rng, dims, 600)
+    F64 = rand_factors(rng, dims, 4)
+    F32 = [f.astype(np.float32) for f in F64]
+    O.als(t, F64, 0.5, 2)
+    O.als(t, F32, 0.5, 2)
+    for a, b in zip(F64, F32):
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-4
