@@ -1,0 +1,157 @@
+// ctc_internal.h — shared host-side declarations of libctc (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ctc.h"
+
+namespace ctc {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char *fmt, ...);
+ctc_status fail(ctc_status s, const char *fmt, ...);
+
+#define CTC_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::ctc::fail(CTC_ERR_HIP, "%s failed: %s (%s:%d)", #call,                 \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+#define CTC_TRY(call)                                                                 \
+  do {                                                                                \
+    ctc_status s_ = (call);                                                           \
+    if (s_ != CTC_OK) return s_;                                                      \
+  } while (0)
+
+// ------------------------------------------------------------- host utils --
+unsigned host_threads();
+
+// Stable counting sort of `order` by key[order[p]] in [0, dim).
+void counting_sort(const uint32_t *key, uint64_t dim, std::vector<uint32_t> &order,
+                   std::vector<uint32_t> &tmp);
+
+// ------------------------------------------------------------ host CISS --
+struct HostCiss {
+  uint32_t nmodes = 0, mode = 0;
+  uint64_t nnz = 0, ntiles = 0, dlength = 0;
+  uint64_t dims[CTC_MAX_MODES] = {0, 0, 0, 0};
+  std::vector<uint32_t> directory, rowptr, perm, tile_slot;
+  std::vector<uint32_t> oidx[CTC_MAX_MODES - 1];
+  std::vector<float> vals;
+  std::vector<uint64_t> tile_mask;
+};
+ctc_status build_host_ciss(const ctc_coo *t, uint32_t mode, HostCiss &out);
+
+// ----------------------------------------------------------- device CISS --
+// Device copy of one mode's CISS plus the per-span piece tables used by the
+// deterministic segmented reductions (see kernels.hip).
+struct DevCiss {
+  uint32_t nmodes = 0, mode = 0;
+  uint64_t nnz = 0, ntiles = 0, dlength = 0;
+  uint32_t *oidx[CTC_MAX_MODES - 1] = {nullptr, nullptr, nullptr};
+  float *vals = nullptr;
+  uint32_t *tile_slot = nullptr;
+  uint64_t *tile_mask = nullptr;
+  uint32_t *directory = nullptr;
+  uint32_t *rowptr = nullptr;
+  // piece tables for span_tiles tiles per span
+  uint32_t span_tiles = 0;
+  uint64_t nspans = 0, npieces = 0;
+  uint32_t *span_off = nullptr;  // nspans: piece id = slot + span_off[span]
+  uint32_t *slot_pb = nullptr;   // dlength + 1: pieces of slot s
+  // host copies needed by the piece-table builder
+  std::vector<uint32_t> h_tile_slot;
+  std::vector<uint64_t> h_tile_mask;
+  std::vector<uint32_t> h_rowptr;
+};
+
+// Device COO used for evaluation (validation / test).
+struct DevCoo {
+  uint32_t nmodes = 0;
+  uint64_t nnz = 0;
+  uint32_t *ind[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  float *vals = nullptr;
+};
+
+}  // namespace ctc
+
+// ------------------------------------------------------- opaque handles --
+struct ctc_ciss {
+  ctc::HostCiss h;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct ctc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool profiling = false;
+  std::map<std::string, KernelStat> stats;
+  // pending profiled launches: name, start, stop
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  hipDeviceProp_t props{};
+};
+
+struct ctc_model {
+  ctc_context *ctx = nullptr;
+  uint32_t nmodes = 0, rank = 0;
+  uint64_t dims[CTC_MAX_MODES] = {0, 0, 0, 0};
+  uint64_t train_nnz = 0;
+  // host copy of the training tensor (CISS copies are built from it on demand)
+  std::unique_ptr<ctc_coo, void (*)(ctc_coo *)> train{nullptr, ctc_coo_free};
+  ctc::DevCiss ciss[CTC_MAX_MODES];
+  bool ciss_ready[CTC_MAX_MODES] = {false, false, false, false};
+  ctc::DevCoo valid, test;
+  float *F[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  float *best[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  // scratch
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  double *dstat = nullptr;  // small device buffer for reductions (256 doubles)
+  double *hstat = nullptr;  // pinned host mirror
+  uint32_t *dflag = nullptr;  // device status flags (not-SPD etc.)
+  // CCD++ residuals per mode (in that mode's CISS order)
+  float *resid[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  bool resid_valid = false;
+  float *colsave = nullptr;  // CCD++ saved columns
+  size_t colsave_elems = 0;
+  float *FT[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};  // column-major mirrors
+  int ccd_pending = -1;  // column whose new component the residual copies still hold
+};
+
+namespace ctc {
+ctc_status ensure_scratch(ctc_model *m, size_t bytes);
+ctc_status ensure_ciss(ctc_model *m, uint32_t mode);
+ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles);
+// one CCD++ epoch on the device (ccd.cpp)
+ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o);
+// profiled launch bracketing
+void prof_begin(ctc_context *c, const char *name, hipEvent_t *a, hipEvent_t *b);
+void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b);
+ctc_status prof_flush(ctc_context *c);
+// RAII bracket of one profiled launch (no-op unless profiling is enabled)
+struct Prof {
+  ctc_context *c;
+  const char *name;
+  hipEvent_t a, b;
+  Prof(ctc_context *c_, const char *n) : c(c_), name(n) { prof_begin(c, n, &a, &b); }
+  ~Prof() { prof_end(c, name, a, b); }
+};
+}  // namespace ctc

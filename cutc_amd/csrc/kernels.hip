@@ -1,0 +1,828 @@
+// kernels.hip — gfx950 kernels of libctc: CISS tile walks for SGD, ALS
+// (MFMA Gram + MTTKRP, FP64 Cholesky solve), MTTKRP, the loss adjoint and the
+// RMSE reductions.
+//
+// Execution model shared by the walks ("walkers"):
+//   * The nonzeros of a mode's CISS are cut into spans of whole 64-nonzero
+//     tiles.  A walker owns one span and visits its nonzeros in order.
+//   * A walker is W = min(R, 64) lanes; lane w holds columns w, w+W, ... of
+//     every factor row it touches, so each row access is one contiguous
+//     W*4-byte load or atomic (coalesced), and a dot product is a W-lane DPP /
+//     swizzle reduction.  64/W walkers share a wave.
+//   * A wave loads W nonzeros at a time (indices, value, row-start bit from the
+//     tile mask) with one coalesced load per stream, then hands nonzero q to
+//     its walker lanes with ds_bpermute; the row starts come from one ballot.
+//   * Row (directory-slot) changes are found from the tile mask; the slot of a
+//     walker's first nonzero is the tile header's first slot.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels_common.h"
+#include "kernels_launch.h"
+
+namespace ctc {
+
+// ==========================================================================
+// SGD epoch (reference rule SG/sgd.cu:75-99: every row of a nonzero is
+// updated from its pre-update values; step = lr * scale).
+// ==========================================================================
+template <int R, int N>
+__global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, uint32_t span,
+                                             float step, float lam) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int WPW = 64 / W;
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  const int lane = lane_id();
+  const int sub = lane / W;
+  const int wl = lane % W;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t walker = wave * WPW + sub;
+  const uint64_t s0 = walker * span;
+  const uint64_t s1 = s0 + span < c.nnz ? s0 + span : c.nnz;
+  if (wave * WPW * (uint64_t)span >= c.nnz) return;  // whole wave idle (uniform)
+
+  float *A = fa.f[0];
+  float *Bf[N - 1];
+#pragma unroll
+  for (int s = 0; s < N - 1; ++s) Bf[s] = fa.f[1 + s];
+
+  float a[CPL], adel[CPL];
+  float b[N - 1][CPL], bdel[N - 1][CPL];
+  uint32_t cur[N - 1];
+  uint32_t arow = NONE, slot = 0;
+#pragma unroll
+  for (int s = 0; s < N - 1; ++s) {
+    cur[s] = NONE;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) bdel[s][q] = 0.f, b[s][q] = 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) a[q] = 0.f, adel[q] = 0.f;
+
+  for (uint32_t ch = 0; ch < span; ch += W) {
+    const uint64_t el = s0 + ch + wl;
+    const bool vl = el < s1;
+    uint32_t il[N - 1];
+    float xl = 0.f;
+    bool hl = false;
+    if (vl) {
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) il[s] = c.oidx[s][el];
+      xl = c.vals[el];
+      hl = (c.tile_mask[el >> 6] >> (el & 63)) & 1ull;
+    } else {
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) il[s] = 0;
+    }
+    const uint64_t hm = __ballot(vl && hl);
+    for (int q = 0; q < W; ++q) {
+      const int src = sub * W + q;
+      uint32_t idx[N - 1];
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) idx[s] = (uint32_t)__shfl((int)il[s], src, 64);
+      const float x = __shfl(xl, src, 64);
+      const uint64_t e = s0 + ch + q;
+      if (e < s1) {
+        const bool head = (hm >> src) & 1ull;
+        if (arow == NONE || head) {
+          if (arow != NONE) {
+#pragma unroll
+            for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(A + (size_t)arow * R + wl + q2 * W, adel[q2]);
+            ++slot;
+          } else {
+            slot = c.tile_slot[s0 >> 6];
+          }
+          arow = c.directory[slot];
+#pragma unroll
+          for (int q2 = 0; q2 < CPL; ++q2) {
+            a[q2] = A[(size_t)arow * R + wl + q2 * W];
+            adel[q2] = 0.f;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) {
+          if (idx[s] != cur[s]) {
+            if (cur[s] != NONE) {
+#pragma unroll
+              for (int q2 = 0; q2 < CPL; ++q2)
+                atomic_add_f32(Bf[s] + (size_t)cur[s] * R + wl + q2 * W, bdel[s][q2]);
+            }
+            cur[s] = idx[s];
+#pragma unroll
+            for (int q2 = 0; q2 < CPL; ++q2) {
+              b[s][q2] = Bf[s][(size_t)idx[s] * R + wl + q2 * W];
+              bdel[s][q2] = 0.f;
+            }
+          }
+        }
+        // prediction and error
+        float part = 0.f;
+#pragma unroll
+        for (int q2 = 0; q2 < CPL; ++q2) {
+          float v = a[q2];
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) v *= b[s][q2];
+          part += v;
+        }
+        const float err = x - group_sum<W>(part);
+#pragma unroll
+        for (int q2 = 0; q2 < CPL; ++q2) {
+          // products of all rows but one, from pre-update values
+          float prod_b = 1.f;
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) prod_b *= b[s][q2];
+          float oth[N - 1];  // a * product of the other non-own rows
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) {
+            float o = a[q2];
+#pragma unroll
+            for (int s2 = 0; s2 < N - 1; ++s2)
+              if (s2 != s) o *= b[s2][q2];
+            oth[s] = o;
+          }
+          const float da = step * (err * prod_b - lam * a[q2]);
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) {
+            const float db = step * (err * oth[s] - lam * b[s][q2]);
+            b[s][q2] += db;
+            bdel[s][q2] += db;
+          }
+          a[q2] += da;
+          adel[q2] += da;
+        }
+      }
+    }
+  }
+  if (arow != NONE) {
+#pragma unroll
+    for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(A + (size_t)arow * R + wl + q2 * W, adel[q2]);
+  }
+#pragma unroll
+  for (int s = 0; s < N - 1; ++s)
+    if (cur[s] != NONE) {
+#pragma unroll
+      for (int q2 = 0; q2 < CPL; ++q2)
+        atomic_add_f32(Bf[s] + (size_t)cur[s] * R + wl + q2 * W, bdel[s][q2]);
+    }
+}
+
+// ==========================================================================
+// MTTKRP / adjoint walk: per row, acc = sum over its nonzeros of w * h with
+// h = hadamard of the other modes' rows and
+//   w = x                       (MTTKRP, SG/als.cu:295-312 intent), or
+//   w = x - <F_mode(row), h>    (loss adjoint: residual-weighted MTTKRP).
+// Rows may span walkers, so each walker adds its partial row with atomics
+// into a zeroed output.  For the adjoint, out receives -acc and loss_part
+// the per-block sum of squared residuals.
+// ==========================================================================
+template <int R, int N, bool ADJ>
+__global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint32_t span,
+                                                float *out, double *loss_part) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int WPW = 64 / W;
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  __shared__ double red[4];
+  const int lane = lane_id();
+  const int sub = lane / W;
+  const int wl = lane % W;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t walker = wave * WPW + sub;
+  const uint64_t s0 = walker * span;
+  const uint64_t s1 = s0 + span < c.nnz ? s0 + span : c.nnz;
+  const bool wave_idle = wave * WPW * (uint64_t)span >= c.nnz;
+
+  float acc[CPL], own[CPL];
+  double lsum = 0.0;
+  uint32_t arow = NONE, slot = 0;
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) acc[q] = 0.f, own[q] = 0.f;
+
+  if (!wave_idle) {
+    for (uint32_t ch = 0; ch < span; ch += W) {
+      const uint64_t el = s0 + ch + wl;
+      const bool vl = el < s1;
+      uint32_t il[N - 1];
+      float xl = 0.f;
+      bool hl = false;
+      if (vl) {
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) il[s] = c.oidx[s][el];
+        xl = c.vals[el];
+        hl = (c.tile_mask[el >> 6] >> (el & 63)) & 1ull;
+      } else {
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) il[s] = 0;
+      }
+      const uint64_t hm = __ballot(vl && hl);
+      for (int q = 0; q < W; ++q) {
+        const int src = sub * W + q;
+        uint32_t idx[N - 1];
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) idx[s] = (uint32_t)__shfl((int)il[s], src, 64);
+        const float x = __shfl(xl, src, 64);
+        const uint64_t e = s0 + ch + q;
+        if (e < s1) {
+          const bool head = (hm >> src) & 1ull;
+          if (arow == NONE || head) {
+            if (arow != NONE) {
+#pragma unroll
+              for (int q2 = 0; q2 < CPL; ++q2)
+                atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, ADJ ? -acc[q2] : acc[q2]);
+              ++slot;
+            } else {
+              slot = c.tile_slot[s0 >> 6];
+            }
+            arow = c.directory[slot];
+#pragma unroll
+            for (int q2 = 0; q2 < CPL; ++q2) {
+              acc[q2] = 0.f;
+              if (ADJ) own[q2] = fa.f[0][(size_t)arow * R + wl + q2 * W];
+            }
+          }
+          float h[CPL];
+#pragma unroll
+          for (int q2 = 0; q2 < CPL; ++q2) {
+            float v = fa.f[1][(size_t)idx[0] * R + wl + q2 * W];
+#pragma unroll
+            for (int s = 1; s < N - 1; ++s) v *= fa.f[1 + s][(size_t)idx[s] * R + wl + q2 * W];
+            h[q2] = v;
+          }
+          float w = x;
+          if (ADJ) {
+            float part = 0.f;
+#pragma unroll
+            for (int q2 = 0; q2 < CPL; ++q2) part += own[q2] * h[q2];
+            w = x - group_sum<W>(part);
+            if (wl == 0) lsum += (double)w * (double)w;
+          }
+#pragma unroll
+          for (int q2 = 0; q2 < CPL; ++q2) acc[q2] += w * h[q2];
+        }
+      }
+    }
+    if (arow != NONE) {
+#pragma unroll
+      for (int q2 = 0; q2 < CPL; ++q2)
+        atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, ADJ ? -acc[q2] : acc[q2]);
+    }
+  }
+  if (ADJ) {
+    double t = wave_sum_d(lsum);
+    if (lane == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w2 = 0; w2 < (int)(blockDim.x >> 6); ++w2) s += red[w2];
+      loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+// ==========================================================================
+// ALS, phase 1: Gram + MTTKRP walk on the FP64 matrix cores.
+//
+// One wave per span.  Per row segment the wave accumulates
+//   G += h h^T  with v_mfma_f64_16x16x4_f64 (h = hadamard of the other
+//               modes' rows, formed exactly in FP64 from the FP32 factors)
+//   m += x h    in FP64 on the VALUs
+// and writes (G, m) as an FP64 "piece" when the row ends or the span ends.
+// Piece ids are slot + span_off[span], so the pieces of a slot are contiguous
+// and written exactly once: no atomics, fixed summation order
+// (deterministic).  Lane l holds column (l % 16) + 16 q of nonzero l / 16 of
+// the current 4-nonzero step, which is exactly the A (16 x 4) and B (4 x 16)
+// operand layout of the 16x16x4 shape, so both operands are the same h
+// registers; block (bi, bj) of G (bi >= bj) takes A = h[bi], B = h[bj].
+// FP64 accumulation keeps the normal equations within 1e-5 of an FP64
+// reference even when (G + lambda I) is poorly conditioned.
+// ==========================================================================
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <int R, int N>
+__global__ void __launch_bounds__(256) k_als_gram(CissArgs c, FactorArgs fa, uint32_t span_tiles,
+                                                  uint64_t nspans, const uint32_t *span_off,
+                                                  double *pieces) {
+  constexpr int CG = (R + 15) / 16;      // 16-column groups
+  constexpr int NB = CG * (CG + 1) / 2;  // lower-triangle 16x16 blocks
+  constexpr int NPS = 4;                 // nonzeros per MFMA step
+  constexpr size_t PSZ = (size_t)R * R + R;
+  const int lane = lane_id();
+  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nspans) return;
+  const uint64_t t0 = w * span_tiles;
+  const uint64_t t1 = t0 + span_tiles < c.ntiles ? t0 + span_tiles : c.ntiles;
+  const uint32_t poff = span_off[w];
+  uint32_t slot = c.tile_slot[t0];
+  const int col = lane & 15;
+  const int nsub = lane >> 4;
+
+  doublex4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = doublex4{0.0, 0.0, 0.0, 0.0};
+  double macc[CG];
+#pragma unroll
+  for (int q = 0; q < CG; ++q) macc[q] = 0.0;
+
+  auto flush = [&](uint32_t pid) {
+    double *P = pieces + (size_t)pid * PSZ;
+    int b = 0;
+#pragma unroll
+    for (int bi = 0; bi < CG; ++bi) {
+#pragma unroll
+      for (int bj = 0; bj <= bi; ++bj, ++b) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // f64 16x16x4 output: accumulator v of lane l is row 4 v + l / 16
+          const int row = 16 * bi + 4 * v + nsub, cc = 16 * bj + col;
+          if (row < R && cc < R) P[row * R + cc] = acc[b][v];
+          acc[b][v] = 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CG; ++q) {
+      double mt = macc[q];
+      mt += __shfl_xor(mt, 16, 64);
+      mt += __shfl_xor(mt, 32, 64);
+      if (lane < 16 && 16 * q + lane < R) P[R * R + 16 * q + lane] = mt;
+      macc[q] = 0.0;
+    }
+  };
+
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint64_t base = t * kTile;
+    const uint32_t cnt = (uint32_t)((c.nnz - base) < (uint64_t)kTile ? (c.nnz - base) : kTile);
+    uint32_t jl = 0, kl = 0, ll = 0;
+    float xl = 0.f;
+    if ((uint32_t)lane < cnt) {
+      jl = c.oidx[0][base + lane];
+      kl = c.oidx[1][base + lane];
+      if (N == 4) ll = c.oidx[2][base + lane];
+      xl = c.vals[base + lane];
+    }
+    uint64_t mask = c.tile_mask[t];
+    if (cnt < 64) mask &= (1ull << cnt) - 1ull;
+    uint32_t pos = 0;
+    while (pos < cnt) {
+      if ((mask >> pos) & 1ull) {
+        if (!(t == t0 && pos == 0)) {
+          flush(slot + poff);
+          ++slot;
+        }
+      }
+      const uint64_t rest = (pos >= 63) ? 0ull : ((mask >> (pos + 1)) << (pos + 1));
+      uint32_t nxt = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1u : cnt;
+      if (nxt > cnt) nxt = cnt;
+      for (uint32_t e = pos; e < nxt; e += NPS) {
+        const uint32_t my = e + (uint32_t)nsub;
+        const bool valid = my < nxt;
+        const int src = (int)(my < 64 ? my : 63);
+        const uint32_t j = (uint32_t)__shfl((int)jl, src, 64);
+        const uint32_t k = (uint32_t)__shfl((int)kl, src, 64);
+        const double x = (double)__shfl(xl, src, 64);
+        uint32_t l = 0;
+        if (N == 4) l = (uint32_t)__shfl((int)ll, src, 64);
+        double h[CG];
+#pragma unroll
+        for (int q = 0; q < CG; ++q) {
+          const int cq = col + 16 * q;
+          double v = 0.0;
+          if (valid && cq < R) {
+            v = (double)fa.f[1][(size_t)j * R + cq] * (double)fa.f[2][(size_t)k * R + cq];
+            if (N == 4) v *= (double)fa.f[3][(size_t)l * R + cq];
+          }
+          h[q] = v;
+          macc[q] += x * v;
+        }
+        int b = 0;
+#pragma unroll
+        for (int bi = 0; bi < CG; ++bi)
+#pragma unroll
+          for (int bj = 0; bj <= bi; ++bj, ++b)
+            acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(h[bi], h[bj], acc[b], 0, 0, 0);
+      }
+      pos = nxt;
+    }
+  }
+  flush(slot + poff);
+}
+
+// ==========================================================================
+// ALS, phase 2: per directory slot, sum its pieces in FP64, add lambda to the
+// diagonal (p_update_matrice, SG/als.cu:484-487), factor the R x R system by
+// Cholesky (right-looking, Golub & Van Loan 4.2) in FP64 in LDS, solve, and
+// optionally refine once: x += (G+lambda I)^-1 (m - (G+lambda I) x).
+// One 64-thread workgroup per slot.  Replaces cusolverDnDpotrfBatched +
+// DpotrsBatched (SG/als.cu:715-716).
+// ==========================================================================
+__device__ inline int tri_row(int t) {
+  int a = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  if ((a + 1) * (a + 2) / 2 <= t) ++a;
+  if (a * (a + 1) / 2 > t) --a;
+  return a;
+}
+
+template <int R, bool REFINE>
+__global__ void __launch_bounds__(64) k_als_solve(const double *pieces, const uint32_t *slot_pb,
+                                                  const uint32_t *directory, uint64_t dlength,
+                                                  double lam, float *A, uint32_t *flag) {
+  constexpr int LD = R + 1;
+  constexpr size_t PSZ = (size_t)R * R + R;
+  constexpr int PER = (R * R + 63) / 64;
+  __shared__ double Gs[R * LD];
+  __shared__ double G0[REFINE ? R * LD : 1];
+  __shared__ double ys[R], rs[R];
+  const int lane = threadIdx.x;
+  const uint64_t s = blockIdx.x;
+  if (s >= dlength) return;
+  const uint32_t p0 = slot_pb[s], p1 = slot_pb[s + 1];
+
+  double g[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) g[t] = 0.0;
+  double mv = 0.0;
+  for (uint32_t p = p0; p < p1; ++p) {
+    const double *P = pieces + (size_t)p * PSZ;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int idx = lane + 64 * t;
+      if (idx < R * R) {
+        const int i = idx / R, j = idx % R;
+        const int hi = i > j ? i : j, lo = i > j ? j : i;
+        g[t] += P[hi * R + lo];
+      }
+    }
+    if (lane < R) mv += P[R * R + lane];
+  }
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    const int idx = lane + 64 * t;
+    if (idx < R * R) {
+      const int i = idx / R, j = idx % R;
+      const double v = g[t] + (i == j ? lam : 0.0);
+      Gs[i * LD + j] = v;
+      if (REFINE) G0[i * LD + j] = v;
+    }
+  }
+  if (lane < R) {
+    ys[lane] = mv;
+    rs[lane] = mv;
+  }
+  __syncthreads();
+
+  bool bad = false;
+  for (int k = 0; k < R; ++k) {
+    double d = Gs[k * LD + k];
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
+    }
+    const double lkk = sqrt(d), inv = 1.0 / lkk;
+    __syncthreads();
+    if (lane == 0) Gs[k * LD + k] = lkk;
+    for (int i = k + 1 + lane; i < R; i += 64) Gs[i * LD + k] *= inv;
+    __syncthreads();
+    const int n = R - k - 1;
+    const int T = n * (n + 1) / 2;
+    for (int t = lane; t < T; t += 64) {
+      const int a = tri_row(t), b = t - a * (a + 1) / 2;
+      const int i = k + 1 + a, j = k + 1 + b;
+      Gs[i * LD + j] -= Gs[i * LD + k] * Gs[j * LD + k];
+    }
+    __syncthreads();
+  }
+
+  auto solve = [&](double *y) {
+    for (int k = 0; k < R; ++k) {  // L y = b
+      const double yk = y[k] / Gs[k * LD + k];
+      __syncthreads();
+      if (lane == 0) y[k] = yk;
+      for (int i = k + 1 + lane; i < R; i += 64) y[i] -= Gs[i * LD + k] * yk;
+      __syncthreads();
+    }
+    for (int k = R - 1; k >= 0; --k) {  // L^T x = y
+      const double xk = y[k] / Gs[k * LD + k];
+      __syncthreads();
+      if (lane == 0) y[k] = xk;
+      for (int i = lane; i < k; i += 64) y[i] -= Gs[k * LD + i] * xk;
+      __syncthreads();
+    }
+  };
+  solve(ys);
+  if (REFINE) {
+    if (lane < R) {
+      double r = rs[lane];
+      for (int j = 0; j < R; ++j) {
+        const int hi = lane > j ? lane : j, lo = lane > j ? j : lane;
+        r -= G0[hi * LD + lo] * ys[j];
+      }
+      rs[lane] = r;
+    }
+    __syncthreads();
+    solve(rs);
+    if (lane < R) ys[lane] += rs[lane];
+    __syncthreads();
+  }
+  const uint32_t row = directory[s];
+  if (lane < R) A[(size_t)row * R + lane] = bad ? 0.f : (float)ys[lane];
+  if (bad && lane == 0) atomicOr(flag, 1u);
+}
+
+// ==========================================================================
+// Evaluation: sum of squared errors over a COO (validation / test) or over a
+// mode's CISS (training), FP64 accumulation, one partial per block (fixed
+// grid, fixed order -> deterministic).
+// ==========================================================================
+template <int N>
+__device__ inline float predict_row(const float *const *F, const uint32_t *ix, int R) {
+  const float *r0 = F[0] + (size_t)ix[0] * R;
+  const float *r1 = F[1] + (size_t)ix[1] * R;
+  const float *r2 = N > 2 ? F[2] + (size_t)ix[2] * R : nullptr;
+  const float *r3 = N > 3 ? F[3] + (size_t)ix[3] * R : nullptr;
+  float acc = 0.f;
+  for (int f = 0; f < R; f += 4) {
+    float4 a = *(const float4 *)(r0 + f);
+    float4 b = *(const float4 *)(r1 + f);
+    float4 v = make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
+    if (N > 2) {
+      float4 cc = *(const float4 *)(r2 + f);
+      v = make_float4(v.x * cc.x, v.y * cc.y, v.z * cc.z, v.w * cc.w);
+    }
+    if (N > 3) {
+      float4 d = *(const float4 *)(r3 + f);
+      v = make_float4(v.x * d.x, v.y * d.y, v.z * d.z, v.w * d.w);
+    }
+    acc += (v.x + v.y) + (v.z + v.w);
+  }
+  return acc;
+}
+
+__device__ inline void block_store_sum(double v, double *out) {
+  __shared__ double red[8];
+  v = wave_sum_d(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    *out = s;
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_eval_coo(CooArgs c, FactorArgs fa, int R, double *part) {
+  double acc = 0.0;
+  const float *F[4] = {fa.f[0], fa.f[1], fa.f[2], fa.f[3]};
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < c.nnz;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t ix[4];
+#pragma unroll
+    for (int m = 0; m < N; ++m) ix[m] = c.ind[m][p];
+    const double e = (double)c.vals[p] - (double)predict_row<N>(F, ix, R);
+    acc += e * e;
+  }
+  block_store_sum(acc, part + blockIdx.x);
+}
+
+// training loss over the CISS of `mode` (factor pointers in CISS-relative order)
+template <int N>
+__global__ void __launch_bounds__(256) k_eval_ciss(CissArgs c, FactorArgs fa, int R, double *part) {
+  double acc = 0.0;
+  const float *F[4] = {fa.f[0], fa.f[1], fa.f[2], fa.f[3]};
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < c.nnz;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = p >> 6;
+    const uint32_t slot = tile_lane_slot(c.tile_slot[t], c.tile_mask[t], (int)(p & 63));
+    uint32_t ix[4];
+    ix[0] = c.directory[slot];
+#pragma unroll
+    for (int s = 0; s < N - 1; ++s) ix[1 + s] = c.oidx[s][p];
+    const double e = (double)c.vals[p] - (double)predict_row<N>(F, ix, R);
+    acc += e * e;
+  }
+  block_store_sum(acc, part + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(const float *x, uint64_t n, double *part) {
+  double acc = 0.0;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const double v = x[p];
+    acc += v * v;
+  }
+  block_store_sum(acc, part + blockIdx.x);
+}
+
+// out[k] = sum of part[k * nb .. k * nb + nb) in a fixed order
+__global__ void __launch_bounds__(64) k_reduce_parts(const double *part, uint32_t nb, double *out) {
+  const double *p = part + (size_t)blockIdx.x * nb;
+  double acc = 0.0;
+  for (uint32_t i = threadIdx.x; i < nb; i += 64) acc += p[i];
+  acc = wave_sum_d(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+// ==========================================================================
+// elementwise helpers
+// ==========================================================================
+__global__ void __launch_bounds__(256) k_fill(float *x, uint64_t n, float v) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    x[p] = v;
+}
+
+// grad += lambda * c_i * F  for the rows of a CISS directory (c_i = row nnz)
+// or, when per_occurrence == 0, for every row (c_i = 1).
+__global__ void __launch_bounds__(256) k_reg_rows(const uint32_t *directory, const uint32_t *rowptr,
+                                                  uint64_t nrows, int R, float lam, int per_occ,
+                                                  const float *F, float *G, double *part) {
+  double acc = 0.0;
+  const uint64_t total = nrows * (uint64_t)R;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = p / R;
+    const int f = (int)(p % R);
+    uint64_t row = s;
+    float cnt = 1.f;
+    if (per_occ) {
+      row = directory[s];
+      cnt = (float)(rowptr[s + 1] - rowptr[s]);
+    }
+    const float a = F[row * R + f];
+    G[row * R + f] += lam * cnt * a;
+    acc += 0.5 * (double)lam * (double)cnt * (double)a * (double)a;
+  }
+  block_store_sum(acc, part + blockIdx.x);
+}
+
+// ==========================================================================
+// launchers
+// ==========================================================================
+#define CTC_R_SWITCH(R, BODY)                                       \
+  switch (R) {                                                      \
+    case 8: { constexpr int RR = 8; BODY; } break;                  \
+    case 16: { constexpr int RR = 16; BODY; } break;                \
+    case 32: { constexpr int RR = 32; BODY; } break;                \
+    case 64: { constexpr int RR = 64; BODY; } break;                \
+    case 128: { constexpr int RR = 128; BODY; } break;              \
+    default: return hipErrorInvalidValue;                           \
+  }
+
+static inline unsigned grid_for(uint64_t waves, unsigned block) {
+  const uint64_t wpb = block / 64;
+  return (unsigned)((waves + wpb - 1) / wpb);
+}
+
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                      uint32_t span, float step, float lam, unsigned block) {
+  if (c.nnz == 0) return hipSuccess;
+  const uint64_t walkers = (c.nnz + span - 1) / span;
+  const int W = R < 64 ? R : 64;
+  const uint64_t waves = (walkers + (64 / W) - 1) / (64 / W);
+  const unsigned grid = grid_for(waves, block);
+  if (nmodes == 3) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, 3>), dim3(grid), dim3(block), 0, st, c, f, span, step, lam));
+  } else if (nmodes == 4) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, 4>), dim3(grid), dim3(block), 0, st, c, f, span, step, lam));
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                         uint32_t span, bool adjoint, float *out, double *loss_part,
+                         unsigned *grid_out, unsigned block) {
+  const uint64_t walkers = (c.nnz + span - 1) / span;
+  const int W = R < 64 ? R : 64;
+  const uint64_t waves = (walkers + (64 / W) - 1) / (64 / W);
+  const unsigned grid = grid_for(waves, block);
+  *grid_out = grid;
+  if (c.nnz == 0) return hipSuccess;
+  if (nmodes == 3) {
+    if (adjoint) {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, true>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+    } else {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, false>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+    }
+  } else if (nmodes == 4) {
+    if (adjoint) {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, true>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+    } else {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, false>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block) {
+  const uint64_t walkers = (nnz + span - 1) / span;
+  const int W = R < 64 ? R : 64;
+  const uint64_t waves = (walkers + (64 / W) - 1) / (64 / W);
+  return grid_for(waves, block);
+}
+
+hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                           uint32_t span_tiles, uint64_t nspans, const uint32_t *span_off,
+                           double *pieces, unsigned block) {
+  if (c.nnz == 0) return hipSuccess;
+  const unsigned grid = grid_for(nspans, block);
+#define CTC_GRAM(RR)                                                                                     \
+  do {                                                                                                    \
+    if (nmodes == 3)                                                                                      \
+      hipLaunchKernelGGL((k_als_gram<RR, 3>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans, \
+                         span_off, pieces);                                                               \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_als_gram<RR, 4>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans, \
+                         span_off, pieces);                                                               \
+  } while (0)
+  switch (R) {
+    case 8: CTC_GRAM(8); break;
+    case 16: CTC_GRAM(16); break;
+    case 32: CTC_GRAM(32); break;
+    case 64: CTC_GRAM(64); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef CTC_GRAM
+  return hipGetLastError();
+}
+
+hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
+                            const uint32_t *directory, uint64_t dlength, int R, double lam, bool refine,
+                            float *A, uint32_t *flag) {
+  if (dlength == 0) return hipSuccess;
+  const unsigned grid = (unsigned)dlength;
+#define CTC_SOLVE(RR)                                                                                      \
+  do {                                                                                                      \
+    if (refine)                                                                                             \
+      hipLaunchKernelGGL((k_als_solve<RR, true>), dim3(grid), dim3(64), 0, st, pieces, slot_pb, directory, \
+                         dlength, lam, A, flag);                                                            \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_als_solve<RR, false>), dim3(grid), dim3(64), 0, st, pieces, slot_pb,           \
+                         directory, dlength, lam, A, flag);                                                 \
+  } while (0)
+  switch (R) {
+    case 8: CTC_SOLVE(8); break;
+    case 16: CTC_SOLVE(16); break;
+    case 32: CTC_SOLVE(32); break;
+    case 64: CTC_SOLVE(64); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef CTC_SOLVE
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f, int nmodes, int R,
+                           unsigned nblocks, double *part) {
+  if (nmodes == 3)
+    hipLaunchKernelGGL((k_eval_coo<3>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
+  else if (nmodes == 4)
+    hipLaunchKernelGGL((k_eval_coo<4>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                            unsigned nblocks, double *part) {
+  if (nmodes == 3)
+    hipLaunchKernelGGL((k_eval_ciss<3>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
+  else if (nmodes == 4)
+    hipLaunchKernelGGL((k_eval_ciss<4>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_sumsq(hipStream_t st, const float *x, uint64_t n, unsigned nblocks, double *part) {
+  hipLaunchKernelGGL(k_sumsq, dim3(nblocks), dim3(256), 0, st, x, n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_parts(hipStream_t st, const double *part, uint32_t nb, uint32_t groups,
+                               double *out) {
+  hipLaunchKernelGGL(k_reduce_parts, dim3(groups), dim3(64), 0, st, part, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t st, float *x, uint64_t n, float v) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, x, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_reg_rows(hipStream_t st, const uint32_t *directory, const uint32_t *rowptr,
+                           uint64_t nrows, int R, float lam, int per_occ, const float *F, float *G,
+                           unsigned nblocks, double *part) {
+  hipLaunchKernelGGL(k_reg_rows, dim3(nblocks), dim3(256), 0, st, directory, rowptr, nrows, R, lam,
+                     per_occ, F, G, part);
+  return hipGetLastError();
+}
+
+}  // namespace ctc

@@ -1,0 +1,253 @@
+// kernels_ccd.hip — CCD++ column kernels for gfx950 (canonical CCD++ as the
+// reference states it in comments, SG/ccd.cu:532-554 and NC/ccd.cu:28-180:
+// for each column f, add the rank-one component back to the residual, update
+// column f of every mode as a 1-D ridge solve a_if = sum(r h) / (lambda +
+// sum h^2), then subtract the new component).
+//
+// Layout.  Every mode's CISS carries its own copy of the residual in its own
+// nonzero order, so each column pass streams it coalesced.  The factors are
+// mirrored column-major (R x dims) during an epoch so a column is a
+// contiguous, L2-resident vector and the per-nonzero gathers touch 4 bytes.
+// A copy stores "x minus every component except column `pending`"; the pass
+// of column f over mode m first completes the previous column (subtracting
+// its new component), then adds column f back with its pre-update values
+// (saved before the column's first mode update), and sums num/den.
+//
+// Reductions.  One wave walks a span of tiles, one lane per nonzero.  Row
+// sums inside a tile use a segmented inclusive scan (DPP row shifts and
+// ds_bpermute for the cross-row steps) with the row starts of the tile
+// mask as flags; the row still open at the end of a tile is carried in a
+// wave-uniform register into the next tile.  Completed rows are written as
+// (num, den) pieces with the same piece numbering as the ALS Gram pieces, so
+// the per-row sums have a fixed order (deterministic, no atomics).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels_ccd.h"
+#include "kernels_common.h"
+
+namespace ctc {
+
+// inclusive segmented scan over the wave: f = segment-start flag
+__device__ inline void seg_scan(float &a, float &b, bool f) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float ua = __shfl_up(a, d, 64);
+    const float ub = __shfl_up(b, d, 64);
+    const int uf = __shfl_up((int)f, d, 64);
+    if (lane >= d) {
+      if (!f) {
+        a += ua;
+        b += ub;
+      }
+      f = f || uf;
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
+  const int lane = lane_id();
+  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= a.nspans) return;
+  const CissArgs &c = a.c;
+  const uint64_t t0 = w * a.span_tiles;
+  const uint64_t t1 = t0 + a.span_tiles < c.ntiles ? t0 + a.span_tiles : c.ntiles;
+  const uint32_t poff = a.span_off[w];
+  // column pointers, CISS-relative (0 = own mode)
+  const float *cur[N], *sub[N], *old[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) {
+    cur[s] = a.ft[s] + (size_t)a.col * a.dims[s];
+    sub[s] = a.sub_col >= 0 ? a.ft[s] + (size_t)a.sub_col * a.dims[s] : nullptr;
+    old[s] = a.saved[s];
+  }
+  double carry_n = 0.0, carry_d = 0.0;
+  uint32_t carry_slot = 0;
+  bool have_carry = false;
+
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint64_t base = t * kTile;
+    const uint32_t cnt = (uint32_t)((c.nnz - base) < (uint64_t)kTile ? (c.nnz - base) : kTile);
+    uint64_t mask = c.tile_mask[t];
+    if (cnt < 64) mask &= (1ull << cnt) - 1ull;
+    const uint32_t first_slot = c.tile_slot[t];
+    const bool valid = (uint32_t)lane < cnt;
+    float num = 0.f, den = 0.f;
+    uint32_t myslot = first_slot;
+    if (valid) {
+      const uint64_t p = base + lane;
+      myslot = tile_lane_slot(first_slot, mask, lane);
+      uint32_t ix[N];
+      ix[0] = c.directory[myslot];
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) ix[1 + s] = c.oidx[s][p];
+      float e = a.resid[p];
+      if (a.sub_col >= 0 || a.add_back) {
+        if (a.sub_col >= 0) {
+          float v = sub[0][ix[0]];
+#pragma unroll
+          for (int s = 1; s < N; ++s) v *= sub[s][ix[s]];
+          e -= v;
+        }
+        if (a.add_back) {
+          float v = old[0][ix[0]];
+#pragma unroll
+          for (int s = 1; s < N; ++s) v *= old[s][ix[s]];
+          e += v;
+        }
+        a.resid[p] = e;
+      }
+      float h = cur[1][ix[1]];
+#pragma unroll
+      for (int s = 2; s < N; ++s) h *= cur[s][ix[s]];
+      num = e * h;
+      den = h * h;
+    }
+    const bool head = valid && ((mask >> lane) & 1ull);
+    seg_scan(num, den, lane == 0 || head);
+
+    // first segment: lanes [0, fh); fh = first row start after lane 0
+    const uint64_t later = mask & ~1ull;
+    const uint32_t fh = later ? (uint32_t)__ffsll((unsigned long long)later) - 1u : cnt;
+    const bool join = (t > t0) && !(mask & 1ull);  // lane 0 continues the carried row
+    if (have_carry && !join) {
+      if (lane == 0) {
+        a.pieces[(size_t)(carry_slot + poff) * 2] = carry_n;
+        a.pieces[(size_t)(carry_slot + poff) * 2 + 1] = carry_d;
+      }
+      have_carry = false;
+      carry_n = carry_d = 0.0;
+    }
+    const double fn = (double)__shfl(num, (int)fh - 1, 64) + (join ? carry_n : 0.0);
+    const double fd = (double)__shfl(den, (int)fh - 1, 64) + (join ? carry_d : 0.0);
+    if (fh < cnt) {
+      if (lane == 0) {
+        a.pieces[(size_t)(first_slot + poff) * 2] = fn;
+        a.pieces[(size_t)(first_slot + poff) * 2 + 1] = fd;
+      }
+      // complete middle segments end at lanes i in [fh, cnt-1) whose next lane
+      // starts a row; the last segment (ending at cnt-1) becomes the carry
+      const bool tail = valid && (uint32_t)lane >= fh && (uint32_t)lane + 1 < cnt &&
+                        ((mask >> (lane + 1)) & 1ull);
+      if (tail) {
+        a.pieces[(size_t)(myslot + poff) * 2] = (double)num;
+        a.pieces[(size_t)(myslot + poff) * 2 + 1] = (double)den;
+      }
+      carry_n = (double)__shfl(num, (int)cnt - 1, 64);
+      carry_d = (double)__shfl(den, (int)cnt - 1, 64);
+      carry_slot = (uint32_t)__shfl((int)myslot, (int)cnt - 1, 64);
+    } else {
+      carry_n = fn;
+      carry_d = fd;
+      carry_slot = first_slot;
+    }
+    have_carry = true;
+  }
+  if (have_carry && lane == 0) {
+    a.pieces[(size_t)(carry_slot + poff) * 2] = carry_n;
+    a.pieces[(size_t)(carry_slot + poff) * 2 + 1] = carry_d;
+  }
+}
+
+// a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form)
+__global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const uint32_t *slot_pb,
+                                                    const uint32_t *directory, uint64_t dlength,
+                                                    double lam, float *col) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= dlength) return;
+  double n = 0.0, d = 0.0;
+  for (uint32_t p = slot_pb[s]; p < slot_pb[s + 1]; ++p) {
+    n += pieces[2 * (size_t)p];
+    d += pieces[2 * (size_t)p + 1];
+  }
+  col[directory[s]] = (float)(n / (lam + d));
+}
+
+// residual copy of a mode: r = x - <rows>, row-major factors in CISS-relative order
+template <int N>
+__global__ void __launch_bounds__(256) k_ccd_resid(CissArgs c, FactorArgs fa, int R, float *resid) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < c.nnz;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = p >> 6;
+    const uint32_t slot = tile_lane_slot(c.tile_slot[t], c.tile_mask[t], (int)(p & 63));
+    const float *rows[N];
+    rows[0] = fa.f[0] + (size_t)c.directory[slot] * R;
+#pragma unroll
+    for (int s = 0; s < N - 1; ++s) rows[1 + s] = fa.f[1 + s] + (size_t)c.oidx[s][p] * R;
+    float acc = 0.f;
+    for (int f = 0; f < R; ++f) {
+      float v = rows[0][f];
+#pragma unroll
+      for (int s = 1; s < N; ++s) v *= rows[s][f];
+      acc += v;
+    }
+    resid[p] = c.vals[p] - acc;
+  }
+}
+
+// out (R x rows, column-major) <-> in (rows x R, row-major)
+__global__ void __launch_bounds__(256) k_transpose(const float *in, float *out, uint64_t rows, int R,
+                                                   int to_colmajor) {
+  const uint64_t n = rows * (uint64_t)R;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    if (to_colmajor) {
+      const uint64_t f = p / rows, i = p % rows;  // write-coalesced
+      out[p] = in[i * R + f];
+    } else {
+      const uint64_t i = p / R, f = p % R;
+      out[p] = in[f * rows + i];
+    }
+  }
+}
+
+hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsigned block) {
+  if (a.c.nnz == 0) return hipSuccess;
+  const uint64_t wpb = block / 64;
+  const unsigned grid = (unsigned)((a.nspans + wpb - 1) / wpb);
+  if (nmodes == 3)
+    hipLaunchKernelGGL((k_ccd_col<3>), dim3(grid), dim3(block), 0, st, a);
+  else if (nmodes == 4)
+    hipLaunchKernelGGL((k_ccd_col<4>), dim3(grid), dim3(block), 0, st, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
+                             const uint32_t *directory, uint64_t dlength, double lam, float *col) {
+  if (dlength == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 255) / 256)), dim3(256), 0, st, pieces,
+                     slot_pb, directory, dlength, lam, col);
+  return hipGetLastError();
+}
+
+hipError_t launch_ccd_resid(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                            float *resid) {
+  if (c.nnz == 0) return hipSuccess;
+  uint64_t blocks = (c.nnz + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (nmodes == 3)
+    hipLaunchKernelGGL((k_ccd_resid<3>), dim3((unsigned)blocks), dim3(256), 0, st, c, f, R, resid);
+  else if (nmodes == 4)
+    hipLaunchKernelGGL((k_ccd_resid<4>), dim3((unsigned)blocks), dim3(256), 0, st, c, f, R, resid);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(hipStream_t st, const float *in, float *out, uint64_t rows, int R,
+                            bool to_colmajor) {
+  const uint64_t n = rows * (uint64_t)R;
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)blocks), dim3(256), 0, st, in, out, rows, R,
+                     to_colmajor ? 1 : 0);
+  return hipGetLastError();
+}
+
+}  // namespace ctc

@@ -1,0 +1,72 @@
+// kernels_common.h — device helpers and kernel argument blocks shared by the
+// HIP kernels of libctc (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ctc {
+
+constexpr int kTile = 64;  // nonzeros per CISS tile (one wave load)
+
+// CISS of one mode as the kernels see it.  oidx[s] holds the indices of mode
+// (mode + 1 + s) % N for every nonzero in CISS order.
+struct CissArgs {
+  const uint32_t *oidx[3];
+  const float *vals;
+  const uint32_t *tile_slot;
+  const uint64_t *tile_mask;
+  const uint32_t *directory;
+  const uint32_t *rowptr;
+  uint64_t nnz;
+  uint64_t ntiles;
+  uint64_t dlength;
+};
+
+// COO tensor (validation / test) on the device.
+struct CooArgs {
+  const uint32_t *ind[4];
+  const float *vals;
+  uint64_t nnz;
+};
+
+// factor pointers in CISS-relative order: f[0] = the CISS mode's own factor,
+// f[1 + s] = factor of mode (mode + 1 + s) % N.
+struct FactorArgs {
+  float *f[4];
+};
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Sum across groups of W consecutive lanes (W in {1,2,...,64}); every lane of
+// the group receives the total.  DPP quad permutes for distances 1 and 2,
+// ds_swizzle (xor mode, within 32 lanes) for 4..16, ds_bpermute across halves.
+template <int W>
+__device__ inline float group_sum(float v) {
+  if constexpr (W >= 2)
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+  if constexpr (W >= 4)
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+  if constexpr (W >= 8) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+  if constexpr (W >= 16) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x201F));
+  if constexpr (W >= 32) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  if constexpr (W >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+__device__ inline double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Hardware float atomic add (global_atomic_add_f32) on device memory.
+__device__ inline void atomic_add_f32(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// slot of nonzero e (0..63) of a tile whose first nonzero has slot `first`
+__device__ inline uint32_t tile_lane_slot(uint32_t first, uint64_t mask, int e) {
+  uint64_t below = (e == 63) ? ~0ull : ((2ull << e) - 1ull);
+  return first + (uint32_t)__popcll(mask & below & ~1ull);
+}
+
+}  // namespace ctc

@@ -1,0 +1,34 @@
+// kernels_launch.h — host-side launchers of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels_common.h"
+
+namespace ctc {
+
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                      uint32_t span, float step, float lam, unsigned block);
+hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                         uint32_t span, bool adjoint, float *out, double *loss_part,
+                         unsigned *grid_out, unsigned block);
+uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block);
+hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                           uint32_t span_tiles, uint64_t nspans, const uint32_t *span_off,
+                           double *pieces, unsigned block);
+hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
+                            const uint32_t *directory, uint64_t dlength, int R, double lam, bool refine,
+                            float *A, uint32_t *flag);
+hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f, int nmodes, int R,
+                           unsigned nblocks, double *part);
+hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                            unsigned nblocks, double *part);
+hipError_t launch_sumsq(hipStream_t st, const float *x, uint64_t n, unsigned nblocks, double *part);
+hipError_t launch_reduce_parts(hipStream_t st, const double *part, uint32_t nb, uint32_t groups,
+                               double *out);
+hipError_t launch_fill(hipStream_t st, float *x, uint64_t n, float v);
+hipError_t launch_reg_rows(hipStream_t st, const uint32_t *directory, const uint32_t *rowptr,
+                           uint64_t nrows, int R, float lam, int per_occ, const float *F, float *G,
+                           unsigned nblocks, double *part);
+
+}  // namespace ctc

@@ -1,0 +1,947 @@
+// model.cpp — device context, device-resident model and the solver loops of
+// libctc (C ABI in include/ctc.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "ctc_internal.h"
+#include "host_parallel.h"
+#include "kernels_launch.h"
+#include "model_util.h"
+
+using namespace ctc;
+
+namespace {
+
+constexpr unsigned kEvalBlocks = 512;  // fixed grid for deterministic reductions
+
+template <class T>
+ctc_status dev_alloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  CTC_HIP(hipMalloc((void **)p, n * sizeof(T)));
+  return CTC_OK;
+}
+
+template <class T>
+ctc_status upload(T **p, const T *src, size_t n, hipStream_t st) {
+  CTC_TRY(dev_alloc(p, n));
+  if (n) CTC_HIP(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+  return CTC_OK;
+}
+
+void dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+bool rank_supported(uint32_t R) { return R == 8 || R == 16 || R == 32 || R == 64 || R == 128; }
+
+double median(std::vector<double> v) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  size_t n = v.size();
+  return n % 2 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+}  // namespace
+
+namespace ctc {
+
+// ------------------------------------------------------------- profiling --
+void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b) {
+  *a = *b = nullptr;
+  if (!c->profiling) return;
+  if (c->event_pool.size() < 2) {
+    hipEvent_t e1, e2;
+    if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) return;
+    c->event_pool.push_back(e1);
+    c->event_pool.push_back(e2);
+  }
+  *a = c->event_pool.back();
+  c->event_pool.pop_back();
+  *b = c->event_pool.back();
+  c->event_pool.pop_back();
+  (void)hipEventRecord(*a, c->stream);
+}
+
+void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b) {
+  if (!c->profiling || !a) return;
+  (void)hipEventRecord(b, c->stream);
+  c->pending.push_back({name, a, b});
+}
+
+ctc_status prof_flush(ctc_context *c) {
+  if (c->pending.empty()) return CTC_OK;
+  CTC_HIP(hipStreamSynchronize(c->stream));
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    CTC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto &s = c->stats[p.name];
+    s.launches++;
+    s.total_ms += ms;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+  return CTC_OK;
+}
+
+// ------------------------------------------------------------- scratch --
+ctc_status ensure_scratch(ctc_model *m, size_t bytes) {
+  if (bytes <= m->scratch_bytes) return CTC_OK;
+  if (m->scratch) {
+    CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+    dev_free(m->scratch);
+    m->scratch = nullptr;
+    m->scratch_bytes = 0;
+  }
+  CTC_HIP(hipMalloc(&m->scratch, bytes));
+  m->scratch_bytes = bytes;
+  return CTC_OK;
+}
+
+// ------------------------------------------------------------ CISS upload --
+ctc_status ensure_ciss(ctc_model *m, uint32_t mode) {
+  if (mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "mode %u out of range", mode);
+  if (m->ciss_ready[mode]) return CTC_OK;
+  HostCiss h;
+  CTC_TRY(build_host_ciss(m->train.get(), mode, h));
+  DevCiss &d = m->ciss[mode];
+  hipStream_t st = m->ctx->stream;
+  d.nmodes = h.nmodes;
+  d.mode = mode;
+  d.nnz = h.nnz;
+  d.ntiles = h.ntiles;
+  d.dlength = h.dlength;
+  for (uint32_t s = 0; s + 1 < h.nmodes; ++s) CTC_TRY(upload(&d.oidx[s], h.oidx[s].data(), h.nnz, st));
+  CTC_TRY(upload(&d.vals, h.vals.data(), h.nnz, st));
+  CTC_TRY(upload(&d.tile_slot, h.tile_slot.data(), h.ntiles, st));
+  CTC_TRY(upload(&d.tile_mask, h.tile_mask.data(), h.ntiles, st));
+  CTC_TRY(upload(&d.directory, h.directory.data(), h.dlength, st));
+  CTC_TRY(upload(&d.rowptr, h.rowptr.data(), h.dlength + 1, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  d.h_tile_slot.swap(h.tile_slot);
+  d.h_tile_mask.swap(h.tile_mask);
+  d.h_rowptr.swap(h.rowptr);
+  d.span_tiles = 0;
+  m->ciss_ready[mode] = true;
+  return CTC_OK;
+}
+
+// Piece tables for spans of `span_tiles` tiles (see k_als_gram).  Piece ids
+// run over (span, slot) pairs in order, so piece(span, slot) = slot +
+// span_off[span] with span_off[span] = pieces before the span - its first slot.
+ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles) {
+  DevCiss &d = m->ciss[mode];
+  if (d.span_tiles == span_tiles && d.span_off) return CTC_OK;
+  const uint64_t nspans = (d.ntiles + span_tiles - 1) / span_tiles;
+  std::vector<uint32_t> off(nspans), pb(d.dlength + 1);
+  uint64_t base = 0;
+  for (uint64_t w = 0; w < nspans; ++w) {
+    const uint64_t t0 = w * span_tiles;
+    const uint64_t last = std::min<uint64_t>((t0 + span_tiles) * 64, d.nnz) - 1;
+    const uint32_t first = d.h_tile_slot[t0];
+    const uint64_t tl = last >> 6;
+    const int e = (int)(last & 63);
+    const uint64_t below = e == 63 ? ~0ull : ((2ull << e) - 1ull);
+    const uint32_t lastslot = d.h_tile_slot[tl] + (uint32_t)__builtin_popcountll(d.h_tile_mask[tl] & below & ~1ull);
+    off[w] = (uint32_t)(base - first);
+    base += (uint64_t)(lastslot - first) + 1;
+  }
+  if (base > 0xFFFFFFFFull) return fail(CTC_ERR_UNSUPPORTED, "too many pieces");
+  for (uint64_t s = 0; s < d.dlength; ++s) {
+    const uint64_t w = ((uint64_t)d.h_rowptr[s] >> 6) / span_tiles;
+    pb[s] = (uint32_t)(s + off[w]);
+  }
+  pb[d.dlength] = (uint32_t)base;
+  dev_free(d.span_off);
+  dev_free(d.slot_pb);
+  d.span_off = d.slot_pb = nullptr;
+  hipStream_t st = m->ctx->stream;
+  CTC_TRY(upload(&d.span_off, off.data(), nspans, st));
+  CTC_TRY(upload(&d.slot_pb, pb.data(), d.dlength + 1, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  d.span_tiles = span_tiles;
+  d.nspans = nspans;
+  d.npieces = base;
+  return CTC_OK;
+}
+
+}  // namespace ctc
+
+// =========================================================================
+// context
+// =========================================================================
+extern "C" ctc_status ctc_context_create(int device, ctc_context **out) {
+  if (!out) return fail(CTC_ERR_INVALID_ARG, "ctc_context_create: null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CTC_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(CTC_ERR_INVALID_ARG, "device %d of %d", device, n);
+  CTC_HIP(hipSetDevice(device));
+  ctc_context *c = new ctc_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(CTC_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  (void)hipGetDeviceProperties(&c->props, device);
+  *out = c;
+  return CTC_OK;
+}
+
+extern "C" void ctc_context_destroy(ctc_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" ctc_status ctc_context_stream(ctc_context *c, void **stream) {
+  if (!c || !stream) return fail(CTC_ERR_INVALID_ARG, "ctc_context_stream");
+  *stream = (void *)c->stream;
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_context_set_profiling(ctc_context *c, int enabled) {
+  if (!c) return fail(CTC_ERR_INVALID_ARG, "null context");
+  if (!enabled) CTC_TRY(prof_flush(c));
+  c->profiling = enabled != 0;
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_context_kernel_stats(ctc_context *c, const char *kernel, uint64_t *launches,
+                                               double *total_ms) {
+  if (!c || !kernel) return fail(CTC_ERR_INVALID_ARG, "ctc_context_kernel_stats");
+  CTC_TRY(prof_flush(c));
+  auto it = c->stats.find(kernel);
+  if (launches) *launches = it == c->stats.end() ? 0 : it->second.launches;
+  if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.total_ms;
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_context_reset_stats(ctc_context *c) {
+  if (!c) return fail(CTC_ERR_INVALID_ARG, "null context");
+  CTC_TRY(prof_flush(c));
+  c->stats.clear();
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_context_copy_bandwidth(ctc_context *c, uint64_t bytes, double *gbps) {
+  if (!c || !gbps || bytes < 1024) return fail(CTC_ERR_INVALID_ARG, "ctc_context_copy_bandwidth");
+  CTC_HIP(hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  CTC_HIP(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    dev_free(a);
+    return fail(CTC_ERR_NOMEM, "copy bandwidth buffers");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+  (void)hipEventRecord(e0, c->stream);
+  const int reps = 10;
+  for (int r = 0; r < reps; ++r) (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+  (void)hipEventRecord(e1, c->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(a);
+  dev_free(b);
+  *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;  // read + write
+  return CTC_OK;
+}
+
+// =========================================================================
+// options
+// =========================================================================
+extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
+  if (!o || algorithm < 0 || algorithm > 2) return fail(CTC_ERR_INVALID_ARG, "ctc_options_default");
+  memset(o, 0, sizeof(*o));
+  o->algorithm = algorithm;
+  o->rank = 16;                                                        // SG/base.h:21
+  o->lambda = algorithm == CTC_ALS ? 10.0 : algorithm == CTC_SGD ? 1e-3 : 100.0;  // :39-41
+  o->learning_rate = 1e-3;                                             // :58
+  o->lr_schedule = CTC_LR_BOLD_DRIVER;                                 // SG/sgd.cu:314-323
+  o->lr_decay = 0.95;
+  o->sgd_scale = 1e-5;                                                 // :42-44
+  o->max_epochs = algorithm == CTC_SGD ? 999 : 1000;                   // :22
+  o->tolerance = 1e-4;                                                 // :36
+  o->max_bad_epochs = 3;                                               // :32
+  o->tile_len = 256;
+  o->block = 256;
+  o->seed = 0;
+  o->mode_order = CTC_ORDER_FIXED;
+  o->precision = CTC_PREC_FP32;
+  o->inner_iters = 1;                                                  // :30
+  o->refine = 0;
+  o->eval_every_epoch = 1;
+  o->verbose = 0;
+  return CTC_OK;
+}
+
+// =========================================================================
+// model
+// =========================================================================
+static ctc_status copy_coo(const ctc_coo *src, ctc_coo **dst) {
+  CTC_TRY(ctc_coo_alloc(src->nmodes, src->nnz, src->dims, dst));
+  for (uint32_t m = 0; m < src->nmodes; ++m) {
+    memcpy((*dst)->ind[m], src->ind[m], src->nnz * sizeof(uint32_t));
+    (*dst)->base[m] = src->base[m];
+  }
+  memcpy((*dst)->vals, src->vals, src->nnz * sizeof(double));
+  return CTC_OK;
+}
+
+static ctc_status upload_coo(const ctc_coo *t, DevCoo &d, hipStream_t st) {
+  d.nmodes = t->nmodes;
+  d.nnz = t->nnz;
+  for (uint32_t m = 0; m < t->nmodes; ++m) CTC_TRY(upload(&d.ind[m], t->ind[m], t->nnz, st));
+  std::vector<float> v(t->nnz);
+  for (uint64_t p = 0; p < t->nnz; ++p) v[p] = (float)t->vals[p];
+  CTC_TRY(upload(&d.vals, v.data(), t->nnz, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  return CTC_OK;
+}
+
+static ctc_status check_coo(const ctc_coo *t, const ctc_coo *train, const char *what) {
+  if (!t) return CTC_OK;
+  if (t->nmodes != train->nmodes) return fail(CTC_ERR_INVALID_ARG, "%s: mode count differs", what);
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    const uint64_t dim = train->dims[m];
+    for (uint64_t p = 0; p < t->nnz; ++p)
+      if (t->ind[m][p] >= dim)
+        return fail(CTC_ERR_INVALID_ARG, "%s: index %u of mode %u beyond training dim %llu", what,
+                    t->ind[m][p], m, (unsigned long long)dim);
+  }
+  return CTC_OK;
+}
+
+extern "C" void ctc_model_destroy(ctc_model *m) {
+  if (!m) return;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  for (uint32_t md = 0; md < CTC_MAX_MODES; ++md) {
+    DevCiss &d = m->ciss[md];
+    for (int s = 0; s < 3; ++s) dev_free(d.oidx[s]);
+    dev_free(d.vals);
+    dev_free(d.tile_slot);
+    dev_free(d.tile_mask);
+    dev_free(d.directory);
+    dev_free(d.rowptr);
+    dev_free(d.span_off);
+    dev_free(d.slot_pb);
+    dev_free(m->F[md]);
+    dev_free(m->best[md]);
+    dev_free(m->resid[md]);
+    dev_free(m->FT[md]);
+  }
+  for (DevCoo *c : {&m->valid, &m->test}) {
+    for (int s = 0; s < 4; ++s) dev_free(c->ind[s]);
+    dev_free(c->vals);
+  }
+  dev_free(m->scratch);
+  dev_free(m->dstat);
+  dev_free(m->dflag);
+  dev_free(m->colsave);
+  if (m->hstat) (void)hipHostFree(m->hstat);
+  delete m;
+}
+
+extern "C" ctc_status ctc_model_create(ctc_context *ctx, const ctc_coo *train, const ctc_coo *validation,
+                                       const ctc_coo *test, uint32_t rank, ctc_model **out) {
+  if (!ctx || !train || !out) return fail(CTC_ERR_INVALID_ARG, "ctc_model_create: null argument");
+  if (!rank_supported(rank)) return fail(CTC_ERR_UNSUPPORTED, "rank %u: supported ranks are 8,16,32,64,128", rank);
+  if (train->nmodes < 3 || train->nmodes > 4)
+    return fail(CTC_ERR_UNSUPPORTED, "%u-way tensors: supported are 3- and 4-way", train->nmodes);
+  if (train->nnz == 0) return fail(CTC_ERR_INVALID_ARG, "empty training tensor");
+  for (uint32_t md = 0; md < train->nmodes; ++md)
+    for (uint64_t p = 0; p < train->nnz; ++p)
+      if (train->ind[md][p] >= train->dims[md])
+        return fail(CTC_ERR_INVALID_ARG, "train index beyond dims in mode %u", md);
+  CTC_TRY(check_coo(validation, train, "validation"));
+  CTC_TRY(check_coo(test, train, "test"));
+  CTC_HIP(hipSetDevice(ctx->device));
+  ctc_model *m = new ctc_model();
+  m->ctx = ctx;
+  m->nmodes = train->nmodes;
+  m->rank = rank;
+  m->train_nnz = train->nnz;
+  for (uint32_t md = 0; md < m->nmodes; ++md) m->dims[md] = train->dims[md];
+  ctc_coo *tc = nullptr;
+  ctc_status s = copy_coo(train, &tc);
+  if (s != CTC_OK) {
+    delete m;
+    return s;
+  }
+  m->train.reset(tc);
+  auto bail = [&](ctc_status st) {
+    ctc_model_destroy(m);
+    return st;
+  };
+  hipStream_t st = ctx->stream;
+  if (validation && validation->nnz) {
+    if ((s = upload_coo(validation, m->valid, st)) != CTC_OK) return bail(s);
+  }
+  if (test && test->nnz) {
+    if ((s = upload_coo(test, m->test, st)) != CTC_OK) return bail(s);
+  }
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    if ((s = dev_alloc(&m->F[md], m->dims[md] * rank)) != CTC_OK) return bail(s);
+    if ((s = dev_alloc(&m->best[md], m->dims[md] * rank)) != CTC_OK) return bail(s);
+  }
+  if ((s = dev_alloc(&m->dstat, 4096)) != CTC_OK) return bail(s);
+  if ((s = dev_alloc(&m->dflag, 16)) != CTC_OK) return bail(s);
+  if (hipHostMalloc((void **)&m->hstat, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(CTC_ERR_NOMEM, "pinned host buffer"));
+  *out = m;
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_model_init_random(ctc_model *m, uint64_t seed) {
+  if (!m) return fail(CTC_ERR_INVALID_ARG, "null model");
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    std::vector<float> h(m->dims[md] * m->rank);
+    // U(-1,1) entries (SG/matrixprocess.c:156-165) from a seeded generator
+    std::mt19937_64 g(seed * 0x9E3779B97F4A7C15ull + md + 1);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (auto &x : h) x = (float)u(g);
+    CTC_TRY(ctc_model_set_factor(m, md, h.data()));
+  }
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_model_set_factor(ctc_model *m, uint32_t mode, const float *host) {
+  if (!m || !host || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_model_set_factor");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  const size_t n = m->dims[mode] * m->rank;
+  CTC_HIP(hipMemcpyAsync(m->F[mode], host, n * sizeof(float), hipMemcpyHostToDevice, m->ctx->stream));
+  CTC_HIP(hipMemcpyAsync(m->best[mode], m->F[mode], n * sizeof(float), hipMemcpyDeviceToDevice,
+                         m->ctx->stream));
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  m->resid_valid = false;
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_model_get_factor(ctc_model *m, uint32_t mode, int best, float *host) {
+  if (!m || !host || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_model_get_factor");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  const size_t n = m->dims[mode] * m->rank;
+  CTC_HIP(hipMemcpyAsync(host, best ? m->best[mode] : m->F[mode], n * sizeof(float),
+                         hipMemcpyDeviceToHost, m->ctx->stream));
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_model_factor_device(ctc_model *m, uint32_t mode, float **dev) {
+  if (!m || !dev || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_model_factor_device");
+  *dev = m->F[mode];
+  return CTC_OK;
+}
+
+extern "C" ctc_status ctc_model_prepare(ctc_model *m, const ctc_options *o) {
+  if (!m || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_model_prepare");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  if (o->algorithm == CTC_SGD) return ensure_ciss(m, 0);
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    CTC_TRY(ensure_ciss(m, md));
+    if (o->algorithm == CTC_ALS) CTC_TRY(ensure_pieces(m, md, span_tiles_of(o)));
+  }
+  return CTC_OK;
+}
+
+// =========================================================================
+// single steps
+// =========================================================================
+static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
+  const uint32_t R = m->rank;
+  if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "ALS at rank %u is not implemented", R);
+  if (o->precision != CTC_PREC_FP32)
+    return fail(CTC_ERR_UNSUPPORTED, "mixed-precision ALS is not implemented");
+  const uint32_t S = span_tiles_of(o);
+  CTC_TRY(ensure_ciss(m, mode));
+  CTC_TRY(ensure_pieces(m, mode, S));
+  DevCiss &d = m->ciss[mode];
+  const size_t psz = (size_t)R * R + R;
+  CTC_TRY(ensure_scratch(m, d.npieces * psz * sizeof(double)));
+  hipStream_t st = m->ctx->stream;
+  double *pieces = (double *)m->scratch;
+  CissArgs ca = ciss_args(d);
+  FactorArgs fa = rel_factors(m->F, m->nmodes, mode);
+  {
+    Prof p(m->ctx, "als_gram");
+    CTC_HIP(launch_als_gram(st, ca, fa, (int)m->nmodes, (int)R, S, d.nspans, d.span_off, pieces,
+                            block_of(o)));
+  }
+  // rows absent from the directory become 0 (SG/als.cu:699 zeroes the factor)
+  CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
+  {
+    Prof p(m->ctx, "als_solve");
+    CTC_HIP(launch_als_solve(st, pieces, d.slot_pb, d.directory, d.dlength, (int)R, o->lambda,
+                             o->refine != 0 || o->precision == CTC_PREC_MIXED, m->F[mode], m->dflag));
+  }
+  m->resid_valid = false;
+  return CTC_OK;
+}
+
+static ctc_status check_flag(ctc_model *m) {
+  uint32_t f = 0;
+  CTC_HIP(hipMemcpyAsync(&f, m->dflag, sizeof(uint32_t), hipMemcpyDeviceToHost, m->ctx->stream));
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  if (f) {
+    CTC_HIP(hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), m->ctx->stream));
+    CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+    return fail(CTC_ERR_NOT_SPD, "an ALS normal system (G + lambda I) was not positive definite");
+  }
+  return CTC_OK;
+}
+
+static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
+  CTC_TRY(ensure_ciss(m, 0));
+  const float step = (float)(lr * o->sgd_scale);
+  Prof p(m->ctx, "sgd");
+  CTC_HIP(launch_sgd(m->ctx->stream, ciss_args(m->ciss[0]), rel_factors(m->F, m->nmodes, 0), (int)m->nmodes,
+                     (int)m->rank, span_tiles_of(o) * 64, step, (float)o->lambda, block_of(o)));
+  m->resid_valid = false;
+  return CTC_OK;
+}
+
+static ctc_status ccd_epoch(ctc_model *m, const ctc_options *o) { return ctc::ccd_epoch_impl(m, o); }
+
+extern "C" ctc_status ctc_als_mode_update(ctc_model *m, uint32_t mode, const ctc_options *o) {
+  if (!m || !o || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_als_mode_update");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_HIP(hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), m->ctx->stream));
+  CTC_TRY(als_mode(m, mode, o));
+  CTC_TRY(check_flag(m));
+  return prof_flush(m->ctx);
+}
+
+extern "C" ctc_status ctc_sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
+  if (!m || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_sgd_epoch");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_TRY(sgd_epoch(m, o, lr));
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  return prof_flush(m->ctx);
+}
+
+extern "C" ctc_status ctc_mttkrp(ctc_model *m, uint32_t mode, float *host_out) {
+  if (!m || !host_out || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_mttkrp");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_TRY(ensure_ciss(m, mode));
+  const uint32_t R = m->rank;
+  const size_t n = m->dims[mode] * R;
+  CTC_TRY(ensure_scratch(m, n * sizeof(float)));
+  float *out = (float *)m->scratch;
+  hipStream_t st = m->ctx->stream;
+  CTC_HIP(launch_fill(st, out, n, 0.f));
+  unsigned grid = 0;
+  {
+    Prof p(m->ctx, "mttkrp");
+    CTC_HIP(launch_mttkrp(st, ciss_args(m->ciss[mode]), rel_factors(m->F, m->nmodes, mode), (int)m->nmodes,
+                          (int)R, 256, false, out, nullptr, &grid, 256));
+  }
+  CTC_HIP(hipMemcpyAsync(host_out, out, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  return prof_flush(m->ctx);
+}
+
+// =========================================================================
+// evaluation (replaces tc_loss_sq / tc_rmse / tc_frob_sq,
+// SG/completion.c:123-214, with FP64 device reductions)
+// =========================================================================
+extern "C" ctc_status ctc_evaluate(ctc_model *m, double lambda, double *loss, double *frob,
+                                   double *rmse_tr, double *rmse_vl, double *rmse_te) {
+  if (!m) return fail(CTC_ERR_INVALID_ARG, "ctc_evaluate: null model");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_TRY(ensure_ciss(m, 0));
+  hipStream_t st = m->ctx->stream;
+  const uint32_t N = m->nmodes, R = m->rank;
+  const uint32_t groups = 3 + N;
+  double *part = m->dstat + 64;  // [groups][kEvalBlocks] after the 64 result slots
+  if ((size_t)groups * kEvalBlocks + 64 > 4096) return fail(CTC_ERR_INVALID_ARG, "eval buffer");
+  FactorArgs f0 = rel_factors(m->F, N, 0);
+  FactorArgs fa{};
+  for (uint32_t md = 0; md < N; ++md) fa.f[md] = m->F[md];
+  {
+    Prof p(m->ctx, "eval");
+    CTC_HIP(launch_eval_ciss(st, ciss_args(m->ciss[0]), f0, (int)N, (int)R, kEvalBlocks, part));
+    CooArgs cv{}, ct{};
+    for (uint32_t md = 0; md < N; ++md) {
+      cv.ind[md] = m->valid.ind[md];
+      ct.ind[md] = m->test.ind[md];
+    }
+    cv.vals = m->valid.vals;
+    cv.nnz = m->valid.nnz;
+    ct.vals = m->test.vals;
+    ct.nnz = m->test.nnz;
+    CTC_HIP(launch_eval_coo(st, cv, fa, (int)N, (int)R, kEvalBlocks, part + kEvalBlocks));
+    CTC_HIP(launch_eval_coo(st, ct, fa, (int)N, (int)R, kEvalBlocks, part + 2 * kEvalBlocks));
+    for (uint32_t md = 0; md < N; ++md)
+      CTC_HIP(launch_sumsq(st, m->F[md], m->dims[md] * R, kEvalBlocks, part + (3 + md) * kEvalBlocks));
+    CTC_HIP(launch_reduce_parts(st, part, kEvalBlocks, groups, m->dstat));
+  }
+  CTC_HIP(hipMemcpyAsync(m->hstat, m->dstat, groups * sizeof(double), hipMemcpyDeviceToHost, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  const double ltr = m->hstat[0], lvl = m->hstat[1], lte = m->hstat[2];
+  double fr = 0.0;
+  for (uint32_t md = 0; md < N; ++md) fr += lambda * m->hstat[3 + md];
+  if (loss) *loss = ltr;
+  if (frob) *frob = fr;
+  if (rmse_tr) *rmse_tr = std::sqrt(ltr / (double)m->train_nnz);
+  if (rmse_vl) *rmse_vl = m->valid.nnz ? std::sqrt(lvl / (double)m->valid.nnz) : 0.0;
+  if (rmse_te) *rmse_te = m->test.nnz ? std::sqrt(lte / (double)m->test.nnz) : 0.0;
+  return prof_flush(m->ctx);
+}
+
+// =========================================================================
+// gradient (adjoint)
+// =========================================================================
+extern "C" ctc_status ctc_grad(ctc_model *m, const float *const *factors, float *const *grads,
+                               double lambda, int reg_per_occurrence, double *loss, void *stream) {
+  if (!m || !factors || !grads) return fail(CTC_ERR_INVALID_ARG, "ctc_grad: null argument");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  const uint32_t N = m->nmodes, R = m->rank;
+  hipStream_t st = stream ? (hipStream_t)stream : m->ctx->stream;
+  float *F[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  for (uint32_t md = 0; md < N; ++md) {
+    if (!factors[md] || !grads[md]) return fail(CTC_ERR_INVALID_ARG, "ctc_grad: null factor/grad");
+    F[md] = (float *)factors[md];
+    CTC_TRY(ensure_ciss(m, md));
+  }
+  // partial sums: N walks + N regularisers, each up to `nb` blocks
+  const uint32_t span = 256;
+  uint64_t nb = 0;
+  for (uint32_t md = 0; md < N; ++md) nb = std::max<uint64_t>(nb, mttkrp_grid(m->train_nnz, (int)R, span, 256));
+  nb = std::max<uint64_t>(nb, kEvalBlocks);
+  CTC_TRY(ensure_scratch(m, (size_t)2 * N * nb * sizeof(double)));
+  double *part = (double *)m->scratch;
+  CTC_HIP(hipMemsetAsync(part, 0, (size_t)2 * N * nb * sizeof(double), st));
+  for (uint32_t md = 0; md < N; ++md) {
+    CTC_HIP(launch_fill(st, grads[md], m->dims[md] * R, 0.f));
+    unsigned grid = 0;
+    {
+      Prof p(m->ctx, "adjoint");
+      CTC_HIP(launch_mttkrp(st, ciss_args(m->ciss[md]), rel_factors(F, N, md), (int)N, (int)R, span, true,
+                            grads[md], part + (size_t)md * nb, &grid, 256));
+    }
+    const DevCiss &d = m->ciss[md];
+    const uint64_t nrows = reg_per_occurrence ? d.dlength : m->dims[md];
+    CTC_HIP(launch_reg_rows(st, d.directory, d.rowptr, nrows, (int)R, (float)lambda, reg_per_occurrence, F[md],
+                            grads[md], kEvalBlocks, part + (size_t)(N + md) * nb));
+  }
+  CTC_HIP(launch_reduce_parts(st, part, (uint32_t)nb, 2 * N, m->dstat));
+  CTC_HIP(hipMemcpyAsync(m->hstat, m->dstat, 2 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  if (loss) {
+    double l = 0.5 * m->hstat[0];  // squared residuals from the mode-0 walk
+    for (uint32_t md = 0; md < N; ++md) l += m->hstat[N + md];
+    *loss = l;
+  }
+  return prof_flush(m->ctx);
+}
+
+// =========================================================================
+// solver loop
+// =========================================================================
+extern "C" ctc_status ctc_ccd_epoch(ctc_model *m, const ctc_options *o) {
+  if (!m || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_ccd_epoch");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_TRY(ccd_epoch(m, o));
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  return prof_flush(m->ctx);
+}
+
+static void print_progress(uint32_t epoch, const ctc_epoch_stats &s) {
+  // the reference's progress line (SG/completion.c:111-113) plus RMSE-te
+  printf("epoch:%u   loss: %0.5e   RMSE-tr: %0.5e   RMSE-vl: %0.5e   RMSE-te: %0.5e   ms: %0.3f\n", epoch,
+         s.loss, s.rmse_tr, s.rmse_vl, s.rmse_te, s.epoch_ms);
+  fflush(stdout);
+}
+
+extern "C" ctc_status ctc_solve(ctc_model *m, const ctc_options *o, ctc_result *res) {
+  if (!m || !o || !res) return fail(CTC_ERR_INVALID_ARG, "ctc_solve: null argument");
+  if (o->rank != 0 && o->rank != m->rank)
+    return fail(CTC_ERR_INVALID_ARG, "options rank %u differs from model rank %u", o->rank, m->rank);
+  if (o->algorithm < 0 || o->algorithm > 2) return fail(CTC_ERR_INVALID_ARG, "unknown algorithm");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  CTC_TRY(ctc_model_prepare(m, o));
+  hipStream_t st = m->ctx->stream;
+  const uint32_t N = m->nmodes;
+  res->epochs = 0;
+  res->best_epoch = 0;
+  res->converged = 0;
+  res->train_nnz = m->train_nnz;
+  hipEvent_t e0, e1;
+  CTC_HIP(hipEventCreate(&e0));
+  CTC_HIP(hipEventCreate(&e1));
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } guard{e0, e1};
+
+  // convergence state (SG/completion.c:454-458 initial values)
+  double best_rmse = 1e7;
+  uint32_t nbad = 0;
+  auto record = [&](uint32_t e, const ctc_epoch_stats &s) {
+    if (res->history && e < res->history_cap) res->history[e] = s;
+  };
+  auto evaluate = [&](ctc_epoch_stats &s) -> ctc_status {
+    return ctc_evaluate(m, o->lambda, &s.loss, &s.frob, &s.rmse_tr, &s.rmse_vl, &s.rmse_te);
+  };
+  // returns 1 when converged (SG/completion.c:296-362)
+  auto converge = [&](uint32_t e, const ctc_epoch_stats &s) -> ctc_status {
+    const double r = m->valid.nnz ? s.rmse_vl : s.rmse_tr;
+    if (r - best_rmse < -o->tolerance) {
+      nbad = 0;
+      best_rmse = r;
+      res->best_epoch = e;
+      for (uint32_t md = 0; md < N; ++md)
+        CTC_HIP(hipMemcpyAsync(m->best[md], m->F[md], m->dims[md] * m->rank * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+      ++nbad;
+      if (nbad == o->max_bad_epochs) res->converged = 1;
+    }
+    return CTC_OK;
+  };
+
+  ctc_epoch_stats s0{};
+  CTC_TRY(evaluate(s0));
+  s0.lr = o->learning_rate;
+  record(0, s0);
+  if (o->verbose) print_progress(0, s0);
+  CTC_TRY(converge(0, s0));
+  double lr = o->learning_rate;
+  double prev_obj = s0.loss + s0.frob;
+  std::mt19937_64 rot(o->seed ^ 0xA15ull);
+  std::vector<double> times;
+  CTC_HIP(hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), st));
+  for (uint32_t e = 1; e <= o->max_epochs; ++e) {
+    CTC_HIP(hipEventRecord(e0, st));
+    if (o->algorithm == CTC_ALS) {
+      const uint32_t start = o->mode_order == CTC_ORDER_ROTATE ? (uint32_t)(rot() % N) : 0;
+      for (uint32_t k = 0; k < N; ++k) CTC_TRY(als_mode(m, (start + k) % N, o));
+    } else if (o->algorithm == CTC_SGD) {
+      CTC_TRY(sgd_epoch(m, o, lr));
+    } else {
+      CTC_TRY(ccd_epoch(m, o));
+    }
+    CTC_HIP(hipEventRecord(e1, st));
+    CTC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CTC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    times.push_back(ms);
+    if (o->algorithm == CTC_ALS) CTC_TRY(check_flag(m));
+    res->epochs = e;
+    ctc_epoch_stats s{};
+    s.epoch_ms = ms;
+    s.lr = lr;
+    if (o->eval_every_epoch || e == o->max_epochs) {
+      CTC_TRY(evaluate(s));
+      record(e, s);
+      if (o->verbose) print_progress(e, s);
+      CTC_TRY(converge(e, s));
+    } else {
+      record(e, s);
+    }
+    if (o->algorithm == CTC_SGD) {
+      if (o->lr_schedule == CTC_LR_BOLD_DRIVER && o->eval_every_epoch) {
+        const double obj = s.loss + s.frob;  // SG/sgd.cu:309, 314-323
+        if (e > 1) lr *= obj < prev_obj ? 1.05 : 0.50;
+        prev_obj = obj;
+      } else if (o->lr_schedule == CTC_LR_EXP_DECAY) {
+        lr *= o->lr_decay;
+      }
+    }
+    if (res->converged) break;
+  }
+  CTC_TRY(prof_flush(m->ctx));
+  std::vector<double> tail(times.size() > 1 ? times.begin() + 1 : times.begin(), times.end());
+  res->ms_per_epoch = median(tail);
+  res->nnz_per_s = res->ms_per_epoch > 0 ? (double)m->train_nnz / (res->ms_per_epoch * 1e-3) : 0.0;
+  res->best_rmse = best_rmse;
+  return CTC_OK;
+}
+
+// =========================================================================
+// reference-shaped entry points
+// =========================================================================
+namespace {
+
+ctc_status run_reference_shaped(const ctc_coo *traina, const ctc_coo *validation, const ctc_coo *test,
+                                ctc_matrix **mats, ctc_matrix **best_mats, int alg, double lambda, double lr,
+                                double *best_rmse, double *tolerance, uint64_t *nbad, uint64_t *bestep,
+                                uint64_t *maxbad) {
+  if (!traina || !mats) return fail(CTC_ERR_INVALID_ARG, "reference entry: null argument");
+  const uint32_t R = (uint32_t)mats[0]->J;
+  ctc_context *ctx = nullptr;
+  CTC_TRY(ctc_context_create(0, &ctx));
+  ctc_model *m = nullptr;
+  ctc_status s = ctc_model_create(ctx, traina, validation, test, R, &m);
+  if (s != CTC_OK) {
+    ctc_context_destroy(ctx);
+    return s;
+  }
+  for (uint32_t md = 0; md < traina->nmodes && s == CTC_OK; ++md) {
+    std::vector<float> h(mats[md]->I * R);
+    for (size_t x = 0; x < h.size(); ++x) h[x] = (float)mats[md]->values[x];
+    s = ctc_model_set_factor(m, md, h.data());
+  }
+  ctc_options o;
+  ctc_options_default(alg, &o);
+  o.rank = R;
+  o.lambda = lambda;
+  o.learning_rate = lr;
+  if (tolerance) o.tolerance = *tolerance;
+  if (maxbad) o.max_bad_epochs = (uint32_t)*maxbad;
+  o.verbose = 1;
+  ctc_result r{};
+  if (s == CTC_OK) s = ctc_solve(m, &o, &r);
+  for (uint32_t md = 0; md < traina->nmodes && s == CTC_OK; ++md) {
+    std::vector<float> h(mats[md]->I * R);
+    s = ctc_model_get_factor(m, md, 0, h.data());
+    for (size_t x = 0; x < h.size() && s == CTC_OK; ++x) mats[md]->values[x] = h[x];
+    if (best_mats && best_mats[md] && s == CTC_OK) {
+      s = ctc_model_get_factor(m, md, 1, h.data());
+      for (size_t x = 0; x < h.size() && s == CTC_OK; ++x) best_mats[md]->values[x] = h[x];
+    }
+  }
+  if (best_rmse) *best_rmse = r.best_rmse;
+  if (bestep) *bestep = r.best_epoch;
+  if (nbad) *nbad = r.converged ? o.max_bad_epochs : 0;
+  ctc_model_destroy(m);
+  ctc_context_destroy(ctx);
+  return s;
+}
+
+}  // namespace
+
+extern "C" ctc_status ctc_tc_als(const ctc_coo *traina, const ctc_coo *, const ctc_coo *,
+                                 const ctc_coo *validation, const ctc_coo *test, ctc_matrix **mats,
+                                 ctc_matrix **best_mats, int, double regularization_index, double *best_rmse,
+                                 double *tolerance, uint64_t *nbadepochs, uint64_t *bestepochs,
+                                 uint64_t *max_badepochs) {
+  return run_reference_shaped(traina, validation, test, mats, best_mats, CTC_ALS, regularization_index, 1e-3,
+                              best_rmse, tolerance, nbadepochs, bestepochs, max_badepochs);
+}
+
+extern "C" ctc_status ctc_tc_sgd(const ctc_coo *traina, const ctc_coo *, const ctc_coo *,
+                                 const ctc_coo *validation, const ctc_coo *test, ctc_matrix **mats,
+                                 ctc_matrix **best_mats, int, double regularization_index, double learning_rate,
+                                 double *best_rmse, double *tolerance, uint64_t *nbadepochs,
+                                 uint64_t *bestepochs, uint64_t *max_badepochs) {
+  return run_reference_shaped(traina, validation, test, mats, best_mats, CTC_SGD, regularization_index,
+                              learning_rate, best_rmse, tolerance, nbadepochs, bestepochs, max_badepochs);
+}
+
+extern "C" ctc_status ctc_tc_ccd(const ctc_coo *traina, const ctc_coo *, const ctc_coo *,
+                                 const ctc_coo *validation, const ctc_coo *test, ctc_matrix **mats,
+                                 ctc_matrix **best_mats, int, double regularization_index, double *best_rmse,
+                                 double *tolerance, uint64_t *nbadepochs, uint64_t *bestepochs,
+                                 uint64_t *max_badepochs) {
+  return run_reference_shaped(traina, validation, test, mats, best_mats, CTC_CCD, regularization_index, 1e-3,
+                              best_rmse, tolerance, nbadepochs, bestepochs, max_badepochs);
+}
+
+extern "C" ctc_status ctc_tc_main_ciss(const ctc_coo *traina, const ctc_coo *, const ctc_coo *,
+                                       const ctc_coo *validation, const ctc_coo *test, int algorithm_index,
+                                       const ctc_options *opt) {
+  if (!traina) return fail(CTC_ERR_INVALID_ARG, "ctc_tc_main_ciss: null train");
+  ctc_options o;
+  if (opt)
+    o = *opt;
+  else
+    CTC_TRY(ctc_options_default(algorithm_index, &o));
+  o.algorithm = algorithm_index;
+  o.verbose = 1;
+  ctc_context *ctx = nullptr;
+  CTC_TRY(ctc_context_create(0, &ctx));
+  ctc_model *m = nullptr;
+  ctc_status s = ctc_model_create(ctx, traina, validation, test, o.rank, &m);
+  if (s == CTC_OK) s = ctc_model_init_random(m, o.seed);
+  ctc_result r{};
+  if (s == CTC_OK) s = ctc_solve(m, &o, &r);
+  if (m) ctc_model_destroy(m);
+  ctc_context_destroy(ctx);
+  return s;
+}
+
+// =========================================================================
+// autotuning (replaces the GA over blocksize x tile length, GA/fitness.c:31-100)
+// =========================================================================
+extern "C" ctc_status ctc_autotune(ctc_model *m, const ctc_options *o, ctc_tune_result *out) {
+  if (!m || !o || !out) return fail(CTC_ERR_INVALID_ARG, "ctc_autotune");
+  CTC_HIP(hipSetDevice(m->ctx->device));
+  const uint32_t N = m->nmodes;
+  // save factors: tuning sweeps modify them
+  std::vector<std::vector<float>> saved(N);
+  for (uint32_t md = 0; md < N; ++md) {
+    saved[md].resize(m->dims[md] * m->rank);
+    CTC_TRY(ctc_model_get_factor(m, md, 0, saved[md].data()));
+  }
+  const uint32_t tiles[] = {64, 128, 256, 512, 1024, 2048, 4096};
+  const unsigned blocks[] = {64, 128, 256};
+  hipStream_t st = m->ctx->stream;
+  hipEvent_t e0, e1;
+  CTC_HIP(hipEventCreate(&e0));
+  CTC_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  ctc_tune_result r{o->tile_len, o->block, 0.0, 0};
+  ctc_status s = CTC_OK;
+  for (uint32_t tl : tiles) {
+    for (unsigned b : blocks) {
+      ctc_options t = *o;
+      t.tile_len = tl;
+      t.block = b;
+      double tot = 0.0;
+      const int reps = 3;
+      for (int rep = 0; rep <= reps && s == CTC_OK; ++rep) {
+        (void)hipEventRecord(e0, st);
+        if (o->algorithm == CTC_SGD)
+          s = sgd_epoch(m, &t, 0.0);  // lr 0: same memory traffic, factors unchanged
+        else if (o->algorithm == CTC_ALS)
+          for (uint32_t md = 0; md < N && s == CTC_OK; ++md) s = als_mode(m, md, &t);
+        else
+          s = ccd_epoch(m, &t);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) tot += ms;
+      }
+      if (s != CTC_OK) break;
+      r.trials++;
+      const double avg = tot / reps;
+      if (avg < best) {
+        best = avg;
+        r.tile_len = tl;
+        r.block = b;
+      }
+    }
+    if (s != CTC_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (uint32_t md = 0; md < N; ++md) {
+    ctc_status s2 = ctc_model_set_factor(m, md, saved[md].data());
+    if (s == CTC_OK) s = s2;
+  }
+  r.ms = best;
+  *out = r;
+  return s;
+}

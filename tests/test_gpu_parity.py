@@ -1,0 +1,287 @@
+"""GPU parity tests: the HIP kernels of libctc (through the C ABI) against the
+CPU oracle on the same seeded inputs.
+
+Tolerances (from the reference's update rules as the oracle restates them):
+  * MTTKRP, Gram-based ALS updates, CCD++ and the gradient: relative Frobenius
+    error vs the FP64 oracle, FP32 on the GPU -> <= 1e-5 per factor.
+  * SGD (Hogwild on the GPU vs sequential in the oracle): per-epoch test RMSE
+    within 1e-3.
+  * Loss / RMSE: FP64 accumulation on both sides -> 1e-6 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cutc_amd as ct
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ct.Context(0)
+
+
+def ocoo(t):
+    ind, vals = t.arrays()
+    return O.Coo(ind, vals, t.dims)
+
+
+def init_factors(dims, R, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def small_problem(nmodes=3, nnz=20000, zipf=None, seed=1, test_fraction=0.1, dims=None):
+    dims = dims or ((300, 200, 60) if nmodes == 3 else (120, 80, 30, 12))
+    zipf = zipf or tuple([1.0, 0.7] + [0.0] * (nmodes - 2))
+    return ct.synth(dims, nnz, zipf=zipf, true_rank=4, value_map=ct.VALUES_AFFINE_1_5,
+                    noise_std=0.1, test_fraction=test_fraction, seed=seed)
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("R", [8, 16, 32, 64, 128])
+@pytest.mark.parametrize("nmodes", [3, 4])
+def test_mttkrp(ctx, R, nmodes):
+    tr, te = small_problem(nmodes, seed=R + nmodes)
+    say(f"[mttkrp] R={R} N={nmodes} nnz={tr.nnz}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 3)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    F64 = [f.astype(np.float64) for f in F]
+    for mode in range(nmodes):
+        got = m.mttkrp(mode)
+        exp = O.mttkrp(oc, mode, F64)
+        assert relerr(got, exp) < 1e-5, (mode, relerr(got, exp))
+
+
+@pytest.mark.parametrize("nmodes", [3, 4])
+def test_evaluate(ctx, nmodes):
+    tr, te = small_problem(nmodes, seed=9)
+    say(f"[evaluate] N={nmodes}")
+    m = ct.Model(ctx, tr, te, te, rank=16)
+    F = init_factors(tr.dims, 16, 4)
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    ev = m.evaluate(lam=0.5)
+    assert ev["loss"] == pytest.approx(O.loss_sq(ocoo(tr), F64), rel=1e-6)
+    assert ev["rmse_te"] == pytest.approx(O.rmse(ocoo(te), F64), rel=1e-6)
+    assert ev["rmse_vl"] == pytest.approx(ev["rmse_te"], rel=1e-12)
+    assert ev["frob"] == pytest.approx(O.frob_sq(tr.dims, F64, 0.5), rel=1e-6)
+
+
+@pytest.mark.parametrize("R", [8, 16, 32, 64])
+@pytest.mark.parametrize("tile_len", [64, 256, 4096])
+def test_als_mode_update(ctx, R, tile_len):
+    tr, te = small_problem(3, nnz=30000, seed=20 + R)
+    say(f"[als_mode] R={R} tile_len={tile_len}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 5)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    F64 = [f.astype(np.float64) for f in F]
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, tile_len=tile_len)
+    for mode in range(3):
+        m.als_mode_update(mode, opt)
+        assert O.als_mode(oc, mode, F64, 0.05) == 0
+        got = m.factor(mode)
+        e = relerr(got, F64[mode])
+        assert e < 1e-5, (mode, e)
+        # continue from identical inputs: re-seed the GPU with the oracle's factors
+        F64 = [f.astype(np.float32).astype(np.float64) for f in F64]
+        m.set_factors([f.astype(np.float32) for f in F64])
+
+
+def test_als_deterministic(ctx):
+    tr, te = small_problem(3, nnz=30000, seed=77)
+    say("[als_deterministic]")
+    outs = []
+    for _ in range(2):
+        m = ct.Model(ctx, tr, None, te, rank=32)
+        m.set_factors(init_factors(tr.dims, 32, 1))
+        opt = ct.Options(algorithm=ct.ALS, rank=32, lambda_=0.05, tile_len=128)
+        for mode in range(3):
+            m.als_mode_update(mode, opt)
+        outs.append(m.factors())
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_als_config1_ten_iterations(ctx):
+    """Config 1 of the benchmark definition at full size: 2000x1500x300,
+    1M nonzeros, rank-8 truth -> [1,5] + N(0,0.1), R=16, lambda=0.05, 90/10
+    split, 10 iterations; every factor within 1e-5 of the FP64 oracle after
+    every iteration."""
+    tr, te = ct.synth((2000, 1500, 300), 1_000_000, true_rank=8, value_map=ct.VALUES_AFFINE_1_5,
+                      noise_std=0.1, test_fraction=0.1, seed=0x5EED0001)
+    say(f"[als_config1] train={tr.nnz} test={te.nnz}")
+    R = 16
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 11)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    ot = ocoo(te)
+    F64 = [f.astype(np.float64) for f in F]
+    orders = [O.mode_order(oc, md) for md in range(3)]
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05)
+    worst = 0.0
+    for it in range(10):
+        for mode in range(3):
+            m.als_mode_update(mode, opt)
+            assert O.als_mode(oc, mode, F64, 0.05, order=orders[mode], nthreads=8) == 0
+        for mode in range(3):
+            worst = max(worst, relerr(m.factor(mode), F64[mode]))
+        rm_gpu = m.evaluate()["rmse_te"]
+        rm_orc = O.rmse(ot, F64)
+        say(f"  it={it} worst_rel={worst:.2e} rmse_te gpu={rm_gpu:.6f} oracle={rm_orc:.6f}")
+        assert abs(rm_gpu - rm_orc) < 1e-4
+    assert worst < 1e-5, worst
+
+
+def test_als_not_spd_is_reported(ctx):
+    # lambda = 0 and rows with fewer nonzeros than R: singular normal systems
+    tr, _ = ct.synth((500, 40, 30), 1500, test_fraction=0.0, seed=3)
+    say("[als_not_spd]")
+    m = ct.Model(ctx, tr, None, None, rank=16)
+    m.set_factors(init_factors(tr.dims, 16, 2))
+    with pytest.raises(ct.CtcError) as e:
+        m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.0))
+    assert e.value.status == 7
+    # the model stays usable afterwards
+    m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.1))
+
+
+# --------------------------------------------------------------------------
+def test_sgd_test_rmse_trajectory(ctx):
+    """SGD (lr 1e-3, x0.95 per epoch, lambda 0.05, scale 1) test-RMSE per
+    epoch within 1e-3 of the sequential FP64 oracle."""
+    tr, te = ct.synth((3000, 1500, 200), 150_000, zipf=(1.0, 1.0, 0.0), true_rank=10,
+                      value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.01, seed=0x5EED0002)
+    R = 32
+    say(f"[sgd] train={tr.nnz}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = [0.5 * f for f in init_factors(tr.dims, R, 6)]
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    epochs = 12
+    rm_orc, _ = O.sgd(ocoo(tr), F64, 1e-3, 0.05, epochs, test=ocoo(te), scale=1.0, decay=0.95)
+    opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.05, learning_rate=1e-3,
+                     lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0, max_epochs=epochs,
+                     max_bad_epochs=0, tile_len=64)
+    res = m.solve(opt)
+    rm_gpu = [h["rmse_te"] for h in res.history[1:]]
+    say("  oracle", np.round(rm_orc, 5))
+    say("  gpu   ", np.round(rm_gpu, 5))
+    assert len(rm_gpu) == epochs
+    gap = np.max(np.abs(np.array(rm_gpu) - np.array(rm_orc)))
+    assert gap < 1e-3, gap
+    assert rm_gpu[-1] < rm_gpu[0]
+
+
+@pytest.mark.parametrize("R", [8, 16, 64, 128])
+def test_sgd_single_walker_matches_sequential(ctx, R):
+    """With one tile the GPU walk is one sequential walker: factors match the
+    sequential oracle to FP32 rounding."""
+    tr, _ = ct.synth((10, 8, 6), 64, test_fraction=0.0, seed=R)
+    say(f"[sgd_single] R={R}")
+    m = ct.Model(ctx, tr, None, None, rank=R)
+    F = init_factors(tr.dims, R, 8)
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    O.sgd_epoch(ocoo(tr), F64, 1e-2, 0.1, 1.0)
+    m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.1, sgd_scale=1.0, tile_len=64), lr=1e-2)
+    for md in range(3):
+        assert relerr(m.factor(md), F64[md]) < 1e-5
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nmodes", [3, 4])
+@pytest.mark.parametrize("per_occ", [False, True])
+def test_grad(ctx, nmodes, per_occ):
+    import torch
+    tr, _ = small_problem(nmodes, seed=30 + nmodes, test_fraction=0.0)
+    R = 32
+    say(f"[grad] N={nmodes} per_occ={per_occ}")
+    m = ct.Model(ctx, tr, None, None, rank=R)
+    F = init_factors(tr.dims, R, 9)
+    dev = torch.device("cuda:0")
+    Ft = [torch.from_numpy(f).to(dev) for f in F]
+    Gt = [torch.empty_like(f) for f in Ft]
+    torch.cuda.synchronize()
+    loss = m.grad_device([f.data_ptr() for f in Ft], [g.data_ptr() for g in Gt], 0.3, per_occ)
+    torch.cuda.synchronize()
+    oloss, OG = O.grad(ocoo(tr), [f.astype(np.float64) for f in F], 0.3, per_occ)
+    assert loss == pytest.approx(oloss, rel=1e-5)
+    for md in range(nmodes):
+        assert relerr(Gt[md].cpu().numpy(), OG[md]) < 1e-5
+
+
+# --------------------------------------------------------------------------
+def test_solve_loop_and_reference_shaped_entry(ctx):
+    tr, te = small_problem(3, nnz=20000, seed=41)
+    say("[solve_loop]")
+    m = ct.Model(ctx, tr, te, te, rank=16)
+    m.init_random(3)
+    res = m.solve(ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.05, max_epochs=30))
+    assert res.epochs >= 2 and res.history[0]["rmse_vl"] > res.best_rmse
+    assert res.ms_per_epoch > 0 and res.nnz_per_s > 0
+    best = m.factors(best=True)
+    m2 = ct.Model(ctx, tr, te, te, rank=16)
+    m2.set_factors(best)
+    assert m2.evaluate()["rmse_vl"] == pytest.approx(res.best_rmse, rel=1e-9)
+    mats = [np.ascontiguousarray(f, dtype=np.float64) for f in init_factors(tr.dims, 16, 2)]
+    bm = [np.zeros_like(f) for f in mats]
+    out = ct.tc_als(tr, None, None, te, te, mats, bm, regularization_index=0.05)
+    assert out["best_rmse"] < 2.0 and np.abs(bm[0]).sum() > 0
+
+
+# --------------------------------------------------------------------------
+# CCD++
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("R,tile_len,inner,nmodes", [
+    (8, 64, 1, 3), (32, 256, 1, 3), (32, 4096, 2, 3), (16, 128, 1, 4), (64, 512, 1, 3)])
+def test_ccd_epochs(ctx, R, tile_len, inner, nmodes):
+    """Canonical CCD++: every factor within 1e-5 of the FP64 oracle after each
+    of 4 epochs, from the same initial factors."""
+    tr, te = small_problem(nmodes, nnz=25000, seed=50 + R + nmodes, test_fraction=0.05)
+    say(f"[ccd] R={R} tile_len={tile_len} inner={inner} N={nmodes}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 12)
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    oc = ocoo(tr)
+    orders = [O.mode_order(oc, md) for md in range(nmodes)]
+    res = O.residual(oc, F64)
+    opt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=0.05, tile_len=tile_len, inner_iters=inner)
+    for ep in range(4):
+        m.ccd_epoch(opt)
+        O.ccd_epoch(oc, F64, 0.05, res, inner=inner, orders=orders)
+        errs = [relerr(m.factor(md), F64[md]) for md in range(nmodes)]
+        say(f"  epoch {ep} errs {['%.1e' % e for e in errs]}")
+        assert max(errs) < 1e-5, errs
+
+
+def test_ccd_deterministic_and_solve(ctx):
+    tr, te = small_problem(3, nnz=20000, seed=61)
+    say("[ccd_det]")
+    outs = []
+    for _ in range(2):
+        m = ct.Model(ctx, tr, te, te, rank=16)
+        m.set_factors(init_factors(tr.dims, 16, 3))
+        res = m.solve(ct.Options(algorithm=ct.CCD, rank=16, lambda_=0.05, max_epochs=5, max_bad_epochs=0))
+        outs.append(m.factors())
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert res.history[-1]["rmse_tr"] < res.history[0]["rmse_tr"]

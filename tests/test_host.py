@@ -1,0 +1,207 @@
+"""CPU tests of libctc's host side: the C ABI exports, the .tns reader, the
+synthetic generator and the CISS builder (checked against the oracle)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cutc_amd as ct
+import oracle as O
+from cutc_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "ctc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ctc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+    # and the ctypes table covers the header exactly
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_status_strings():
+    L = _lib.lib()
+    for code in range(10):
+        assert L.ctc_status_string(code)
+    assert L.ctc_version().startswith(b"ctc")
+
+
+@pytest.mark.parametrize("name", ["base1.tns", "base0.tns", "fourway.tns", "badbase.tns"])
+def test_tns_reader_matches_oracle(name):
+    path = os.path.join(GOLD, "tns", name)
+    a = ct.tt_read(path)
+    b = O.tns_read(path)
+    assert a.dims == b.dims and a.nnz == b.nnz
+    assert a.base == b.offset
+    for m in range(a.nmodes):
+        assert np.array_equal(a.ind(m), b.ind[m])
+    assert np.array_equal(a.vals, b.vals)
+
+
+def test_tns_reader_errors(tmp_path):
+    with pytest.raises(ct.CtcError) as e:
+        ct.tt_read(tmp_path / "missing.tns")
+    assert e.value.status == 2
+    p = tmp_path / "empty.tns"
+    p.write_text("# nothing\n\n")
+    with pytest.raises(ct.CtcError) as e:
+        ct.tt_read(p)
+    assert e.value.status == 3
+    p = tmp_path / "bad.tns"
+    p.write_text("1 2 3 4.0\n1 x 3 2.0\n")
+    with pytest.raises(ct.CtcError) as e:
+        ct.tt_read(p)
+    assert e.value.status == 3
+
+
+def test_tns_write_roundtrip(tmp_path):
+    tr, _ = ct.synth((30, 20, 10), 500, test_fraction=0.0, seed=3)
+    p = tmp_path / "x.tns"
+    tr.write(p, base=1)
+    b = ct.tt_read(p)
+    assert b.base == (1, 1, 1)
+    for m in range(3):
+        assert np.array_equal(b.ind(m), tr.ind(m))
+    assert np.array_equal(b.vals, tr.vals)
+    tr.write(p, base=0, decimals=1)  # tvtensor.py-style %.1f
+    c = ct.tt_read(p)
+    assert np.allclose(c.vals, np.round(tr.vals, 1), atol=1e-12)
+
+
+def test_synth_is_seeded_distinct_and_exact():
+    a, at = ct.synth((200, 150, 30), 20000, zipf=(1.0, 1.0, 0.0), true_rank=4,
+                     value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.05, seed=7)
+    b, bt = ct.synth((200, 150, 30), 20000, zipf=(1.0, 1.0, 0.0), true_rank=4,
+                     value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.05, seed=7)
+    assert a.nnz + at.nnz == 20000 and at.nnz == 1000
+    for m in range(3):
+        assert np.array_equal(a.ind(m), b.ind(m))
+    assert np.array_equal(a.vals, b.vals)
+    keys = np.concatenate([(a.ind(0).astype(np.int64) * 150 + a.ind(1)) * 30 + a.ind(2),
+                           (at.ind(0).astype(np.int64) * 150 + at.ind(1)) * 30 + at.ind(2)])
+    assert np.unique(keys).size == 20000
+    ka = (a.ind(0).astype(np.int64) * 150 + a.ind(1)) * 30 + a.ind(2)
+    assert np.all(np.diff(ka) > 0)  # train sorted by mode 0
+    assert set(np.unique(a.vals)) <= {1.0, 2.0, 3.0, 4.0, 5.0}
+    # Zipf skew on mode 0: row 0 is the heaviest
+    c = np.bincount(a.ind(0), minlength=200)
+    assert c[0] == c.max() and c[0] > 5 * np.median(c)
+    d, _ = ct.synth((200, 150, 30), 20000, seed=8, test_fraction=0.0)
+    assert not np.array_equal(d.ind(0), a.ind(0)) or not np.array_equal(d.ind(1), a.ind(1))
+
+
+def test_synth_affine_range():
+    tr, te = ct.synth((100, 80, 60), 5000, true_rank=8, value_map=ct.VALUES_AFFINE_1_5,
+                      noise_std=0.0, test_fraction=0.1, seed=1)
+    v = np.concatenate([tr.vals, te.vals])
+    assert v.min() == pytest.approx(1.0) and v.max() == pytest.approx(5.0)
+
+
+def _oracle_coo(t):
+    ind, vals = t.arrays()
+    return O.Coo(ind, vals, t.dims)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_ciss_native_structure(mode):
+    tr, _ = ct.synth((300, 200, 50), 30000, zipf=(1.0, 0.5, 0.0), test_fraction=0.0, seed=11)
+    c = ct.ciss_alloc(tr, mode)
+    oc = _oracle_coo(tr)
+    perm, rowptr = O.mode_order(oc, mode)
+    assert np.array_equal(c.perm, perm)
+    rows = tr.ind(mode)[c.perm]
+    np.testing.assert_array_equal(c.directory, np.unique(rows))
+    np.testing.assert_array_equal(np.diff(c.rowptr), np.bincount(rows)[np.unique(rows)])
+    for s in range(2):
+        np.testing.assert_array_equal(c.oidx(s), tr.ind((mode + 1 + s) % 3)[c.perm])
+    np.testing.assert_array_equal(c.vals, tr.vals[c.perm].astype(np.float32))
+    # tile headers: slot of each nonzero from (tile_slot, tile_mask) equals the
+    # slot from the row pointers
+    slot_true = np.repeat(np.arange(c.dlength), np.diff(c.rowptr))
+    n = c.nnz
+    heads = np.zeros(n, bool)
+    heads[c.rowptr[:-1]] = True
+    for t in range(c.ntiles):
+        b = t * 64
+        e = min(n, b + 64)
+        mask = int(c.tile_mask[t])
+        exp_mask = sum(1 << (p - b) for p in range(b, e) if heads[p])
+        assert mask == exp_mask
+        assert c.tile_slot[t] == slot_true[b]
+
+
+@pytest.mark.parametrize("L", [1, 3, 8, 32, 52])
+def test_ciss_reference_layout_equals_oracle(L):
+    tr, _ = ct.synth((40, 30, 20), 2000, zipf=(1.0, 0.0, 0.0), test_fraction=0.0, seed=5)
+    oc = _oracle_coo(tr)
+    for mode in range(3):
+        c = ct.ciss_alloc(tr, mode)
+        d, dc, ent = c.reference_layout(L)
+        od, odc, oent = O.ciss_reference_layout(O.sort_by_mode(oc, mode), mode, L)
+        assert np.array_equal(d, od)
+        assert np.array_equal(dc, odc)
+        same = (ent == oent) | (np.isnan(ent) & np.isnan(oent))
+        assert same.all()
+        z = (oent == 0)
+        assert np.array_equal(np.signbit(ent[z]), np.signbit(oent[z]))
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_ciss_reference_layout_hand_worked(idx):
+    import json
+    c = json.load(open(os.path.join(GOLD, "ciss_cases.json")))["cases"][idx]
+    a = np.array(c["coo"], dtype=np.float64)
+    t = ct.Coo.from_arrays([a[:, m].astype(np.int64) - 1 for m in range(3)], a[:, 3], c["dims"])
+    d, dc, ent = ct.ciss_alloc(t, c["mode"]).reference_layout(c["tile_len"])
+    assert list(d) == c["directory"] and list(dc) == c["dcounter"]
+    exp = np.array([[np.nan if v is None else v for v in r] for r in c["entries"]])
+    assert ((exp == ent) | (np.isnan(exp) & np.isnan(ent))).all()
+
+
+def test_ciss_unsorted_input_is_sorted_internally():
+    rng = np.random.default_rng(2)
+    tr, _ = ct.synth((50, 40, 30), 3000, test_fraction=0.0, seed=2)
+    ind, vals = tr.arrays()
+    p = rng.permutation(tr.nnz)
+    sh = ct.Coo.from_arrays([a[p] for a in ind], vals[p], tr.dims)
+    for mode in range(3):
+        a = ct.ciss_alloc(tr, mode)
+        b = ct.ciss_alloc(sh, mode)
+        assert np.array_equal(a.directory, b.directory)
+        assert np.array_equal(a.oidx(0), b.oidx(0)) and np.array_equal(a.vals, b.vals)
+
+
+def test_ciss_nnz_multiple_of_64_and_single_row():
+    t = ct.Coo.from_arrays([np.zeros(128, np.int64), np.arange(128) % 16, np.arange(128) // 16],
+                           np.arange(128, dtype=float), (1, 16, 8))
+    c = ct.ciss_alloc(t, 0)
+    assert c.ntiles == 2 and c.dlength == 1
+    assert list(c.tile_slot) == [0, 0]
+    assert list(c.tile_mask) == [1, 0]
+
+
+def test_options_defaults_follow_reference():
+    for alg, lam in ((ct.ALS, 10.0), (ct.SGD, 1e-3), (ct.CCD, 100.0)):
+        o = ct.Options(algorithm=alg).to_c()
+        assert o.lambda_ == lam and o.rank == 16 and o.tolerance == 1e-4 and o.max_bad_epochs == 3
+    assert ct.Options(algorithm=ct.SGD).to_c().sgd_scale == 1e-5
+
+
+def test_context_without_gpu_reports_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(ct.CtcError) as e:
+        ct.Context(0)
+    assert e.value.status in (6, 9)
