@@ -23,8 +23,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
-                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, VALUES_AFFINE_1_5, VALUES_INT_1_5,
-                   VALUES_RAW, CtcError, check, lib)
+                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_HOGWILD, SGD_PRIVATE, SGD_RACY,
+                   VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
 __all__ = ["Coo", "Ciss", "Context", "Model", "Options", "synth", "tt_read", "ciss_alloc",
            "tc_main_ciss", "tc_als", "tc_sgd", "tc_ccd", "CtcError", "ALS", "SGD", "CCD"]
@@ -236,6 +236,8 @@ class Options:
     refine: int = 0
     eval_every_epoch: int = 1
     verbose: int = 0
+    sgd_update: int = SGD_HOGWILD
+    spread: int = 1
 
     def to_c(self):
         o = _lib.Options()
@@ -265,6 +267,9 @@ class Context:
         s = C.c_void_p()
         check(lib().ctc_context_stream(self._h, C.byref(s)))
         return s.value
+
+    def synchronize(self):
+        check(lib().ctc_context_synchronize(self._h))
 
     def set_profiling(self, on=True):
         check(lib().ctc_context_set_profiling(self._h, 1 if on else 0))

@@ -15,6 +15,7 @@ ALS, SGD, CCD = 0, 1, 2
 LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
 ORDER_FIXED, ORDER_ROTATE = 0, 1
+SGD_HOGWILD, SGD_RACY, SGD_PRIVATE = 0, 1, 2
 VALUES_RAW, VALUES_AFFINE_1_5, VALUES_INT_1_5 = 0, 1, 2
 
 STATUS = {
@@ -94,6 +95,8 @@ class Options(C.Structure):
         ("refine", C.c_int32),
         ("eval_every_epoch", C.c_int32),
         ("verbose", C.c_int32),
+        ("sgd_update", C.c_int32),
+        ("spread", C.c_int32),
     ]
 
 
@@ -163,6 +166,7 @@ SIGNATURES = {
     "ctc_context_create": (C.c_int, [C.c_int, P(vp)]),
     "ctc_context_destroy": (None, [vp]),
     "ctc_context_stream": (C.c_int, [vp, P(vp)]),
+    "ctc_context_synchronize": (C.c_int, [vp]),
     "ctc_context_set_profiling": (C.c_int, [vp, C.c_int]),
     "ctc_context_kernel_stats": (C.c_int, [vp, C.c_char_p, P(C.c_uint64), P(C.c_double)]),
     "ctc_context_reset_stats": (C.c_int, [vp]),
@@ -215,10 +219,26 @@ def alive():
     return not _shutting_down
 
 
+def _share_hip_runtime_with_torch():
+    """libctc and PyTorch-ROCm must share one HIP runtime per process.  Both
+    runtimes carry the SONAME libamdhip64.so.7, but torch's libraries ask for
+    the unversioned name, so if libctc loaded /opt/rocm's runtime first torch
+    would load a second copy that cannot see the GPU.  Importing torch first
+    (when installed) makes libctc bind to torch's already-loaded runtime.
+    Set CTC_NO_TORCH_PRELOAD=1 to skip (torch must then not be used)."""
+    if os.environ.get("CTC_NO_TORCH_PRELOAD") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded libctc.  Raises if the library is missing (no fallback)."""
     global _lib
     if _lib is None:
+        _share_hip_runtime_with_torch()
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libctc.so not built at {LIB_PATH}: run `python -m cutc_amd.build`")
         L = C.CDLL(LIB_PATH)

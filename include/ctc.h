@@ -146,6 +146,8 @@ ctc_status ctc_context_create(int device, ctc_context **out);
 void ctc_context_destroy(ctc_context *ctx);
 /* The context's hipStream_t. */
 ctc_status ctc_context_stream(ctc_context *ctx, void **stream);
+/* Waits for all work queued on the context's stream. */
+ctc_status ctc_context_synchronize(ctc_context *ctx);
 /* Per-kernel timing with HIP events on the context stream (off by default). */
 ctc_status ctc_context_set_profiling(ctc_context *ctx, int enabled);
 ctc_status ctc_context_kernel_stats(ctc_context *ctx, const char *kernel, uint64_t *launches,
@@ -169,6 +171,15 @@ typedef enum ctc_precision {
   CTC_PREC_MIXED = 1  /* BF16 MFMA Gram/MTTKRP, FP32 accumulate; FP64 Cholesky + 1 refinement */
 } ctc_precision;
 typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mode_order;
+/* How concurrent SGD walkers combine updates of a shared factor row. */
+typedef enum ctc_sgd_update {
+  CTC_SGD_HOGWILD = 0, /* re-read rows every nonzero, add each update with atomics
+                          (SG/sgd.cu:96-98 with fresh reads) */
+  CTC_SGD_RACY = 1,    /* keep rows in registers, store after every nonzero
+                          (concurrent writers overwrite each other) */
+  CTC_SGD_PRIVATE = 2  /* keep rows in registers, add the run's delta with
+                          atomics when the run ends */
+} ctc_sgd_update;
 
 typedef struct ctc_options {
   int32_t algorithm;       /* ctc_algorithm; SG/tc.c:22 */
@@ -191,6 +202,9 @@ typedef struct ctc_options {
   int32_t refine;          /* extra iterative-refinement step in the ALS solve */
   int32_t eval_every_epoch;/* evaluate RMSE after every epoch (convergence needs it) */
   int32_t verbose;         /* print the reference's progress line per epoch */
+  int32_t sgd_update;      /* ctc_sgd_update */
+  int32_t spread;          /* 1: walkers visit spans in a strided order that
+                              spreads a hot row's spans over the sweep */
 } ctc_options;
 
 /* Reference defaults for an algorithm (SG/base.h, SG/completion.c:460-480). */

@@ -285,3 +285,75 @@ def test_ccd_deterministic_and_solve(ctx):
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
     assert res.history[-1]["rmse_tr"] < res.history[0]["rmse_tr"]
+
+
+# --------------------------------------------------------------------------
+# differentiation: autograd.Function over the adjoint kernel
+# --------------------------------------------------------------------------
+def test_autograd_finite_differences_and_sgd_direction(ctx):
+    """4-way tensor shaped like the differentiation configuration (Zipf modes
+    0-1, uniform 2-3), gradient through torch.autograd against central finite
+    differences (eps = 1e-3, FP64 oracle loss) on a 1000-nonzero subsample,
+    and against the summed reference SGD step (per-occurrence L2)."""
+    import torch
+    from cutc_amd.autograd import completion_loss
+    tr, _ = ct.synth((2000, 1000, 100, 24), 1000, zipf=(1.0, 1.0, 0.0, 0.0), true_rank=16,
+                     value_map=ct.VALUES_RAW, noise_std=0.2, test_fraction=0.0, seed=0x5EED0003)
+    say("[autograd] nnz", tr.nnz)
+    R, lam = 32, 0.1
+    m = ct.Model(ctx, tr, None, None, rank=R)
+    F = [0.3 * f for f in init_factors(tr.dims, R, 13)]
+    dev = torch.device("cuda:0")
+    oc = ocoo(tr)
+    for per_occ in (False, True):
+        Ft = [torch.from_numpy(f).to(dev).requires_grad_(True) for f in F]
+        loss = completion_loss(m, Ft, lam, per_occ)
+        loss.backward()
+        G = [f.grad.detach().cpu().numpy().astype(np.float64) for f in Ft]
+        F64 = [f.astype(np.float64) for f in F]
+        oloss, _ = O.grad(oc, F64, lam, per_occ)
+        assert float(loss) == pytest.approx(oloss, rel=1e-5)
+        rng = np.random.default_rng(5)
+        eps = 1e-3
+        for md in range(4):
+            rows = np.unique(tr.ind(md))
+            for _ in range(5):
+                i, r = int(rng.choice(rows)), int(rng.integers(R))
+                Fp = [f.copy() for f in F64]
+                Fm = [f.copy() for f in F64]
+                Fp[md][i, r] += eps
+                Fm[md][i, r] -= eps
+                fd = (O.grad(oc, Fp, lam, per_occ)[0] - O.grad(oc, Fm, lam, per_occ)[0]) / (2 * eps)
+                assert G[md][i, r] == pytest.approx(fd, rel=1e-4, abs=1e-5), (md, i, r, per_occ)
+        if per_occ:
+            # summed reference SGD direction (SG/sgd.cu:93-95) = -gradient
+            acc = [np.zeros_like(f) for f in F64]
+            ind = [tr.ind(q) for q in range(4)]
+            for p in range(tr.nnz):
+                rows = [F64[q][ind[q][p]] for q in range(4)]
+                e = tr.vals[p] - np.sum(rows[0] * rows[1] * rows[2] * rows[3])
+                for q in range(4):
+                    o = np.prod([rows[s] for s in range(4) if s != q], axis=0)
+                    acc[q][ind[q][p]] += e * o - lam * rows[q]
+            for q in range(4):
+                assert relerr(-G[q], acc[q]) < 1e-5
+
+
+def test_autograd_training_decreases_loss(ctx):
+    import torch
+    from cutc_amd.autograd import completion_loss
+    tr, te = ct.synth((400, 300, 50, 24), 60000, zipf=(1.0, 1.0, 0.0, 0.0), true_rank=16,
+                      value_map=ct.VALUES_RAW, noise_std=0.2, test_fraction=0.01, seed=0x5EED0003)
+    say("[autograd_train]")
+    m = ct.Model(ctx, tr, None, te, rank=32)
+    dev = torch.device("cuda:0")
+    Ft = [torch.from_numpy(0.5 * f).to(dev).requires_grad_(True) for f in init_factors(tr.dims, 32, 2)]
+    opt = torch.optim.SGD(Ft, lr=2e-4, momentum=0.9)
+    losses = []
+    for _ in range(30):
+        opt.zero_grad()
+        loss = completion_loss(m, Ft, 0.05)
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    assert losses[-1] < 0.5 * losses[0], losses[::5]
