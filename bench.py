@@ -53,7 +53,6 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--algs", default="sgd,ccd,als")
-    ap.add_argument("--trace-rmse", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -72,55 +71,57 @@ def main():
     log(f"synthetic tensor: train {train.nnz} test {test.nnz} in {time.time() - t0:.1f}s")
     ctx = ct.Context(0)
     model = ct.Model(ctx, train, None, test, rank=R)
-    model.init_random(1)
     ctx.set_profiling(True)
     algs = set(a.algs.split(","))
     out_algs = {}
 
     # ------------------------------------------------------------------ SGD
-    sgd_tile = a.tile_len or 64
+    rng = np.random.default_rng(1)  # the oracle trajectory fixture starts from this init
+    F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
+    model.set_factors(F0)
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3,
-                     lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0, tile_len=sgd_tile)
+                     lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0,
+                     tile_len=a.tile_len or None)
     t0 = time.time()
     model.prepare(opt)
     log(f"SGD CISS build+upload {time.time() - t0:.1f}s")
     rm0 = model.evaluate(lam)["rmse_te"]
-    lr = 1e-3
-    for _ in range(a.warmup):
-        model.sgd_epoch(opt, lr=lr)
+    lr, times, traj = 1e-3, [], []
+    for e in range(a.warmup + a.steps):
+        if e == a.warmup:
+            ctx.reset_stats()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        model.sgd_epoch(opt, lr=lr)  # returns after the stream is drained
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        if e >= a.warmup:
+            times.append(dt)
         lr *= 0.95
-    ctx.reset_stats()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        model.sgd_epoch(opt, lr=lr)
-        lr *= 0.95
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    if a.trace_rmse:  # untimed: per-epoch test RMSE of a second run from the same start
-        model.init_random(1)
-        lr2, trace = 1e-3, []
-        for _ in range(a.warmup + a.steps):
-            model.sgd_epoch(opt, lr=lr2)
-            lr2 *= 0.95
-            trace.append(round(model.evaluate(lam)["rmse_te"], 5))
-        log("SGD test RMSE per epoch:", trace)
+        traj.append(model.evaluate(lam)["rmse_te"])  # untimed
     n_sgd, ms_sgd = ctx.kernel_stats("sgd")
-    ev = model.evaluate(lam)
-    wall = t1 - t0
+    wall = sum(times)
     value = train.nnz * a.steps / wall
     kern_ms = ms_sgd / max(n_sgd, 1)
     alg_bytes_per_nnz = 12 + 24 * R
     achieved = alg_bytes_per_nnz * train.nnz / (kern_ms * 1e-3) / 1e9
+    ref = None
+    fx = os.path.join(ROOT, "tests", "golden", "sgd_cfg2_oracle.json")
+    if a.nnz == 100_000_000 and R == 32 and os.path.exists(fx):
+        ref = json.load(open(fx))["rmse_te"]
+    gap = max(abs(x - y) for x, y in zip(traj, ref)) if ref else None
     out_algs["sgd"] = {"s_per_epoch": wall / a.steps, "nnz_per_s": value, "kernel_ms": kern_ms,
-                       "rmse_te_initial": rm0, "rmse_te": ev["rmse_te"], "epochs": a.warmup + a.steps,
-                       "tile_len": sgd_tile, "alg_bytes_per_nnz": alg_bytes_per_nnz}
+                       "rmse_te_initial": rm0, "rmse_te_per_epoch": [round(x, 5) for x in traj],
+                       "oracle_fp64_rmse_te_per_epoch": [round(x, 5) for x in ref] if ref else None,
+                       "max_abs_gap_vs_oracle": gap, "epochs": a.warmup + a.steps,
+                       "tile_len": opt.to_c().tile_len, "policy": "auto", "alg_bytes_per_nnz": alg_bytes_per_nnz}
     log(f"SGD {wall / a.steps * 1e3:.2f} ms/epoch, kernel {kern_ms:.2f} ms, {value / 1e9:.2f} Gnnz/s, "
-        f"rmse_te {rm0:.4f} -> {ev['rmse_te']:.4f}")
+        f"rmse_te {rm0:.4f} -> {traj[-1]:.4f}; max gap vs FP64 sequential oracle {gap}")
 
     # --------------------------------------------------------------- CCD++
     if "ccd" in algs and a.ccd_epochs > 0:
-        copt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=lam, tile_len=a.tile_len or 256)
+        copt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=lam, tile_len=a.tile_len or None)
+        model.set_factors(F0)
         t0 = time.time()
         model.prepare(copt)
         log(f"CCD CISS build+upload {time.time() - t0:.1f}s")
@@ -146,7 +147,8 @@ def main():
 
     # ----------------------------------------------------------------- ALS
     if "als" in algs and a.als_epochs > 0:
-        aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=a.tile_len or 1024)
+        aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=a.tile_len or None)
+        model.set_factors(F0)
         t0 = time.time()
         model.prepare(aopt)
         log(f"ALS CISS build+upload {time.time() - t0:.1f}s")

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
-                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_HOGWILD, SGD_PRIVATE, SGD_RACY,
+                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_RACY,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
 __all__ = ["Coo", "Ciss", "Context", "Model", "Options", "synth", "tt_read", "ciss_alloc",
@@ -227,7 +227,7 @@ class Options:
     max_epochs: int = None
     tolerance: float = 1e-4
     max_bad_epochs: int = 3
-    tile_len: int = 256
+    tile_len: int = None
     block: int = 256
     seed: int = 0
     mode_order: int = ORDER_FIXED
@@ -236,8 +236,8 @@ class Options:
     refine: int = 0
     eval_every_epoch: int = 1
     verbose: int = 0
-    sgd_update: int = SGD_HOGWILD
-    spread: int = 1
+    sgd_update: int = SGD_AUTO
+    spread: int = 0
 
     def to_c(self):
         o = _lib.Options()

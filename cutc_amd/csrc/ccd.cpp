@@ -89,7 +89,6 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
   }
   for (uint32_t md = 0; md < N; ++md)
     CTC_HIP(launch_transpose(st, m->FT[md], m->F[md], m->dims[md], (int)R, false));
-  m->rep_valid = false;
   return CTC_OK;
 }
 

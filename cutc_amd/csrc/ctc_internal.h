@@ -134,10 +134,7 @@ struct ctc_model {
   size_t colsave_elems = 0;
   float *FT[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};  // column-major mirrors
   int ccd_pending = -1;  // column whose new component the residual copies still hold
-  // per-XCD factor replicas for the replicated SGD sweep
-  float *rep[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
-  uint8_t *touch[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
-  bool rep_valid = false;
+
 };
 
 namespace ctc {

@@ -27,38 +27,23 @@ namespace ctc {
 // SGD epoch (reference rule SG/sgd.cu:75-99: every row of a nonzero is
 // updated from its pre-update values; step = lr * scale).
 //
-// Conflict policy between concurrent walkers (V):
-//   SGD_HOGWILD  every nonzero re-reads its rows with agent-scope loads (so
-//                concurrent updates are visible) and adds its update with
-//                device-scope float atomics: the GPU form of Hogwild!
-//                (Recht et al. 2011), the reference's atomicAdd scheme
-//                (SG/sgd.cu:96-98) with fresh reads.
-//   SGD_RACY     a walker keeps the rows of its current run in registers,
-//                applies its own updates there and stores the rows after every
-//                nonzero with plain stores; concurrent writers of a hot row
-//                overwrite each other (lost updates) instead of summing steps
-//                computed from the same stale value.
-//   SGD_PRIVATE  registers as RACY, but the walker adds its accumulated delta
-//                with atomics when its run on a row ends.  Exact for rows one
-//                walker owns; on rows shared by many concurrent walkers the
-//                summed deltas overshoot.
-// Walker w processes span (w * stride) mod nspans, so with a stride near
-// nspans / golden-ratio the concurrently resident walkers are spread over the
-// whole tensor and a hot row's spans run at different times.
+// A walker keeps the rows of its current nonzero in registers.  How a row it
+// shares with concurrent walkers is read and written is chosen per mode
+// (PA for the CISS mode, PB for the next mode, PC for the others):
+//   P_RACY     read at the start of a run of equal indices, apply own updates
+//              in registers, store the row after every nonzero (concurrent
+//              writers overwrite each other).
+//   P_ATOMIC   read every nonzero, add every update with a device-scope float
+//              atomic (the reference's atomicAdd, SG/sgd.cu:96-98).
+//   P_RUN      read at the start of a run, apply own updates in registers, add
+//              the run's accumulated delta with atomics when the run ends.
+// Walker w processes span (w * stride) mod nspans (stride 1 = natural order).
 // ==========================================================================
-enum { SGD_HOGWILD = 0, SGD_RACY = 1, SGD_PRIVATE = 2, SGD_MIXED = 3, SGD_MIXED_L2 = 4, SGD_XCD = 5 };
+enum { P_RACY = 0, P_ATOMIC = 1, P_RUN = 2 };
 
-__device__ inline float agent_load(const float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// load that skips the CU's L1 (reads the L2)
-__device__ inline float l2_load(const float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-template <int R, int N, int V>
-__global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, SgdReplicas rp, uint32_t span,
-                                             uint64_t nspans, uint64_t stride, float step, float lam) {
+template <int R, int N, int PA, int PB, int PC>
+__global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, uint32_t span, uint64_t nspans,
+                                             uint64_t stride, float step, float lam) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int WPW = 64 / W;
@@ -74,30 +59,25 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, SgdRepli
   const uint64_t s0 = sp * span;
   const uint64_t s1 = !live ? 0 : (s0 + span < c.nnz ? s0 + span : c.nnz);
 
-  // SGD_XCD: workgroups are dispatched round-robin over the XCDs, so
-  // replica x = blockIdx % nrep keeps each replica's rows in one XCD's L2.
-  const uint32_t rx = V == SGD_XCD ? blockIdx.x % rp.nrep : 0;
-  float *A = fa.f[0] + (size_t)rx * rp.elems[0];
-  float *Bf[N - 1];
+  // row slots: 0 = CISS mode (A), 1.. = modes mode+1, ... (B, C, D)
+  float *Fm[N];
 #pragma unroll
-  for (int s = 0; s < N - 1; ++s) Bf[s] = fa.f[1 + s] + (size_t)rx * rp.elems[1 + s];
-  uint8_t *tA = V == SGD_XCD ? rp.touch[0] + (size_t)rx * rp.rows[0] : nullptr;
-  uint8_t *tB[N - 1];
+  for (int s = 0; s < N; ++s) Fm[s] = fa.f[s];
+  float v[N][CPL], del[N][CPL];
+  uint32_t cur[N];
 #pragma unroll
-  for (int s = 0; s < N - 1; ++s) tB[s] = V == SGD_XCD ? rp.touch[1 + s] + (size_t)rx * rp.rows[1 + s] : nullptr;
-
-  float a[CPL], adel[CPL];
-  float b[N - 1][CPL], bdel[N - 1][CPL];
-  uint32_t cur[N - 1];
-  uint32_t arow = NONE, slot = 0;
-#pragma unroll
-  for (int s = 0; s < N - 1; ++s) {
+  for (int s = 0; s < N; ++s) {
     cur[s] = NONE;
 #pragma unroll
-    for (int q = 0; q < CPL; ++q) bdel[s][q] = 0.f, b[s][q] = 0.f;
+    for (int q = 0; q < CPL; ++q) v[s][q] = 0.f, del[s][q] = 0.f;
   }
+  uint32_t slot = 0;
+
+  auto pol = [](int s) { return s == 0 ? PA : (s == 1 ? PB : PC); };
+  auto flush_run = [&](int s) {
 #pragma unroll
-  for (int q = 0; q < CPL; ++q) a[q] = 0.f, adel[q] = 0.f;
+    for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(Fm[s] + (size_t)cur[s] * R + wl + q2 * W, del[s][q2]);
+  };
 
   for (uint32_t ch = 0; ch < span; ch += W) {
     const uint64_t el = s0 + ch + wl;
@@ -117,62 +97,29 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, SgdRepli
     const uint64_t hm = __ballot(vl && hl);
     for (int q = 0; q < W; ++q) {
       const int src = sub * W + q;
-      uint32_t idx[N - 1];
+      uint32_t idx[N];
 #pragma unroll
-      for (int s = 0; s < N - 1; ++s) idx[s] = (uint32_t)__shfl((int)il[s], src, 64);
+      for (int s = 0; s < N - 1; ++s) idx[1 + s] = (uint32_t)__shfl((int)il[s], src, 64);
       const float x = __shfl(xl, src, 64);
       const uint64_t e = s0 + ch + q;
       if (e < s1) {
         const bool head = (hm >> src) & 1ull;
-        if (arow == NONE || head) {
-          if (V == SGD_PRIVATE && arow != NONE) {
-#pragma unroll
-            for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(A + (size_t)arow * R + wl + q2 * W, adel[q2]);
-          }
-          slot = arow == NONE ? c.tile_slot[s0 >> 6] : slot + 1;
-          arow = c.directory[slot];
-          if (V != SGD_HOGWILD) {
-#pragma unroll
-            for (int q2 = 0; q2 < CPL; ++q2) {
-              const float *pa = A + (size_t)arow * R + wl + q2 * W;
-              a[q2] = V == SGD_XCD ? l2_load(pa) : *pa;
-              adel[q2] = 0.f;
-            }
-            if (V == SGD_XCD && wl == 0) tA[arow] = 1;
-          }
-        }
-        if (V == SGD_HOGWILD) {
-#pragma unroll
-          for (int q2 = 0; q2 < CPL; ++q2) a[q2] = agent_load(A + (size_t)arow * R + wl + q2 * W);
-#pragma unroll
-          for (int s = 0; s < N - 1; ++s)
-#pragma unroll
-            for (int q2 = 0; q2 < CPL; ++q2) b[s][q2] = agent_load(Bf[s] + (size_t)idx[s] * R + wl + q2 * W);
-        } else if (V == SGD_MIXED || V == SGD_MIXED_L2) {
-#pragma unroll
-          for (int s = 0; s < N - 1; ++s)
-#pragma unroll
-            for (int q2 = 0; q2 < CPL; ++q2) {
-              const float *pb = Bf[s] + (size_t)idx[s] * R + wl + q2 * W;
-              b[s][q2] = V == SGD_MIXED_L2 ? l2_load(pb) : *pb;
-            }
+        if (cur[0] == NONE || head) {
+          slot = cur[0] == NONE ? c.tile_slot[s0 >> 6] : slot + 1;
+          idx[0] = c.directory[slot];
         } else {
+          idx[0] = cur[0];
+        }
 #pragma unroll
-          for (int s = 0; s < N - 1; ++s) {
-            if (idx[s] != cur[s]) {
-              if (V == SGD_PRIVATE && cur[s] != NONE) {
+        for (int s = 0; s < N; ++s) {
+          const int P = pol(s);
+          if (P == P_ATOMIC || idx[s] != cur[s]) {
+            if (P == P_RUN && cur[s] != NONE && idx[s] != cur[s]) flush_run(s);
+            cur[s] = idx[s];
 #pragma unroll
-                for (int q2 = 0; q2 < CPL; ++q2)
-                  atomic_add_f32(Bf[s] + (size_t)cur[s] * R + wl + q2 * W, bdel[s][q2]);
-              }
-              cur[s] = idx[s];
-#pragma unroll
-              for (int q2 = 0; q2 < CPL; ++q2) {
-                const float *pb = Bf[s] + (size_t)idx[s] * R + wl + q2 * W;
-                b[s][q2] = V == SGD_XCD ? l2_load(pb) : *pb;
-                bdel[s][q2] = 0.f;
-              }
-              if (V == SGD_XCD && wl == 0) tB[s][idx[s]] = 1;
+            for (int q2 = 0; q2 < CPL; ++q2) {
+              v[s][q2] = Fm[s][(size_t)idx[s] * R + wl + q2 * W];
+              del[s][q2] = 0.f;
             }
           }
         }
@@ -180,64 +127,45 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, SgdRepli
         float part = 0.f;
 #pragma unroll
         for (int q2 = 0; q2 < CPL; ++q2) {
-          float v = a[q2];
+          float pv = v[0][q2];
 #pragma unroll
-          for (int s = 0; s < N - 1; ++s) v *= b[s][q2];
-          part += v;
+          for (int s = 1; s < N; ++s) pv *= v[s][q2];
+          part += pv;
         }
         const float err = x - group_sum<W>(part);
 #pragma unroll
         for (int q2 = 0; q2 < CPL; ++q2) {
           // products of all rows but one, from pre-update values
-          float prod_b = 1.f;
+          float oth[N];
 #pragma unroll
-          for (int s = 0; s < N - 1; ++s) prod_b *= b[s][q2];
-          float oth[N - 1];  // a * product of the other non-own rows
+          for (int s = 0; s < N; ++s) {
+            float o = 1.f;
 #pragma unroll
-          for (int s = 0; s < N - 1; ++s) {
-            float o = a[q2];
-#pragma unroll
-            for (int s2 = 0; s2 < N - 1; ++s2)
-              if (s2 != s) o *= b[s2][q2];
+            for (int s2 = 0; s2 < N; ++s2)
+              if (s2 != s) o *= v[s2][q2];
             oth[s] = o;
           }
-          const float da = step * (err * prod_b - lam * a[q2]);
           const int col = wl + q2 * W;
 #pragma unroll
-          for (int s = 0; s < N - 1; ++s) {
-            const float db = step * (err * oth[s] - lam * b[s][q2]);
-            if (V == SGD_HOGWILD || V == SGD_MIXED || V == SGD_MIXED_L2) {
-              atomic_add_f32(Bf[s] + (size_t)idx[s] * R + col, db);
+          for (int s = 0; s < N; ++s) {
+            const int P = pol(s);
+            const float d = step * (err * oth[s] - lam * v[s][q2]);
+            float *dst = Fm[s] + (size_t)idx[s] * R + col;
+            if (P == P_ATOMIC) {
+              atomic_add_f32(dst, d);
             } else {
-              b[s][q2] += db;
-              bdel[s][q2] += db;
-              if (V == SGD_RACY || V == SGD_XCD) Bf[s][(size_t)idx[s] * R + col] = b[s][q2];
+              v[s][q2] += d;
+              del[s][q2] += d;
+              if (P == P_RACY) *dst = v[s][q2];
             }
-          }
-          if (V == SGD_HOGWILD) {
-            atomic_add_f32(A + (size_t)arow * R + col, da);
-          } else {
-            a[q2] += da;
-            adel[q2] += da;
-            if (V != SGD_PRIVATE) A[(size_t)arow * R + col] = a[q2];
           }
         }
       }
     }
   }
-  if (V == SGD_PRIVATE) {
-    if (arow != NONE) {
 #pragma unroll
-      for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(A + (size_t)arow * R + wl + q2 * W, adel[q2]);
-    }
-#pragma unroll
-    for (int s = 0; s < N - 1; ++s)
-      if (cur[s] != NONE) {
-#pragma unroll
-        for (int q2 = 0; q2 < CPL; ++q2)
-          atomic_add_f32(Bf[s] + (size_t)cur[s] * R + wl + q2 * W, bdel[s][q2]);
-      }
-  }
+  for (int s = 0; s < N; ++s)
+    if (pol(s) == P_RUN && cur[s] != NONE) flush_run(s);
 }
 
 // ==========================================================================
@@ -747,35 +675,42 @@ static inline unsigned grid_for(uint64_t waves, unsigned block) {
   return (unsigned)((waves + wpb - 1) / wpb);
 }
 
-hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SgdReplicas &rp,
-                      int nmodes, int R, uint32_t span, uint64_t stride, int variant, float step, float lam,
-                      unsigned block) {
+// policy code = PA * 9 + PB * 3 + PC
+template <int N>
+static hipError_t launch_sgd_n(hipStream_t st, const CissArgs &c, const FactorArgs &f, int R, uint32_t span,
+                               uint64_t nspans, uint64_t stride, int code, float step, float lam, unsigned grid,
+                               unsigned block) {
+#define CTC_SGD_P(PA, PB, PC) \
+  case PA * 9 + PB * 3 + PC:  \
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, N, PA, PB, PC>), dim3(grid), dim3(block), 0, st, c, f, span, nspans, stride, step, lam)); \
+    break;
+  switch (code) {
+    CTC_SGD_P(P_RACY, P_ATOMIC, P_ATOMIC)
+    CTC_SGD_P(P_RACY, P_RUN, P_ATOMIC)
+    CTC_SGD_P(P_RACY, P_RACY, P_ATOMIC)
+    CTC_SGD_P(P_RACY, P_ATOMIC, P_RACY)
+    CTC_SGD_P(P_RACY, P_RACY, P_RACY)
+    CTC_SGD_P(P_RUN, P_RUN, P_RUN)
+    CTC_SGD_P(P_RUN, P_RUN, P_ATOMIC)
+    CTC_SGD_P(P_RUN, P_ATOMIC, P_ATOMIC)
+    CTC_SGD_P(P_ATOMIC, P_ATOMIC, P_ATOMIC)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CTC_SGD_P
+  return hipGetLastError();
+}
+
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R, uint32_t span,
+                      uint64_t stride, int code, float step, float lam, unsigned block) {
   if (c.nnz == 0) return hipSuccess;
   const uint64_t nspans = (c.nnz + span - 1) / span;
   const int W = R < 64 ? R : 64;
   const uint64_t waves = (nspans + (64 / W) - 1) / (64 / W);
   const unsigned grid = grid_for(waves, block);
-#define CTC_SGD_L(NN, VV) \
-  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, NN, VV>), dim3(grid), dim3(block), 0, st, c, f, rp, span, nspans, stride, step, lam))
-  if (nmodes == 3) {
-    if (variant == SGD_HOGWILD) { CTC_SGD_L(3, SGD_HOGWILD); }
-    else if (variant == SGD_RACY) { CTC_SGD_L(3, SGD_RACY); }
-    else if (variant == SGD_MIXED) { CTC_SGD_L(3, SGD_MIXED); }
-    else if (variant == SGD_MIXED_L2) { CTC_SGD_L(3, SGD_MIXED_L2); }
-    else if (variant == SGD_XCD) { CTC_SGD_L(3, SGD_XCD); }
-    else { CTC_SGD_L(3, SGD_PRIVATE); }
-  } else if (nmodes == 4) {
-    if (variant == SGD_HOGWILD) { CTC_SGD_L(4, SGD_HOGWILD); }
-    else if (variant == SGD_RACY) { CTC_SGD_L(4, SGD_RACY); }
-    else if (variant == SGD_MIXED) { CTC_SGD_L(4, SGD_MIXED); }
-    else if (variant == SGD_MIXED_L2) { CTC_SGD_L(4, SGD_MIXED_L2); }
-    else if (variant == SGD_XCD) { CTC_SGD_L(4, SGD_XCD); }
-    else { CTC_SGD_L(4, SGD_PRIVATE); }
-  } else {
-    return hipErrorInvalidValue;
-  }
-#undef CTC_SGD_L
-  return hipGetLastError();
+  if (nmodes == 3) return launch_sgd_n<3>(st, c, f, R, span, nspans, stride, code, step, lam, grid, block);
+  if (nmodes == 4) return launch_sgd_n<4>(st, c, f, R, span, nspans, stride, code, step, lam, grid, block);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
@@ -908,75 +843,6 @@ hipError_t launch_reg_rows(hipStream_t st, const uint32_t *directory, const uint
                            unsigned nblocks, double *part) {
   hipLaunchKernelGGL(k_reg_rows, dim3(nblocks), dim3(256), 0, st, directory, rowptr, nrows, R, lam,
                      per_occ, F, G, part);
-  return hipGetLastError();
-}
-
-// ==========================================================================
-// Per-XCD replica merge (SGD_XCD): for each row, F = F + (sum over the
-// replicas that touched it of (replica - F)) / (number of such replicas):
-// the exact update for a row one replica owns, the mean of the replicas'
-// rows for a row shared by several (model averaging, as the reference's
-// multi-GPU SGD averages per-GPU replicas each epoch, NC/sgd.cu:229-264).
-// The replicas are then reset to the merged factor for the next sweep.
-// ==========================================================================
-__global__ void __launch_bounds__(256) k_sgd_merge(float *F, float *rep, uint8_t *touch, uint64_t rows,
-                                                   int R, int nrep) {
-  const uint64_t n = rows * (uint64_t)R;
-  const uint64_t elems = n;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = p / R;
-    const float f0 = F[p];
-    float acc = 0.f;
-    int cnt = 0;
-    for (int x = 0; x < nrep; ++x) {
-      if (touch[(size_t)x * rows + i]) {
-        acc += rep[(size_t)x * elems + p] - f0;
-        ++cnt;
-      }
-    }
-    const float f1 = cnt ? f0 + acc / (float)cnt : f0;
-    F[p] = f1;
-    for (int x = 0; x < nrep; ++x) rep[(size_t)x * elems + p] = f1;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_bytes_zero(uint8_t *t, uint64_t n) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x)
-    t[p] = 0;
-}
-
-__global__ void __launch_bounds__(256) k_replicate(const float *F, float *rep, uint64_t n, int nrep) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const float v = F[p];
-    for (int x = 0; x < nrep; ++x) rep[(size_t)x * n + p] = v;
-  }
-}
-
-hipError_t launch_sgd_merge(hipStream_t st, float *F, float *rep, uint8_t *touch, uint64_t rows, int R,
-                            int nrep) {
-  uint64_t blocks = (rows * R + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sgd_merge, dim3((unsigned)blocks), dim3(256), 0, st, F, rep, touch, rows, R, nrep);
-  return hipGetLastError();
-}
-
-hipError_t launch_bytes_zero(hipStream_t st, uint8_t *t, uint64_t n) {
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bytes_zero, dim3((unsigned)blocks), dim3(256), 0, st, t, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_replicate(hipStream_t st, const float *F, float *rep, uint64_t n, int nrep) {
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_replicate, dim3((unsigned)blocks), dim3(256), 0, st, F, rep, n, nrep);
   return hipGetLastError();
 }
 

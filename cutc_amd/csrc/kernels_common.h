@@ -37,16 +37,6 @@ struct FactorArgs {
   float *f[4];
 };
 
-// Per-XCD factor replicas used by the replicated SGD sweep (nrep = 1 and
-// zero offsets otherwise): replica x of CISS-relative mode s starts at
-// f[s] + x * elems[s]; touch[s] + x * rows[s] flags the rows replica x wrote.
-struct SgdReplicas {
-  uint64_t elems[4];
-  uint8_t *touch[4];
-  uint64_t rows[4];
-  uint32_t nrep;
-};
-
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
 
 // Sum across groups of W consecutive lanes (W in {1,2,...,64}); every lane of

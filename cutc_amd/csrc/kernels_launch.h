@@ -7,13 +7,9 @@
 
 namespace ctc {
 
-hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SgdReplicas &rp,
-                      int nmodes, int R, uint32_t span, uint64_t stride, int variant, float step, float lam,
-                      unsigned block);
-hipError_t launch_sgd_merge(hipStream_t st, float *F, float *rep, uint8_t *touch, uint64_t rows, int R,
-                            int nrep);
-hipError_t launch_bytes_zero(hipStream_t st, uint8_t *t, uint64_t n);
-hipError_t launch_replicate(hipStream_t st, const float *F, float *rep, uint64_t n, int nrep);
+// code = PA * 9 + PB * 3 + PC with P in {0 racy, 1 atomic, 2 run} (see k_sgd)
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R, uint32_t span,
+                      uint64_t stride, int code, float step, float lam, unsigned block);
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);

@@ -16,7 +16,6 @@ using namespace ctc;
 namespace {
 
 constexpr unsigned kEvalBlocks = 512;  // fixed grid for deterministic reductions
-constexpr uint32_t kXcdReplicas = 8;   // MI355X: 8 XCDs, one L2 each
 
 template <class T>
 ctc_status dev_alloc(T **p, size_t n) {
@@ -287,7 +286,7 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->max_epochs = algorithm == CTC_SGD ? 999 : 1000;                   // :22
   o->tolerance = 1e-4;                                                 // :36
   o->max_bad_epochs = 3;                                               // :32
-  o->tile_len = 256;
+  o->tile_len = algorithm == CTC_SGD ? 4096 : algorithm == CTC_ALS ? 1024 : 256;
   o->block = 256;
   o->seed = 0;
   o->mode_order = CTC_ORDER_FIXED;
@@ -296,8 +295,8 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->refine = 0;
   o->eval_every_epoch = 1;
   o->verbose = 0;
-  o->sgd_update = CTC_SGD_HOGWILD;
-  o->spread = 1;
+  o->sgd_update = CTC_SGD_AUTO;
+  o->spread = 0;
   return CTC_OK;
 }
 
@@ -356,8 +355,6 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->best[md]);
     dev_free(m->resid[md]);
     dev_free(m->FT[md]);
-    dev_free(m->rep[md]);
-    dev_free(m->touch[md]);
   }
   for (DevCoo *c : {&m->valid, &m->test}) {
     for (int s = 0; s < 4; ++s) dev_free(c->ind[s]);
@@ -443,7 +440,6 @@ extern "C" ctc_status ctc_model_set_factor(ctc_model *m, uint32_t mode, const fl
                          m->ctx->stream));
   CTC_HIP(hipStreamSynchronize(m->ctx->stream));
   m->resid_valid = false;
-  m->rep_valid = false;
   return CTC_OK;
 }
 
@@ -505,7 +501,6 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
                              o->refine != 0 || o->precision == CTC_PREC_MIXED, m->F[mode], m->dflag));
   }
   m->resid_valid = false;
-  m->rep_valid = false;
   return CTC_OK;
 }
 
@@ -539,50 +534,18 @@ static uint64_t spread_stride(uint64_t n, bool on) {
 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   CTC_TRY(ensure_ciss(m, 0));
-  if (o->sgd_update < 0 || o->sgd_update > 5) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
+  // policy codes of k_sgd: PA * 9 + PB * 3 + PC with 0 racy, 1 atomic
+  static const int code_of[3] = {0 * 9 + 0 * 3 + 1, 0 * 9 + 1 * 3 + 1, 0};
+  if (o->sgd_update < 0 || o->sgd_update > 2) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
+  const int code = code_of[o->sgd_update];
   const float step = (float)(lr * o->sgd_scale);
   const uint32_t span = span_tiles_of(o) * 64;
   const uint64_t nspans = (m->ciss[0].nnz + span - 1) / span;
-  hipStream_t st = m->ctx->stream;
-  const uint32_t N = m->nmodes, R = m->rank;
-  SgdReplicas rp{};
-  rp.nrep = 1;
-  FactorArgs fa = rel_factors(m->F, N, 0);
-  if (o->sgd_update == 5) {  // per-XCD replicas
-    const uint32_t X = kXcdReplicas;
-    for (uint32_t md = 0; md < N; ++md) {
-      if (!m->rep[md]) {
-        CTC_HIP(hipMalloc(&m->rep[md], (size_t)X * m->dims[md] * R * sizeof(float)));
-        CTC_HIP(hipMalloc(&m->touch[md], (size_t)X * m->dims[md]));
-        m->rep_valid = false;
-      }
-    }
-    if (!m->rep_valid) {
-      for (uint32_t md = 0; md < N; ++md)
-        CTC_HIP(launch_replicate(st, m->F[md], m->rep[md], m->dims[md] * R, (int)X));
-      m->rep_valid = true;
-    }
-    for (uint32_t md = 0; md < N; ++md) CTC_HIP(launch_bytes_zero(st, m->touch[md], (size_t)X * m->dims[md]));
-    for (uint32_t s2 = 0; s2 < N; ++s2) {
-      const uint32_t q = s2 % N;  // CISS mode 0: relative order is the identity
-      fa.f[s2] = m->rep[q];
-      rp.elems[s2] = m->dims[q] * R;
-      rp.touch[s2] = m->touch[q];
-      rp.rows[s2] = m->dims[q];
-    }
-    rp.nrep = X;
-  }
   {
     Prof p(m->ctx, "sgd");
-    CTC_HIP(launch_sgd(st, ciss_args(m->ciss[0]), fa, rp, (int)N, (int)R, span,
-                       spread_stride(nspans, o->spread != 0), o->sgd_update, step, (float)o->lambda,
-                       block_of(o)));
-  }
-  if (o->sgd_update != 5) m->rep_valid = false;  // F changed outside the replicas
-  if (o->sgd_update == 5) {
-    Prof p(m->ctx, "sgd_merge");
-    for (uint32_t md = 0; md < N; ++md)
-      CTC_HIP(launch_sgd_merge(st, m->F[md], m->rep[md], m->touch[md], m->dims[md], (int)R, (int)kXcdReplicas));
+    CTC_HIP(launch_sgd(m->ctx->stream, ciss_args(m->ciss[0]), rel_factors(m->F, m->nmodes, 0), (int)m->nmodes,
+                       (int)m->rank, span, spread_stride(nspans, o->spread != 0), code, step,
+                       (float)o->lambda, block_of(o)));
   }
   m->resid_valid = false;
   return CTC_OK;

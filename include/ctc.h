@@ -171,14 +171,17 @@ typedef enum ctc_precision {
   CTC_PREC_MIXED = 1  /* BF16 MFMA Gram/MTTKRP, FP32 accumulate; FP64 Cholesky + 1 refinement */
 } ctc_precision;
 typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mode_order;
-/* How concurrent SGD walkers combine updates of a shared factor row. */
+/* How concurrent SGD walkers combine updates of a factor row they share.  A
+ * walker keeps the rows of its current nonzero in registers; "racy" rows are
+ * stored after every nonzero (concurrent writers overwrite each other),
+ * "atomic" rows are re-read every nonzero and every update is added with a
+ * float atomic (the reference's atomicAdd, SG/sgd.cu:96-98). */
 typedef enum ctc_sgd_update {
-  CTC_SGD_HOGWILD = 0, /* re-read rows every nonzero, add each update with atomics
-                          (SG/sgd.cu:96-98 with fresh reads) */
-  CTC_SGD_RACY = 1,    /* keep rows in registers, store after every nonzero
-                          (concurrent writers overwrite each other) */
-  CTC_SGD_PRIVATE = 2  /* keep rows in registers, add the run's delta with
-                          atomics when the run ends */
+  CTC_SGD_AUTO = 0,   /* rows of the walked mode and of the next mode racy, the
+                         other modes atomic (measured best on Zipf tensors: the
+                         hottest rows never pay serialized atomics) */
+  CTC_SGD_ATOMIC = 1, /* walked mode racy, every other mode atomic */
+  CTC_SGD_RACY = 2    /* every mode racy (fastest; loses updates on shared rows) */
 } ctc_sgd_update;
 
 typedef struct ctc_options {

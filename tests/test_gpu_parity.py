@@ -164,7 +164,8 @@ def test_als_not_spd_is_reported(ctx):
 
 
 # --------------------------------------------------------------------------
-def test_sgd_test_rmse_trajectory(ctx):
+@pytest.mark.parametrize("policy,tile_len", [(ct.SGD_AUTO, None), (ct.SGD_AUTO, 1024), (ct.SGD_ATOMIC, 256)])
+def test_sgd_test_rmse_trajectory(ctx, policy, tile_len):
     """SGD (lr 1e-3, x0.95 per epoch, lambda 0.05, scale 1) test-RMSE per
     epoch within 1e-3 of the sequential FP64 oracle."""
     tr, te = ct.synth((3000, 1500, 200), 150_000, zipf=(1.0, 1.0, 0.0), true_rank=10,
@@ -179,7 +180,7 @@ def test_sgd_test_rmse_trajectory(ctx):
     rm_orc, _ = O.sgd(ocoo(tr), F64, 1e-3, 0.05, epochs, test=ocoo(te), scale=1.0, decay=0.95)
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.05, learning_rate=1e-3,
                      lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0, max_epochs=epochs,
-                     max_bad_epochs=0, tile_len=64)
+                     max_bad_epochs=0, tile_len=tile_len, sgd_update=policy)
     res = m.solve(opt)
     rm_gpu = [h["rmse_te"] for h in res.history[1:]]
     say("  oracle", np.round(rm_orc, 5))
@@ -191,7 +192,8 @@ def test_sgd_test_rmse_trajectory(ctx):
 
 
 @pytest.mark.parametrize("R", [8, 16, 64, 128])
-def test_sgd_single_walker_matches_sequential(ctx, R):
+@pytest.mark.parametrize("policy", [ct.SGD_AUTO, ct.SGD_ATOMIC, ct.SGD_RACY])
+def test_sgd_single_walker_matches_sequential(ctx, R, policy):
     """With one tile the GPU walk is one sequential walker: factors match the
     sequential oracle to FP32 rounding."""
     tr, _ = ct.synth((10, 8, 6), 64, test_fraction=0.0, seed=R)
@@ -201,7 +203,8 @@ def test_sgd_single_walker_matches_sequential(ctx, R):
     m.set_factors(F)
     F64 = [f.astype(np.float64) for f in F]
     O.sgd_epoch(ocoo(tr), F64, 1e-2, 0.1, 1.0)
-    m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.1, sgd_scale=1.0, tile_len=64), lr=1e-2)
+    m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.1, sgd_scale=1.0, tile_len=64, sgd_update=policy),
+                lr=1e-2)
     for md in range(3):
         assert relerr(m.factor(md), F64[md]) < 1e-5
 
@@ -348,12 +351,13 @@ def test_autograd_training_decreases_loss(ctx):
     m = ct.Model(ctx, tr, None, te, rank=32)
     dev = torch.device("cuda:0")
     Ft = [torch.from_numpy(0.5 * f).to(dev).requires_grad_(True) for f in init_factors(tr.dims, 32, 2)]
-    opt = torch.optim.SGD(Ft, lr=2e-4, momentum=0.9)
+    opt = torch.optim.Adam(Ft, lr=0.01)
     losses = []
-    for _ in range(30):
+    for _ in range(40):
         opt.zero_grad()
         loss = completion_loss(m, Ft, 0.05)
         loss.backward()
         opt.step()
         losses.append(float(loss))
-    assert losses[-1] < 0.5 * losses[0], losses[::5]
+    assert all(b < a for a, b in zip(losses, losses[1:])), losses[::5]
+    assert losses[-1] < 0.8 * losses[0], losses[::5]
