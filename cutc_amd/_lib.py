@@ -15,7 +15,7 @@ ALS, SGD, CCD = 0, 1, 2
 LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
 ORDER_FIXED, ORDER_ROTATE = 0, 1
-SGD_AUTO, SGD_ATOMIC, SGD_RACY = 0, 1, 2
+SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY = 0, 1, 2, 3
 VALUES_RAW, VALUES_AFFINE_1_5, VALUES_INT_1_5 = 0, 1, 2
 
 STATUS = {
@@ -97,6 +97,7 @@ class Options(C.Structure):
         ("verbose", C.c_int32),
         ("sgd_update", C.c_int32),
         ("spread", C.c_int32),
+        ("sgd_hot_rows", C.c_uint32),
     ]
 
 

@@ -82,7 +82,10 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
           CTC_HIP(launch_ccd_col(st, a, (int)N, block));
         }
         CTC_HIP(launch_fill(st, col, m->dims[md], 0.f));  // rows with no nonzero -> 0
-        CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, o->lambda, col));
+        {
+          Prof p(m->ctx, "ccd_finish");
+          CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, o->lambda, col));
+        }
       }
     }
     m->ccd_pending = (int)f;

@@ -68,6 +68,12 @@ struct DevCiss {
   uint64_t nspans = 0, npieces = 0;
   uint32_t *span_off = nullptr;  // nspans: piece id = slot + span_off[span]
   uint32_t *slot_pb = nullptr;   // dlength + 1: pieces of slot s
+  // heavy slots (more than kHeavyPieces pieces): their pieces are summed in
+  // groups of kGroupPieces consecutive pieces before the ALS solve
+  uint64_t ngroups = 0;
+  uint32_t *grp_piece = nullptr;  // ngroups: first piece of group g
+  uint32_t *grp_n = nullptr;      // ngroups: pieces in group g
+  uint32_t *slot_gb = nullptr;    // dlength + 1: groups of slot s (empty for light slots)
   // host copies needed by the piece-table builder
   std::vector<uint32_t> h_tile_slot;
   std::vector<uint64_t> h_tile_mask;
@@ -134,6 +140,9 @@ struct ctc_model {
   size_t colsave_elems = 0;
   float *FT[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};  // column-major mirrors
   int ccd_pending = -1;  // column whose new component the residual copies still hold
+  // SGD hot-row bitmaps (rows with more nonzeros than hot_threshold)
+  uint32_t *hotbits[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t hot_threshold = 0;
 
 };
 

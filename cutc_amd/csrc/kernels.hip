@@ -37,13 +37,16 @@ namespace ctc {
 //              atomic (the reference's atomicAdd, SG/sgd.cu:96-98).
 //   P_RUN      read at the start of a run, apply own updates in registers, add
 //              the run's accumulated delta with atomics when the run ends.
+//   P_HOT      per row: P_RACY if the row's bit is set in the mode's hot-row
+//              bitmap (rows with more updates per sweep than a threshold,
+//              whose atomics would serialise on one L2 channel), else P_ATOMIC.
 // Walker w processes span (w * stride) mod nspans (stride 1 = natural order).
 // ==========================================================================
-enum { P_RACY = 0, P_ATOMIC = 1, P_RUN = 2 };
+enum { P_RACY = 0, P_ATOMIC = 1, P_RUN = 2, P_HOT = 3 };
 
 template <int R, int N, int PA, int PB, int PC>
-__global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, uint32_t span, uint64_t nspans,
-                                             uint64_t stride, float step, float lam) {
+__global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows hot, uint32_t span,
+                                             uint64_t nspans, uint64_t stride, float step, float lam) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int WPW = 64 / W;
@@ -110,12 +113,20 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, uint32_t
         } else {
           idx[0] = cur[0];
         }
+        int pr[N];  // per-row policy of this nonzero
 #pragma unroll
         for (int s = 0; s < N; ++s) {
-          const int P = pol(s);
+          int P = pol(s);
+          if (P == P_HOT) P = ((hot.bits[s][idx[s] >> 5] >> (idx[s] & 31)) & 1u) ? P_RACY : P_ATOMIC;
+          pr[s] = P;
+        }
+#pragma unroll
+        for (int s = 0; s < N; ++s) {
+          const int P = pr[s];
           if (P == P_ATOMIC || idx[s] != cur[s]) {
             if (P == P_RUN && cur[s] != NONE && idx[s] != cur[s]) flush_run(s);
-            cur[s] = idx[s];
+            // an atomic row is re-read at every nonzero: forget it as the cached row
+            cur[s] = P == P_ATOMIC ? NONE : idx[s];
 #pragma unroll
             for (int q2 = 0; q2 < CPL; ++q2) {
               v[s][q2] = Fm[s][(size_t)idx[s] * R + wl + q2 * W];
@@ -148,7 +159,7 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, uint32_t
           const int col = wl + q2 * W;
 #pragma unroll
           for (int s = 0; s < N; ++s) {
-            const int P = pol(s);
+            const int P = pr[s];
             const float d = step * (err * oth[s] - lam * v[s][q2]);
             float *dst = Fm[s] + (size_t)idx[s] * R + col;
             if (P == P_ATOMIC) {
@@ -424,8 +435,28 @@ __device__ inline int tri_row(int t) {
   return a;
 }
 
+// Heavy slots (more pieces than fit one workgroup's serial sum) are first
+// reduced in fixed groups of consecutive pieces, one workgroup per group, so
+// the solve sums a handful of group sums instead of thousands of pieces.
+template <int R>
+__global__ void __launch_bounds__(64) k_piece_reduce(const double *pieces, const uint32_t *grp_piece,
+                                                     const uint32_t *grp_n, uint64_t ngroups, double *gsum) {
+  constexpr size_t PSZ = (size_t)R * R + R;
+  const uint64_t g = blockIdx.x;
+  if (g >= ngroups) return;
+  const double *P = pieces + (size_t)grp_piece[g] * PSZ;
+  const uint32_t n = grp_n[g];
+  double *out = gsum + g * PSZ;
+  for (size_t e = threadIdx.x; e < PSZ; e += 64) {
+    double acc = 0.0;
+    for (uint32_t q = 0; q < n; ++q) acc += P[(size_t)q * PSZ + e];
+    out[e] = acc;
+  }
+}
+
 template <int R, bool REFINE>
 __global__ void __launch_bounds__(64) k_als_solve(const double *pieces, const uint32_t *slot_pb,
+                                                  const double *gsum, const uint32_t *slot_gb,
                                                   const uint32_t *directory, uint64_t dlength,
                                                   double lam, float *A, uint32_t *flag) {
   constexpr int LD = R + 1;
@@ -437,14 +468,20 @@ __global__ void __launch_bounds__(64) k_als_solve(const double *pieces, const ui
   const int lane = threadIdx.x;
   const uint64_t s = blockIdx.x;
   if (s >= dlength) return;
-  const uint32_t p0 = slot_pb[s], p1 = slot_pb[s + 1];
+  uint32_t p0 = slot_pb[s], p1 = slot_pb[s + 1];
+  const double *src = pieces;
+  if (slot_gb && slot_gb[s + 1] > slot_gb[s]) {  // heavy slot: read its group sums
+    p0 = slot_gb[s];
+    p1 = slot_gb[s + 1];
+    src = gsum;
+  }
 
   double g[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) g[t] = 0.0;
   double mv = 0.0;
   for (uint32_t p = p0; p < p1; ++p) {
-    const double *P = pieces + (size_t)p * PSZ;
+    const double *P = src + (size_t)p * PSZ;
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int idx = lane + 64 * t;
@@ -675,25 +712,21 @@ static inline unsigned grid_for(uint64_t waves, unsigned block) {
   return (unsigned)((waves + wpb - 1) / wpb);
 }
 
-// policy code = PA * 9 + PB * 3 + PC
+// policy code = PA * 16 + PB * 4 + PC
 template <int N>
-static hipError_t launch_sgd_n(hipStream_t st, const CissArgs &c, const FactorArgs &f, int R, uint32_t span,
+static hipError_t launch_sgd_n(hipStream_t st, const CissArgs &c, const FactorArgs &f, const HotRows &hot, int R,
+                               uint32_t span,
                                uint64_t nspans, uint64_t stride, int code, float step, float lam, unsigned grid,
                                unsigned block) {
 #define CTC_SGD_P(PA, PB, PC) \
-  case PA * 9 + PB * 3 + PC:  \
-    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, N, PA, PB, PC>), dim3(grid), dim3(block), 0, st, c, f, span, nspans, stride, step, lam)); \
+  case PA * 16 + PB * 4 + PC:  \
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, N, PA, PB, PC>), dim3(grid), dim3(block), 0, st, c, f, hot, span, nspans, stride, step, lam)); \
     break;
   switch (code) {
+    CTC_SGD_P(P_RACY, P_HOT, P_HOT)
     CTC_SGD_P(P_RACY, P_ATOMIC, P_ATOMIC)
-    CTC_SGD_P(P_RACY, P_RUN, P_ATOMIC)
-    CTC_SGD_P(P_RACY, P_RACY, P_ATOMIC)
-    CTC_SGD_P(P_RACY, P_ATOMIC, P_RACY)
     CTC_SGD_P(P_RACY, P_RACY, P_RACY)
-    CTC_SGD_P(P_RUN, P_RUN, P_RUN)
-    CTC_SGD_P(P_RUN, P_RUN, P_ATOMIC)
-    CTC_SGD_P(P_RUN, P_ATOMIC, P_ATOMIC)
-    CTC_SGD_P(P_ATOMIC, P_ATOMIC, P_ATOMIC)
+    CTC_SGD_P(P_RACY, P_RACY, P_ATOMIC)
     default:
       return hipErrorInvalidValue;
   }
@@ -701,15 +734,15 @@ static hipError_t launch_sgd_n(hipStream_t st, const CissArgs &c, const FactorAr
   return hipGetLastError();
 }
 
-hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R, uint32_t span,
-                      uint64_t stride, int code, float step, float lam, unsigned block) {
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, const HotRows &hot, int nmodes, int R,
+                      uint32_t span, uint64_t stride, int code, float step, float lam, unsigned block) {
   if (c.nnz == 0) return hipSuccess;
   const uint64_t nspans = (c.nnz + span - 1) / span;
   const int W = R < 64 ? R : 64;
   const uint64_t waves = (nspans + (64 / W) - 1) / (64 / W);
   const unsigned grid = grid_for(waves, block);
-  if (nmodes == 3) return launch_sgd_n<3>(st, c, f, R, span, nspans, stride, code, step, lam, grid, block);
-  if (nmodes == 4) return launch_sgd_n<4>(st, c, f, R, span, nspans, stride, code, step, lam, grid, block);
+  if (nmodes == 3) return launch_sgd_n<3>(st, c, f, hot, R, span, nspans, stride, code, step, lam, grid, block);
+  if (nmodes == 4) return launch_sgd_n<4>(st, c, f, hot, R, span, nspans, stride, code, step, lam, grid, block);
   return hipErrorInvalidValue;
 }
 
@@ -772,19 +805,33 @@ hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &
   return hipGetLastError();
 }
 
-hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
-                            const uint32_t *directory, uint64_t dlength, int R, double lam, bool refine,
-                            float *A, uint32_t *flag) {
+hipError_t launch_piece_reduce(hipStream_t st, const double *pieces, const uint32_t *grp_piece,
+                               const uint32_t *grp_n, uint64_t ngroups, int R, double *gsum) {
+  if (ngroups == 0) return hipSuccess;
+  const unsigned grid = (unsigned)ngroups;
+  switch (R) {
+    case 8: hipLaunchKernelGGL((k_piece_reduce<8>), dim3(grid), dim3(64), 0, st, pieces, grp_piece, grp_n, ngroups, gsum); break;
+    case 16: hipLaunchKernelGGL((k_piece_reduce<16>), dim3(grid), dim3(64), 0, st, pieces, grp_piece, grp_n, ngroups, gsum); break;
+    case 32: hipLaunchKernelGGL((k_piece_reduce<32>), dim3(grid), dim3(64), 0, st, pieces, grp_piece, grp_n, ngroups, gsum); break;
+    case 64: hipLaunchKernelGGL((k_piece_reduce<64>), dim3(grid), dim3(64), 0, st, pieces, grp_piece, grp_n, ngroups, gsum); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb, const double *gsum,
+                            const uint32_t *slot_gb, const uint32_t *directory, uint64_t dlength, int R,
+                            double lam, bool refine, float *A, uint32_t *flag) {
   if (dlength == 0) return hipSuccess;
   const unsigned grid = (unsigned)dlength;
 #define CTC_SOLVE(RR)                                                                                      \
   do {                                                                                                      \
     if (refine)                                                                                             \
-      hipLaunchKernelGGL((k_als_solve<RR, true>), dim3(grid), dim3(64), 0, st, pieces, slot_pb, directory, \
-                         dlength, lam, A, flag);                                                            \
+      hipLaunchKernelGGL((k_als_solve<RR, true>), dim3(grid), dim3(64), 0, st, pieces, slot_pb, gsum,      \
+                         slot_gb, directory, dlength, lam, A, flag);                                        \
     else                                                                                                    \
-      hipLaunchKernelGGL((k_als_solve<RR, false>), dim3(grid), dim3(64), 0, st, pieces, slot_pb,           \
-                         directory, dlength, lam, A, flag);                                                 \
+      hipLaunchKernelGGL((k_als_solve<RR, false>), dim3(grid), dim3(64), 0, st, pieces, slot_pb, gsum,     \
+                         slot_gb, directory, dlength, lam, A, flag);                                        \
   } while (0)
   switch (R) {
     case 8: CTC_SOLVE(8); break;

@@ -152,18 +152,26 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
   }
 }
 
-// a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form)
+// a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form).
+// One wave per slot: lane l sums pieces l, l+64, ... of the slot, then a fixed
+// shuffle tree adds the 64 partial sums (deterministic, parallel over the
+// thousands of pieces of a heavy row).
 __global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const uint32_t *slot_pb,
                                                     const uint32_t *directory, uint64_t dlength,
                                                     double lam, float *col) {
-  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t s = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= dlength) return;
+  const int lane = lane_id();
   double n = 0.0, d = 0.0;
-  for (uint32_t p = slot_pb[s]; p < slot_pb[s + 1]; ++p) {
+  for (uint32_t p = slot_pb[s] + lane; p < slot_pb[s + 1]; p += 64) {
     n += pieces[2 * (size_t)p];
     d += pieces[2 * (size_t)p + 1];
   }
-  col[directory[s]] = (float)(n / (lam + d));
+  for (int o = 32; o > 0; o >>= 1) {
+    n += __shfl_xor(n, o, 64);
+    d += __shfl_xor(d, o, 64);
+  }
+  if (lane == 0) col[directory[s]] = (float)(n / (lam + d));
 }
 
 // residual copy of a mode: r = x - <rows>, row-major factors in CISS-relative order
@@ -220,7 +228,7 @@ hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsig
 hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
                              const uint32_t *directory, uint64_t dlength, double lam, float *col) {
   if (dlength == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 255) / 256)), dim3(256), 0, st, pieces,
+  hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 3) / 4)), dim3(256), 0, st, pieces,
                      slot_pb, directory, dlength, lam, col);
   return hipGetLastError();
 }

@@ -37,6 +37,12 @@ struct FactorArgs {
   float *f[4];
 };
 
+// Hot-row bitmaps, CISS-relative modes: bit (i & 31) of bits[s][i >> 5] set
+// when row i of that mode is written racily by the SGD sweep.
+struct HotRows {
+  const uint32_t *bits[4];
+};
+
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
 
 // Sum across groups of W consecutive lanes (W in {1,2,...,64}); every lane of

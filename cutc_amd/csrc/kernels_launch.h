@@ -7,9 +7,9 @@
 
 namespace ctc {
 
-// code = PA * 9 + PB * 3 + PC with P in {0 racy, 1 atomic, 2 run} (see k_sgd)
-hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R, uint32_t span,
-                      uint64_t stride, int code, float step, float lam, unsigned block);
+// code = PA * 16 + PB * 4 + PC with P in {0 racy, 1 atomic, 2 run, 3 hot-row bitmap} (see k_sgd)
+hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, const HotRows &hot, int nmodes, int R,
+                      uint32_t span, uint64_t stride, int code, float step, float lam, unsigned block);
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);
@@ -17,9 +17,11 @@ uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block);
 hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                            uint32_t span_tiles, uint64_t nspans, const uint32_t *span_off,
                            double *pieces, unsigned block);
-hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
-                            const uint32_t *directory, uint64_t dlength, int R, double lam, bool refine,
-                            float *A, uint32_t *flag);
+hipError_t launch_piece_reduce(hipStream_t st, const double *pieces, const uint32_t *grp_piece,
+                               const uint32_t *grp_n, uint64_t ngroups, int R, double *gsum);
+hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb, const double *gsum,
+                            const uint32_t *slot_gb, const uint32_t *directory, uint64_t dlength, int R,
+                            double lam, bool refine, float *A, uint32_t *flag);
 hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f, int nmodes, int R,
                            unsigned nblocks, double *part);
 hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,

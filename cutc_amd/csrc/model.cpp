@@ -156,12 +156,34 @@ ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles) {
     pb[s] = (uint32_t)(s + off[w]);
   }
   pb[d.dlength] = (uint32_t)base;
+  // group tables for heavy slots
+  constexpr uint32_t kHeavyPieces = 64, kGroupPieces = 32;
+  std::vector<uint32_t> gpiece, gn, gb(d.dlength + 1);
+  for (uint64_t s = 0; s < d.dlength; ++s) {
+    gb[s] = (uint32_t)gpiece.size();
+    const uint32_t n = pb[s + 1] - pb[s];
+    if (n <= kHeavyPieces) continue;
+    for (uint32_t q = 0; q < n; q += kGroupPieces) {
+      gpiece.push_back(pb[s] + q);
+      gn.push_back(std::min(kGroupPieces, n - q));
+    }
+  }
+  gb[d.dlength] = (uint32_t)gpiece.size();
   dev_free(d.span_off);
   dev_free(d.slot_pb);
-  d.span_off = d.slot_pb = nullptr;
+  dev_free(d.grp_piece);
+  dev_free(d.grp_n);
+  dev_free(d.slot_gb);
+  d.span_off = d.slot_pb = d.grp_piece = d.grp_n = d.slot_gb = nullptr;
   hipStream_t st = m->ctx->stream;
   CTC_TRY(upload(&d.span_off, off.data(), nspans, st));
   CTC_TRY(upload(&d.slot_pb, pb.data(), d.dlength + 1, st));
+  d.ngroups = gpiece.size();
+  if (d.ngroups) {
+    CTC_TRY(upload(&d.grp_piece, gpiece.data(), d.ngroups, st));
+    CTC_TRY(upload(&d.grp_n, gn.data(), d.ngroups, st));
+    CTC_TRY(upload(&d.slot_gb, gb.data(), d.dlength + 1, st));
+  }
   CTC_HIP(hipStreamSynchronize(st));
   d.span_tiles = span_tiles;
   d.nspans = nspans;
@@ -297,6 +319,7 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->verbose = 0;
   o->sgd_update = CTC_SGD_AUTO;
   o->spread = 0;
+  o->sgd_hot_rows = 262144;
   return CTC_OK;
 }
 
@@ -351,10 +374,14 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(d.rowptr);
     dev_free(d.span_off);
     dev_free(d.slot_pb);
+    dev_free(d.grp_piece);
+    dev_free(d.grp_n);
+    dev_free(d.slot_gb);
     dev_free(m->F[md]);
     dev_free(m->best[md]);
     dev_free(m->resid[md]);
     dev_free(m->FT[md]);
+    dev_free(m->hotbits[md]);
   }
   for (DevCoo *c : {&m->valid, &m->test}) {
     for (int s = 0; s < 4; ++s) dev_free(c->ind[s]);
@@ -483,9 +510,10 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   CTC_TRY(ensure_pieces(m, mode, S));
   DevCiss &d = m->ciss[mode];
   const size_t psz = (size_t)R * R + R;
-  CTC_TRY(ensure_scratch(m, d.npieces * psz * sizeof(double)));
+  CTC_TRY(ensure_scratch(m, (d.npieces + d.ngroups) * psz * sizeof(double)));
   hipStream_t st = m->ctx->stream;
   double *pieces = (double *)m->scratch;
+  double *gsum = pieces + d.npieces * psz;
   CissArgs ca = ciss_args(d);
   FactorArgs fa = rel_factors(m->F, m->nmodes, mode);
   {
@@ -495,9 +523,14 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   }
   // rows absent from the directory become 0 (SG/als.cu:699 zeroes the factor)
   CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
+  if (d.ngroups) {
+    Prof p(m->ctx, "als_reduce");
+    CTC_HIP(launch_piece_reduce(st, pieces, d.grp_piece, d.grp_n, d.ngroups, (int)R, gsum));
+  }
   {
     Prof p(m->ctx, "als_solve");
-    CTC_HIP(launch_als_solve(st, pieces, d.slot_pb, d.directory, d.dlength, (int)R, o->lambda,
+    CTC_HIP(launch_als_solve(st, pieces, d.slot_pb, d.ngroups ? gsum : nullptr, d.slot_gb, d.directory,
+                             d.dlength, (int)R, o->lambda,
                              o->refine != 0 || o->precision == CTC_PREC_MIXED, m->F[mode], m->dflag));
   }
   m->resid_valid = false;
@@ -534,16 +567,33 @@ static uint64_t spread_stride(uint64_t n, bool on) {
 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   CTC_TRY(ensure_ciss(m, 0));
-  // policy codes of k_sgd: PA * 9 + PB * 3 + PC with 0 racy, 1 atomic
-  static const int code_of[3] = {0 * 9 + 0 * 3 + 1, 0 * 9 + 1 * 3 + 1, 0};
-  if (o->sgd_update < 0 || o->sgd_update > 2) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
+  // policy codes of k_sgd: PA * 16 + PB * 4 + PC with 0 racy, 1 atomic, 3 hot-row bitmap
+  static const int code_of[4] = {0 * 16 + 3 * 4 + 3, 0 * 16 + 1 * 4 + 1, 0, 0 * 16 + 0 * 4 + 1};
+  if (o->sgd_update < 0 || o->sgd_update > 3) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
   const int code = code_of[o->sgd_update];
+  HotRows hot{};
+  if (o->sgd_update == CTC_SGD_AUTO) {
+    const uint64_t thr = o->sgd_hot_rows ? o->sgd_hot_rows : 262144;
+    if (thr != m->hot_threshold || !m->hotbits[0]) {
+      const ctc_coo *t = m->train.get();
+      for (uint32_t md = 0; md < m->nmodes; ++md) {
+        std::vector<uint32_t> cnt(m->dims[md], 0), bits((m->dims[md] + 31) / 32, 0);
+        for (uint64_t p = 0; p < t->nnz; ++p) cnt[t->ind[md][p]]++;
+        for (uint64_t i = 0; i < m->dims[md]; ++i)
+          if (cnt[i] > thr) bits[i >> 5] |= 1u << (i & 31);
+        if (!m->hotbits[md]) CTC_HIP(hipMalloc(&m->hotbits[md], bits.size() * sizeof(uint32_t)));
+        CTC_HIP(hipMemcpy(m->hotbits[md], bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      }
+      m->hot_threshold = thr;
+    }
+    for (uint32_t s2 = 0; s2 < m->nmodes; ++s2) hot.bits[s2] = m->hotbits[s2];  // CISS mode 0: identity order
+  }
   const float step = (float)(lr * o->sgd_scale);
   const uint32_t span = span_tiles_of(o) * 64;
   const uint64_t nspans = (m->ciss[0].nnz + span - 1) / span;
   {
     Prof p(m->ctx, "sgd");
-    CTC_HIP(launch_sgd(m->ctx->stream, ciss_args(m->ciss[0]), rel_factors(m->F, m->nmodes, 0), (int)m->nmodes,
+    CTC_HIP(launch_sgd(m->ctx->stream, ciss_args(m->ciss[0]), rel_factors(m->F, m->nmodes, 0), hot, (int)m->nmodes,
                        (int)m->rank, span, spread_stride(nspans, o->spread != 0), code, step,
                        (float)o->lambda, block_of(o)));
   }

@@ -177,11 +177,12 @@ typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mo
  * "atomic" rows are re-read every nonzero and every update is added with a
  * float atomic (the reference's atomicAdd, SG/sgd.cu:96-98). */
 typedef enum ctc_sgd_update {
-  CTC_SGD_AUTO = 0,   /* rows of the walked mode and of the next mode racy, the
-                         other modes atomic (measured best on Zipf tensors: the
-                         hottest rows never pay serialized atomics) */
-  CTC_SGD_ATOMIC = 1, /* walked mode racy, every other mode atomic */
-  CTC_SGD_RACY = 2    /* every mode racy (fastest; loses updates on shared rows) */
+  CTC_SGD_AUTO = 0,      /* walked-mode rows racy; a row of another mode is racy when
+                            it has more than sgd_hot_rows nonzeros (its atomics would
+                            serialise on one L2 channel), atomic otherwise */
+  CTC_SGD_ATOMIC = 1,    /* walked-mode rows racy, every other row atomic */
+  CTC_SGD_RACY = 2,      /* every row racy (fastest; loses updates on shared rows) */
+  CTC_SGD_NEXT_RACY = 3  /* walked mode and the next mode racy, other modes atomic */
 } ctc_sgd_update;
 
 typedef struct ctc_options {
@@ -208,6 +209,7 @@ typedef struct ctc_options {
   int32_t sgd_update;      /* ctc_sgd_update */
   int32_t spread;          /* 1: walkers visit spans in a strided order that
                               spreads a hot row's spans over the sweep */
+  uint32_t sgd_hot_rows;   /* CTC_SGD_AUTO threshold (nonzeros per row; 0 = 262144) */
 } ctc_options;
 
 /* Reference defaults for an algorithm (SG/base.h, SG/completion.c:460-480). */
