@@ -52,7 +52,9 @@ def main():
     ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--algs", default="sgd,ccd,als")
+    ap.add_argument("--algs", default="sgd,ccd,als,batch")
+    ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--batch-iters", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -172,6 +174,49 @@ def main():
                            "rmse_te": ev["rmse_te"], "epochs": 1 + a.als_epochs}
         log(f"ALS {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), gram {gram_ms:.2f} ms, "
             f"solve {ms_s / max(n_s, 1):.2f} ms, rmse_te {ev['rmse_te']:.4f}")
+
+    # ------------------------------------------------------- batched ALS
+    if "batch" in algs and a.batch_problems > 0:
+        from cutc_amd.workloads import batch_problems, init_factors
+        t0 = time.time()
+        probs = batch_problems(a.batch_problems)
+        log(f"batch: {len(probs)} problems, {sum(p[0].nnz for p in probs)} train nnz, generated in "
+            f"{time.time() - t0:.1f}s")
+        t0 = time.time()
+        batch = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+        brng = np.random.default_rng(4)
+        Fb = [init_factors(batch.dims[i], p[2], brng) for i, p in enumerate(probs)]
+
+        def load():
+            for i in range(len(probs)):
+                for m in range(3):
+                    batch.set_factor(i, m, Fb[i][m])
+
+        bopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam)
+        load()
+        tuned = batch.autotune(bopt)
+        log(f"batch setup+autotune {time.time() - t0:.1f}s: {tuned}")
+        ms_eager = batch.als(bopt, iters=a.batch_iters, graph=False)
+        load()
+        batch.als(bopt, iters=a.batch_iters, graph=True)  # capture + first replay
+        reps = []
+        for _ in range(3):
+            load()
+            reps.append(batch.als(bopt, iters=a.batch_iters, graph=True))
+        ms_graph = float(np.median(reps))
+        rtr, rte = batch.rmse()
+        bn = batch.train_nnz
+        out_algs["batch_als"] = {
+            "problems": len(probs), "classes": batch.classes(), "train_nnz": bn, "iters": a.batch_iters,
+            "graph_ms": ms_graph, "direct_ms": ms_eager, "nnz_per_s": bn * a.batch_iters / (ms_graph * 1e-3),
+            "s_per_iter": ms_graph * 1e-3 / a.batch_iters,
+            "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
+                      for (r, _), t in zip(batch.classes(), tuned)],
+            "rmse_te_median": float(np.median(rte)), "rmse_tr_median": float(np.median(rtr))}
+        log(f"batch ALS: {a.batch_iters} iters of {len(probs)} problems in {ms_graph:.1f} ms (graph), "
+            f"{ms_eager:.1f} ms (direct); {bn * a.batch_iters / (ms_graph * 1e-3) / 1e9:.3f} Gnnz/s; "
+            f"median rmse_te {np.median(rte):.4f}")
+        del batch, probs, Fb
 
     copy_gbs = ctx.copy_bandwidth(1 << 30)
 

@@ -26,7 +26,7 @@ from ._lib import (ALS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FI
                    ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_NEXT_RACY, SGD_RACY,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
-__all__ = ["Coo", "Ciss", "Context", "Model", "Options", "synth", "tt_read", "ciss_alloc",
+__all__ = ["Coo", "Ciss", "Context", "Model", "Batch", "Options", "synth", "tt_read", "ciss_alloc",
            "tc_main_ciss", "tc_als", "tc_sgd", "tc_ccd", "CtcError", "ALS", "SGD", "CCD"]
 
 
@@ -400,6 +400,81 @@ class Model:
         r = _lib.TuneResult()
         check(lib().ctc_autotune(self._h, C.byref(o), C.byref(r)))
         return {"tile_len": r.tile_len, "block": r.block, "ms": r.ms, "trials": r.trials}
+
+
+# ---------------------------------------------------------------------------
+# batched ALS over many independent problems
+# ---------------------------------------------------------------------------
+class Batch:
+    """Independent completion problems solved together with ALS.
+
+    Problems are grouped by rank (8/16/32/64); each rank class is one
+    block-diagonal tensor, so every ALS phase is one kernel launch over all
+    problems of the class, and `als(iters, graph=True)` replays a hipGraph of
+    the whole run with one branch per class.  A problem's factor of mode m has
+    max(train.dims[m], test.dims[m]) rows."""
+
+    def __init__(self, ctx: Context, trains, tests, ranks):
+        n = len(trains)
+        assert len(ranks) == n and (tests is None or len(tests) == n)
+        self.ctx = ctx
+        self._keep = list(trains) + list(tests or [])
+        tr = (C.POINTER(_lib.Coo) * n)(*[t.c for t in trains])
+        te = (C.POINTER(_lib.Coo) * n)(*[(t.c if t is not None else None) for t in tests]) if tests else None
+        rk = (C.c_uint32 * n)(*[int(r) for r in ranks])
+        self.ranks = [int(r) for r in ranks]
+        self.dims = [tuple(max(a, b) for a, b in zip(trains[p].dims, tests[p].dims)) if tests and tests[p] is not None
+                     else tuple(trains[p].dims) for p in range(n)]
+        self.nproblems = n
+        self._h = C.c_void_p()
+        check(lib().ctc_batch_create(ctx._h, n, tr, te, rk, C.byref(self._h)))
+        np_, nc, nnz = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().ctc_batch_info(self._h, C.byref(np_), C.byref(nc), C.byref(nnz)))
+        self.nclasses, self.train_nnz = nc.value, nnz.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib.alive():
+            lib().ctc_batch_destroy(self._h)
+            self._h = None
+
+    def set_factor(self, p, mode, F):
+        F = np.ascontiguousarray(F, dtype=np.float32)
+        assert F.shape == (self.dims[p][mode], self.ranks[p]), F.shape
+        check(lib().ctc_batch_set_factor(self._h, p, mode, F.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def factor(self, p, mode):
+        out = np.empty((self.dims[p][mode], self.ranks[p]), np.float32)
+        check(lib().ctc_batch_get_factor(self._h, p, mode, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def classes(self):
+        out = []
+        for k in range(self.nclasses):
+            r, n = C.c_uint32(), C.c_uint32()
+            check(lib().ctc_batch_class_rank(self._h, k, C.byref(r), C.byref(n)))
+            out.append((r.value, n.value))
+        return out
+
+    def autotune(self, opt: Options):
+        o = opt.to_c()
+        res = (_lib.TuneResult * 4)()
+        n = C.c_uint32()
+        check(lib().ctc_batch_autotune(self._h, C.byref(o), res, C.byref(n)))
+        return [{"tile_len": r.tile_len, "block": r.block, "ms": r.ms, "trials": r.trials} for r in res[:n.value]]
+
+    def als(self, opt: Options, iters=10, graph=True):
+        """Runs `iters` ALS iterations on every problem; returns device ms."""
+        o = opt.to_c()
+        ms = C.c_double()
+        check(lib().ctc_batch_als(self._h, C.byref(o), int(iters), 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def rmse(self):
+        tr = np.zeros(self.nproblems)
+        te = np.zeros(self.nproblems)
+        check(lib().ctc_batch_rmse(self._h, tr.ctypes.data_as(C.POINTER(C.c_double)),
+                                   te.ctypes.data_as(C.POINTER(C.c_double))))
+        return tr, te
 
 
 # ---------------------------------------------------------------------------

@@ -199,6 +199,15 @@ SIGNATURES = {
                              C.c_int, C.c_double, P(C.c_double), P(C.c_double), P(C.c_uint64),
                              P(C.c_uint64), P(C.c_uint64)]),
     "ctc_autotune": (C.c_int, [vp, P(Options), P(TuneResult)]),
+    "ctc_batch_create": (C.c_int, [vp, C.c_uint32, P(P(Coo)), P(P(Coo)), P(C.c_uint32), P(vp)]),
+    "ctc_batch_destroy": (None, [vp]),
+    "ctc_batch_info": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]),
+    "ctc_batch_class_rank": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
+    "ctc_batch_set_factor": (C.c_int, [vp, C.c_uint32, C.c_uint32, P(C.c_float)]),
+    "ctc_batch_get_factor": (C.c_int, [vp, C.c_uint32, C.c_uint32, P(C.c_float)]),
+    "ctc_batch_autotune": (C.c_int, [vp, P(Options), P(TuneResult), P(C.c_uint32)]),
+    "ctc_batch_als": (C.c_int, [vp, P(Options), C.c_uint32, C.c_int32, P(C.c_double)]),
+    "ctc_batch_rmse": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
 }
 
 _lib = None

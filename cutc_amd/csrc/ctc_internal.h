@@ -74,6 +74,11 @@ struct DevCiss {
   uint32_t *grp_piece = nullptr;  // ngroups: first piece of group g
   uint32_t *grp_n = nullptr;      // ngroups: pieces in group g
   uint32_t *slot_gb = nullptr;    // dlength + 1: groups of slot s (empty for light slots)
+  // BF16-operand ALS work items (see kernels_mixed.h) for rows cut at mix_chunk nonzeros
+  uint32_t mix_chunk = 0;
+  uint64_t mix_nitems = 0, mix_nheavy = 0, mix_nparts = 0;
+  uint4 *mix_items = nullptr;
+  uint32_t *mix_heavy_slot = nullptr, *mix_heavy_pb = nullptr;
   // host copies needed by the piece-table builder
   std::vector<uint32_t> h_tile_slot;
   std::vector<uint64_t> h_tile_mask;
@@ -142,6 +147,8 @@ struct ctc_model {
   int ccd_pending = -1;  // column whose new component the residual copies still hold
   // SGD hot-row bitmaps (rows with more nonzeros than hot_threshold)
   uint32_t *hotbits[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  // BF16 mirrors of the factors (mixed-precision ALS)
+  uint16_t *Fbf[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t hot_threshold = 0;
 
 };
@@ -152,6 +159,13 @@ ctc_status ensure_ciss(ctc_model *m, uint32_t mode);
 ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles);
 // one CCD++ epoch on the device (ccd.cpp)
 ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o);
+// one BF16-operand ALS mode update (ranks 32, 64, 128), enqueued on the model's stream
+ctc_status als_mode_mixed(ctc_model *m, uint32_t mode, const ctc_options *o);
+// one ALS mode update, enqueued on the model's stream (no synchronisation, so
+// it can be captured into a graph once the model is prepared)
+ctc_status als_mode_impl(ctc_model *m, uint32_t mode, const ctc_options *o);
+// reads and clears the model's not-positive-definite flag (synchronises)
+ctc_status check_not_spd(ctc_model *m);
 // profiled launch bracketing
 void prof_begin(ctc_context *c, const char *name, hipEvent_t *a, hipEvent_t *b);
 void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b);

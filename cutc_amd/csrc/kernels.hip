@@ -642,6 +642,35 @@ __global__ void __launch_bounds__(256) k_eval_ciss(CissArgs c, FactorArgs fa, in
   block_store_sum(acc, part + blockIdx.x);
 }
 
+// squared error per segment [seg[b], seg[b+1]) of the nonzeros, one block per
+// segment (a segment = one problem of a batch); CISS=true reads mode-0 CISS
+// order, false a COO
+template <int N, bool CISS>
+__global__ void __launch_bounds__(256) k_eval_seg(CissArgs c, CooArgs co, FactorArgs fa, int R,
+                                                  const uint64_t *seg, double *out) {
+  double acc = 0.0;
+  const float *F[4] = {fa.f[0], fa.f[1], fa.f[2], fa.f[3]};
+  const uint64_t p1 = seg[blockIdx.x + 1];
+  for (uint64_t p = seg[blockIdx.x] + threadIdx.x; p < p1; p += blockDim.x) {
+    uint32_t ix[4];
+    float x;
+    if (CISS) {
+      const uint64_t t = p >> 6;
+      ix[0] = c.directory[tile_lane_slot(c.tile_slot[t], c.tile_mask[t], (int)(p & 63))];
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) ix[1 + s] = c.oidx[s][p];
+      x = c.vals[p];
+    } else {
+#pragma unroll
+      for (int m = 0; m < N; ++m) ix[m] = co.ind[m][p];
+      x = co.vals[p];
+    }
+    const double e = (double)x - (double)predict_row<N>(F, ix, R);
+    acc += e * e;
+  }
+  block_store_sum(acc, out + blockIdx.x);
+}
+
 __global__ void __launch_bounds__(256) k_sumsq(const float *x, uint64_t n, double *part) {
   double acc = 0.0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
@@ -668,6 +697,14 @@ __global__ void __launch_bounds__(256) k_fill(float *x, uint64_t n, float v) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x)
     x[p] = v;
+}
+
+// plain streaming copy of n 16-byte words: the bandwidth yardstick and the
+// known-size kernel that calibrates HBM traffic counters
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x)
+    dst[p] = src[p];
 }
 
 // grad += lambda * c_i * F  for the rows of a CISS directory (c_i = row nnz)
@@ -855,6 +892,28 @@ hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f
   return hipGetLastError();
 }
 
+hipError_t launch_eval_seg(hipStream_t st, const CissArgs *c, const CooArgs *co, const FactorArgs &f,
+                           int nmodes, int R, const uint64_t *seg, uint32_t nseg, double *out) {
+  if (nseg == 0) return hipSuccess;
+  const CissArgs ca = c ? *c : CissArgs{};
+  const CooArgs cb = co ? *co : CooArgs{};
+#define CTC_SEG(NN)                                                                                    \
+  do {                                                                                                \
+    if (c)                                                                                            \
+      hipLaunchKernelGGL((k_eval_seg<NN, true>), dim3(nseg), dim3(256), 0, st, ca, cb, f, R, seg, out);  \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_eval_seg<NN, false>), dim3(nseg), dim3(256), 0, st, ca, cb, f, R, seg, out); \
+  } while (0)
+  if (nmodes == 3)
+    CTC_SEG(3);
+  else if (nmodes == 4)
+    CTC_SEG(4);
+  else
+    return hipErrorInvalidValue;
+#undef CTC_SEG
+  return hipGetLastError();
+}
+
 hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                             unsigned nblocks, double *part) {
   if (nmodes == 3)
@@ -882,6 +941,15 @@ hipError_t launch_fill(hipStream_t st, float *x, uint64_t n, float v) {
   uint64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, x, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t st, const void *src, void *dst, uint64_t bytes) {
+  const uint64_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n);
   return hipGetLastError();
 }
 

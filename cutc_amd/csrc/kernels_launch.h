@@ -24,12 +24,16 @@ hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t
                             double lam, bool refine, float *A, uint32_t *flag);
 hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f, int nmodes, int R,
                            unsigned nblocks, double *part);
+// per-segment squared error: exactly one of c (mode-0 CISS order) / co (COO) is non-null
+hipError_t launch_eval_seg(hipStream_t st, const CissArgs *c, const CooArgs *co, const FactorArgs &f,
+                           int nmodes, int R, const uint64_t *seg, uint32_t nseg, double *out);
 hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                             unsigned nblocks, double *part);
 hipError_t launch_sumsq(hipStream_t st, const float *x, uint64_t n, unsigned nblocks, double *part);
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, uint32_t nb, uint32_t groups,
                                double *out);
 hipError_t launch_fill(hipStream_t st, float *x, uint64_t n, float v);
+hipError_t launch_copy(hipStream_t st, const void *src, void *dst, uint64_t bytes);  // bytes % 16 == 0
 hipError_t launch_reg_rows(hipStream_t st, const uint32_t *directory, const uint32_t *rowptr,
                            uint64_t nrows, int R, float lam, int per_occ, const float *F, float *G,
                            unsigned nblocks, double *part);

@@ -1,0 +1,38 @@
+// kernels_mixed.h — BF16-operand ALS mode update (ranks 32, 64, 128).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ctc {
+
+// Work items of one mode: {slot, first nonzero, end nonzero, part}.  part ==
+// kLightRow: the whole row, Gram and solve in one workgroup; otherwise the
+// chunk writes its partial Gram and right-hand side to partial `part`.
+constexpr uint32_t kLightRow = 0xFFFFFFFFu;
+
+struct MixedArgs {
+  const uint32_t *oidx[3];  // other-mode ids in CISS order (CISS-relative modes 1..N-1)
+  const float *vals;
+  const uint32_t *directory;
+  const uint16_t *fb[3];  // BF16 mirrors of the other modes' factors (CISS-relative 1..N-1)
+  float *out;             // factor of the mode being updated, row-major dims x R
+  const uint4 *items;
+  uint64_t nitems;
+  float *part;                 // partials: (R * R + R) floats each
+  const uint32_t *heavy_slot;  // heavy rows: directory slot
+  const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])
+  uint64_t nheavy;
+  double lam;
+  int refine;  // iterative-refinement steps after the FP64 Cholesky solve
+  uint32_t *flag;
+};
+
+// items pass (light rows solved, heavy chunks to partials), then heavy rows
+hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+// FP32 -> BF16 (round to nearest even), n elements
+hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n);
+
+}  // namespace ctc

@@ -1,0 +1,571 @@
+// kernels_mixed.hip — BF16-operand ALS mode update for ranks 32, 64 and 128.
+//
+// The same rule as the FP32 path (SG/als.cu:186-317 Gram + MTTKRP, then
+// SG/als.cu:467-491,715-716 (G + lambda I) a = m per row), in the precision
+// split of the mixed path: h = product of the BF16 factor rows (exact in
+// FP32), G = sum bf16(h) bf16(h)^T on BF16 MFMA with FP32 accumulation,
+// m = sum x h in FP32, and the R x R system factored and solved in FP64 with
+// iterative refinement against the FP32 Gram.
+//
+// One workgroup of 2R threads (R/32 waves) owns one row.  The row-major
+// per-row Gram never touches HBM: wave w accumulates the 32-row stripe w of
+// G (lower 32x32 blocks only) in MFMA accumulators, while the workgroup
+// stages 32 nonzeros' h vectors at a time in LDS as BF16.  The system then
+// moves into LDS as FP64 16x16 tiles (lower triangle, R/16 tile rows) and is
+// factored by a blocked right-looking Cholesky: a wave factors and inverts
+// the diagonal tile in registers, the panel below is multiplied by that
+// inverse and the trailing tiles are updated, both on FP64 MFMA.  The forward
+// substitution rides along the panel steps; the back substitution and the
+// refinement solves walk the tile rows with the inverted diagonal tiles.
+// Rows too long for one workgroup are cut into chunks whose partial Gram and
+// right-hand side go to HBM and are summed by a second pass before the solve.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels_mixed.h"
+
+namespace ctc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ inline float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
+
+// LDS ordering between lanes of one wave (no workgroup barrier)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v_mfma_f32_32x32x16_bf16 accumulator v of lane l: row 8(v/4) + 4(l/32) + v%4, column l%32
+__device__ inline int row32(int v, int l) { return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3); }
+// v_mfma_f64_16x16x4f64 accumulator v of lane l: row 4v + l/16, column l%16
+__device__ inline int row16(int v, int l) { return 4 * v + (l >> 4); }
+
+// 16x16 FP64 tile, element (r, c) with an XOR swizzle of the column
+__device__ inline int tix(int r, int c) { return (r << 4) | (c ^ r); }
+__device__ inline int tileno(int I, int J) { return I * (I + 1) / 2 + J; }
+
+constexpr int kHTS = 20;  // dwords per staged feature row: 32 BF16 nonzeros + pad
+
+template <int R>
+struct MixedSmem {
+  static constexpr int NW = R / 32, RT = R / 16, NTILE = RT * (RT + 1) / 2;
+  double tiles[NTILE * 256];  // FP64 lower tiles; the Gram phase stages h here
+  double vm[R], vx[R], vy[R];
+  double contrib[NW][R];
+  int bad;
+};
+
+// ------------------------------------------------------------ Gram phase --
+// acc[tt] (tt <= w) += stripe w x stripe tt of sum bf16(h) bf16(h)^T over
+// nonzeros [p0, p1); returns this thread's FP32 partial of m for features
+// 8 fg .. 8 fg + 7 (fg = thread % (R/8)) in mp.
+template <int R, int N>
+__device__ inline void gram_chunk(const MixedArgs &a, uint32_t p0, uint32_t p1, f32x16 *acc, float *mp,
+                                  uint32_t *ht) {
+  constexpr int NW = R / 32, FG8 = R / 8;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int fg = tid % FG8, qp = tid / FG8;  // features 8fg..8fg+7, nonzeros 2qp, 2qp+1
+  for (uint32_t base = p0; base < p1; base += 32) {
+    uint32_t hv[8][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t p = base + 2 * qp + h;
+      if (p < p1) {
+        float hf[8];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) hf[f] = 1.f;
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) {
+          const uint4 u = *(const uint4 *)(a.fb[s] + (size_t)a.oidx[s][p] * R + 8 * fg);
+          hf[0] *= bf_lo(u.x);
+          hf[1] *= bf_hi(u.x);
+          hf[2] *= bf_lo(u.y);
+          hf[3] *= bf_hi(u.y);
+          hf[4] *= bf_lo(u.z);
+          hf[5] *= bf_hi(u.z);
+          hf[6] *= bf_lo(u.w);
+          hf[7] *= bf_hi(u.w);
+        }
+        const float x = a.vals[p];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          mp[f] += x * hf[f];
+          hv[f][h] = f2bf(hf[f]);
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) hv[f][h] = 0u;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      const int r = 8 * fg + f;
+      ht[r * kHTS + (qp ^ (4 * ((r >> 3) & 3)))] = hv[f][0] | (hv[f][1] << 16);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int ra = 32 * w + (l & 31), ch = 2 * s + (l >> 5);
+      const uint4 ua = *(const uint4 *)(ht + ra * kHTS + 4 * (ch ^ ((ra >> 3) & 3)));
+      const bf16x8 av = __builtin_bit_cast(bf16x8, ua);
+#pragma unroll
+      for (int tt = 0; tt < NW; ++tt) {
+        if (tt > w) continue;  // wave-uniform: lower stripes only
+        const int rb = 32 * tt + (l & 31);
+        const uint4 ub = *(const uint4 *)(ht + rb * kHTS + 4 * (ch ^ ((rb >> 3) & 3)));
+        acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, ub), acc[tt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// sums the per-thread m partials of the workgroup into sm.contrib[0][*] (as
+// double); ends with a barrier
+template <int R>
+__device__ inline void reduce_m(float *mp, MixedSmem<R> &sm) {
+  constexpr int NW = R / 32, FG8 = R / 8;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, fg = tid % FG8;
+#pragma unroll
+  for (int o = FG8; o < 64; o <<= 1)
+#pragma unroll
+    for (int f = 0; f < 8; ++f) mp[f] += __shfl_xor(mp[f], o, 64);
+  if (l < FG8)
+#pragma unroll
+    for (int f = 0; f < 8; ++f) sm.contrib[w][8 * fg + f] = (double)mp[f];
+  __syncthreads();
+  if (tid < R) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) s += sm.contrib[ww][tid];
+    sm.vm[tid] = s;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------- solve phase --
+// wave-level: factor tile (K, K) (Cholesky), replace it by the inverse of its
+// lower factor; with fold, also y_K = inverse * vy_K in place
+template <int R>
+__device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
+  double *T = sm.tiles + tileno(K, K) * 256;
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  double a[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) a[e] = T[tix(4 * q + e, c)];
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double akk = __shfl(a[k & 3], k + 16 * (k >> 2), 64);
+    ok = ok && (akk > 0.0);
+    const double lkk = sqrt(akk > 0.0 ? akk : 1.0);
+    const double inv = 1.0 / lkk;
+    double cik[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cik[e] = __shfl(a[e], k + 16 * q, 64) * inv;
+    const double t0 = __shfl(a[0], k + 16 * (c >> 2), 64);
+    const double t1 = __shfl(a[1], k + 16 * (c >> 2), 64);
+    const double t2 = __shfl(a[2], k + 16 * (c >> 2), 64);
+    const double t3 = __shfl(a[3], k + 16 * (c >> 2), 64);
+    const double cck = ((c & 3) == 0 ? t0 : (c & 3) == 1 ? t1 : (c & 3) == 2 ? t2 : t3) * inv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = 4 * q + e;
+      if (c == k) {
+        if (i == k)
+          a[e] = lkk;
+        else if (i > k)
+          a[e] = cik[e];
+      } else if (c > k && i > k) {
+        a[e] -= cik[e] * cck;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = (4 * q + e >= c) ? a[e] : 0.0;
+  wave_sync();
+  // X = L^-1, column c: lane (c, q) holds X[4q + e][c]
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (4 * q + e < i) s += T[tix(i, 4 * q + e)] * x[e];
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    const double lii = T[tix(i, i)];
+    if (q == (i >> 2)) x[i & 3] = ((i == c ? 1.0 : 0.0) - s) / lii;
+  }
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = x[e];
+  if (!ok && l == 0) sm.bad = 1;
+  if (fold) {
+    double pv[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pv[e] = x[e] * sm.vy[16 * K + c];
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pv[e] += __shfl_xor(pv[e], o, 64);
+    wave_sync();
+    if (c == 0)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sm.vy[16 * K + 4 * q + e] = pv[e];
+  }
+  wave_sync();
+}
+
+// wave-level: L_IK = A_IK X_KK^T (panel), then vy_I -= L_IK vy_K
+template <int R>
+__device__ inline void panel_step(MixedSmem<R> &sm, int I, int K) {
+  double *Tik = sm.tiles + tileno(I, K) * 256;
+  const double *X = sm.tiles + tileno(K, K) * 256;
+  const int l = threadIdx.x & 63;
+  f64x4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k0 = 0; k0 < 16; k0 += 4) {
+    const double av = Tik[tix(l & 15, k0 + (l >> 4))];
+    const double bv = X[tix(l & 15, k0 + (l >> 4))];
+    d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+  }
+  wave_sync();
+  double pv[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    Tik[tix(row16(v, l), l & 15)] = d[v];
+    pv[v] = d[v] * sm.vy[16 * K + (l & 15)];
+  }
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pv[v] += __shfl_xor(pv[v], o, 64);
+  if ((l & 15) == 0)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sm.vy[16 * I + row16(v, l)] -= pv[v];
+  wave_sync();
+}
+
+// wave-level: A_IJ -= L_IK L_JK^T
+template <int R>
+__device__ inline void trail_step(MixedSmem<R> &sm, int I, int J, int K) {
+  double *Tij = sm.tiles + tileno(I, J) * 256;
+  const double *Tik = sm.tiles + tileno(I, K) * 256;
+  const double *Tjk = sm.tiles + tileno(J, K) * 256;
+  const int l = threadIdx.x & 63;
+  f64x4 d;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) d[v] = Tij[tix(row16(v, l), l & 15)];
+#pragma unroll
+  for (int k0 = 0; k0 < 16; k0 += 4) {
+    const double av = -Tik[tix(l & 15, k0 + (l >> 4))];
+    const double bv = Tjk[tix(l & 15, k0 + (l >> 4))];
+    d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Tij[tix(row16(v, l), l & 15)] = d[v];
+}
+
+// wave-level forward pieces: vy_I -= L_IK vy_K ; vy_K = X_KK vy_K
+template <int R>
+__device__ inline void fwd_update(MixedSmem<R> &sm, int I, int K) {
+  const double *T = sm.tiles + tileno(I, K) * 256;
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  double pv[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) pv[e] = T[tix(4 * q + e, c)] * sm.vy[16 * K + c];
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pv[e] += __shfl_xor(pv[e], o, 64);
+  if (c == 0)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) sm.vy[16 * I + 4 * q + e] -= pv[e];
+  wave_sync();
+}
+
+template <int R>
+__device__ inline void fwd_diag(MixedSmem<R> &sm, int K) {
+  const double *X = sm.tiles + tileno(K, K) * 256;
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  double pv[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) pv[e] = X[tix(4 * q + e, c)] * sm.vy[16 * K + c];
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pv[e] += __shfl_xor(pv[e], o, 64);
+  wave_sync();
+  if (c == 0)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) sm.vy[16 * K + 4 * q + e] = pv[e];
+  wave_sync();
+}
+
+// wave-level backward pieces: vy_J -= L_KJ^T dst_K ; dst_K = X_KK^T vy_K
+template <int R>
+__device__ inline void back_update(MixedSmem<R> &sm, const double *dst, int K, int J) {
+  const double *T = sm.tiles + tileno(K, J) * 256;
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  double s = 0.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) s += T[tix(4 * q + e, c)] * dst[16 * K + 4 * q + e];
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  if (q == 0) sm.vy[16 * J + c] -= s;
+  wave_sync();
+}
+
+template <int R>
+__device__ inline void back_diag(MixedSmem<R> &sm, double *dst, int K) {
+  const double *X = sm.tiles + tileno(K, K) * 256;
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  double s = 0.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) s += X[tix(4 * q + e, c)] * sm.vy[16 * K + 4 * q + e];
+  s += __shfl_xor(s, 16, 64);
+  s += __shfl_xor(s, 32, 64);
+  if (q == 0) dst[16 * K + c] = s;
+  wave_sync();
+}
+
+// dst = L^-T vy (vy is consumed)
+template <int R>
+__device__ inline void back_solve(MixedSmem<R> &sm, double *dst) {
+  constexpr int NW = R / 32, RT = R / 16;
+  const int w = threadIdx.x >> 6;
+  if (w == (RT - 1) % NW) back_diag(sm, dst, RT - 1);
+  __syncthreads();
+  for (int K = RT - 1; K >= 1; --K) {
+    for (int J = 0; J < K; ++J)
+      if (w == J % NW) {
+        back_update(sm, dst, K, J);
+        if (J == K - 1) back_diag(sm, dst, K - 1);
+      }
+    __syncthreads();
+  }
+}
+
+// vy = L^-1 vy
+template <int R>
+__device__ inline void fwd_solve(MixedSmem<R> &sm) {
+  constexpr int NW = R / 32, RT = R / 16;
+  const int w = threadIdx.x >> 6;
+  if (w == 0) fwd_diag(sm, 0);
+  __syncthreads();
+  for (int K = 0; K < RT - 1; ++K) {
+    for (int I = K + 1; I < RT; ++I)
+      if (w == I % NW) {
+        fwd_update(sm, I, K);
+        if (I == K + 1) fwd_diag(sm, K + 1);
+      }
+    __syncthreads();
+  }
+}
+
+// (G + lambda I) x = m with G in the MFMA accumulators (lower stripes) and m
+// in sm.vm; returns false if the system is not positive definite
+template <int R>
+__device__ inline bool solve_row(MixedSmem<R> &sm, const f32x16 *acc, double lam, int refine) {
+  constexpr int NW = R / 32, RT = R / 16;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  // A = G + lambda I into the lower FP64 tiles
+#pragma unroll
+  for (int tt = 0; tt < NW; ++tt) {
+    if (tt > w) continue;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int i = 32 * w + row32(v, l), j = 32 * tt + (l & 31);
+      if ((i >> 4) >= (j >> 4))
+        sm.tiles[tileno(i >> 4, j >> 4) * 256 + tix(i & 15, j & 15)] = (double)acc[tt][v] + (i == j ? lam : 0.0);
+    }
+  }
+  if (tid < R) sm.vy[tid] = sm.vm[tid];
+  if (tid == 0) sm.bad = 0;
+  __syncthreads();
+  // blocked right-looking Cholesky; vy becomes y = L^-1 m on the way
+  for (int K = 0; K < RT; ++K) {
+    if (w == K % NW) diag_step(sm, K, true);
+    __syncthreads();
+    for (int I = K + 1; I < RT; ++I)
+      if (w == I % NW) panel_step(sm, I, K);
+    __syncthreads();
+    if (K + 1 < RT) {
+      int u = 0;
+      for (int I = K + 1; I < RT; ++I)
+        for (int J = K + 1; J <= I; ++J, ++u)
+          if (u % NW == w) trail_step(sm, I, J, K);
+      __syncthreads();
+    }
+  }
+  if (sm.bad) return false;
+  back_solve(sm, sm.vx);
+  for (int it = 0; it < refine; ++it) {
+    // r = m - lambda x - G x; G x from the FP32 accumulators: stripe rows of
+    // every lower block plus, for off-diagonal blocks, the transposed block
+    double rs[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) rs[v] = 0.0;
+    for (int k = l; k < R; k += 64) sm.contrib[w][k] = 0.0;
+    wave_sync();
+#pragma unroll
+    for (int tt = 0; tt < NW; ++tt) {
+      if (tt > w) continue;
+      const double xj = sm.vx[32 * tt + (l & 31)];
+      double cs = 0.0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        rs[v] += (double)acc[tt][v] * xj;
+        cs += (double)acc[tt][v] * sm.vx[32 * w + row32(v, l)];
+      }
+      if (tt < w) {
+        cs += __shfl_xor(cs, 32, 64);
+        if (l < 32) sm.contrib[w][32 * tt + l] = cs;
+      }
+    }
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) rs[v] += __shfl_xor(rs[v], o, 64);
+    if ((l & 31) == 0)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) sm.contrib[w][32 * w + row32(v, l)] = rs[v];
+    __syncthreads();
+    if (tid < R) {
+      double r = sm.vm[tid] - lam * sm.vx[tid];
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) r -= sm.contrib[ww][tid];
+      sm.vy[tid] = r;
+    }
+    __syncthreads();
+    fwd_solve(sm);
+    back_solve(sm, sm.contrib[0]);
+    if (tid < R) sm.vx[tid] += sm.contrib[0][tid];
+    __syncthreads();
+  }
+  return true;
+}
+
+template <int R>
+__device__ inline void store_row(const MixedArgs &a, MixedSmem<R> &sm, uint32_t slot, bool ok) {
+  const int tid = threadIdx.x;
+  float *dst = a.out + (size_t)a.directory[slot] * R;
+  if (tid < R) dst[tid] = ok ? (float)sm.vx[tid] : 0.f;
+  if (!ok && tid == 0) atomicOr(a.flag, 1u);
+}
+
+// ---------------------------------------------------------------- kernels --
+template <int R, int N>
+__global__ void __launch_bounds__(2 * R) k_mixed_items(MixedArgs a) {
+  constexpr int NW = R / 32;
+  __shared__ MixedSmem<R> sm;
+  const uint4 it = a.items[blockIdx.x];
+  f32x16 acc[NW];
+#pragma unroll
+  for (int tt = 0; tt < NW; ++tt)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[tt][v] = 0.f;
+  float mp[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  gram_chunk<R, N>(a, it.y, it.z, acc, mp, (uint32_t *)sm.tiles);
+  reduce_m<R>(mp, sm);
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  if (it.w != kLightRow) {  // chunk of a heavy row: partial out
+    float *P = a.part + (size_t)it.w * ((size_t)R * R + R);
+#pragma unroll
+    for (int tt = 0; tt < NW; ++tt) {
+      if (tt > w) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) P[((w * NW + tt) * 16 + v) * 64 + l] = acc[tt][v];
+    }
+    if (tid < R) P[R * R + tid] = (float)sm.vm[tid];
+    return;
+  }
+  const bool ok = solve_row<R>(sm, acc, a.lam, a.refine);
+  store_row<R>(a, sm, it.x, ok);
+}
+
+template <int R, int N>
+__global__ void __launch_bounds__(2 * R) k_mixed_heavy(MixedArgs a) {
+  constexpr int NW = R / 32;
+  __shared__ MixedSmem<R> sm;
+  const uint32_t h = blockIdx.x;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  f32x16 acc[NW];
+#pragma unroll
+  for (int tt = 0; tt < NW; ++tt)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[tt][v] = 0.f;
+  double mv = 0.0;
+  for (uint32_t pi = a.heavy_pb[h]; pi < a.heavy_pb[h + 1]; ++pi) {
+    const float *P = a.part + (size_t)pi * ((size_t)R * R + R);
+#pragma unroll
+    for (int tt = 0; tt < NW; ++tt) {
+      if (tt > w) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[tt][v] += P[((w * NW + tt) * 16 + v) * 64 + l];
+    }
+    if (tid < R) mv += (double)P[R * R + tid];
+  }
+  if (tid < R) sm.vm[tid] = mv;
+  __syncthreads();
+  const bool ok = solve_row<R>(sm, acc, a.lam, a.refine);
+  store_row<R>(a, sm, a.heavy_slot[h], ok);
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16(const float *x, uint16_t *y, uint64_t n) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x)
+    y[p] = f2bf(x[p]);
+}
+
+#define CTC_MIXED_SWITCH(KERNEL, GRID)                                                                    \
+  do {                                                                                                     \
+    const dim3 g((unsigned)(GRID)), b((unsigned)(2 * R));                                                 \
+    if (nmodes == 3) {                                                                                     \
+      if (R == 32) hipLaunchKernelGGL((KERNEL<32, 3>), g, dim3(64), 0, st, a);                            \
+      else if (R == 64) hipLaunchKernelGGL((KERNEL<64, 3>), g, dim3(128), 0, st, a);                      \
+      else if (R == 128) hipLaunchKernelGGL((KERNEL<128, 3>), g, dim3(256), 0, st, a);                    \
+      else return hipErrorInvalidValue;                                                                    \
+    } else if (nmodes == 4) {                                                                              \
+      if (R == 32) hipLaunchKernelGGL((KERNEL<32, 4>), g, dim3(64), 0, st, a);                            \
+      else if (R == 64) hipLaunchKernelGGL((KERNEL<64, 4>), g, dim3(128), 0, st, a);                      \
+      else if (R == 128) hipLaunchKernelGGL((KERNEL<128, 4>), g, dim3(256), 0, st, a);                    \
+      else return hipErrorInvalidValue;                                                                    \
+    } else {                                                                                               \
+      return hipErrorInvalidValue;                                                                         \
+    }                                                                                                      \
+    (void)b;                                                                                               \
+  } while (0)
+
+hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
+  if (a.nitems == 0) return hipSuccess;
+  CTC_MIXED_SWITCH(k_mixed_items, a.nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
+  if (a.nheavy == 0) return hipSuccess;
+  CTC_MIXED_SWITCH(k_mixed_heavy, a.nheavy);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)blocks), dim3(256), 0, st, x, y, n);
+  return hipGetLastError();
+}
+
+}  // namespace ctc

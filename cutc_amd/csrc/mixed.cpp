@@ -1,0 +1,112 @@
+// mixed.cpp — host driver of the BF16-operand ALS mode update (kernels in
+// kernels_mixed.hip).
+//
+// Per mode update: the other modes' factors are rounded to BF16 mirrors, the
+// factor being updated is zeroed (rows without nonzeros stay 0, as
+// SG/als.cu:699 does), the items pass solves every row of at most
+// `chunk` nonzeros in one workgroup and writes partial systems for the chunks
+// of longer rows, and the heavy pass sums each long row's partials and
+// solves it.
+#include <algorithm>
+#include <vector>
+
+#include "ctc_internal.h"
+#include "kernels_launch.h"
+#include "kernels_mixed.h"
+#include "model_util.h"
+
+namespace ctc {
+
+// nonzeros per work item: the option's tile length, a multiple of 32
+static uint32_t mix_chunk_of(const ctc_options *o) {
+  uint32_t c = o && o->tile_len ? o->tile_len : 2048;
+  return std::max<uint32_t>(32, (c + 31) / 32 * 32);
+}
+
+static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) {
+  DevCiss &d = m->ciss[mode];
+  if (d.mix_items && d.mix_chunk == chunk) return CTC_OK;
+  std::vector<uint4> chunks, light;
+  std::vector<uint32_t> hslot, hpb{0};
+  uint32_t nparts = 0;
+  for (uint64_t s = 0; s < d.dlength; ++s) {
+    const uint32_t p0 = d.h_rowptr[s], p1 = d.h_rowptr[s + 1];
+    if (p1 - p0 <= chunk) {
+      light.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
+      continue;
+    }
+    for (uint32_t p = p0; p < p1; p += chunk) chunks.push_back(make_uint4((uint32_t)s, p, std::min(p1, p + chunk), nparts++));
+    hslot.push_back((uint32_t)s);
+    hpb.push_back(nparts);
+  }
+  // the long rows' chunks first, so the longest work starts earliest
+  std::vector<uint4> items;
+  items.reserve(chunks.size() + light.size());
+  items.insert(items.end(), chunks.begin(), chunks.end());
+  items.insert(items.end(), light.begin(), light.end());
+  for (void *p : {(void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb})
+    if (p) CTC_HIP(hipFree(p));
+  d.mix_items = nullptr;
+  d.mix_heavy_slot = d.mix_heavy_pb = nullptr;
+  hipStream_t st = m->ctx->stream;
+  CTC_HIP(hipMalloc(&d.mix_items, std::max<size_t>(1, items.size()) * sizeof(uint4)));
+  CTC_HIP(hipMalloc(&d.mix_heavy_slot, std::max<size_t>(1, hslot.size()) * sizeof(uint32_t)));
+  CTC_HIP(hipMalloc(&d.mix_heavy_pb, hpb.size() * sizeof(uint32_t)));
+  if (!items.empty())
+    CTC_HIP(hipMemcpyAsync(d.mix_items, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+  if (!hslot.empty())
+    CTC_HIP(hipMemcpyAsync(d.mix_heavy_slot, hslot.data(), hslot.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           st));
+  CTC_HIP(hipMemcpyAsync(d.mix_heavy_pb, hpb.data(), hpb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  CTC_HIP(hipStreamSynchronize(st));
+  d.mix_chunk = chunk;
+  d.mix_nitems = items.size();
+  d.mix_nheavy = hslot.size();
+  d.mix_nparts = nparts;
+  return CTC_OK;
+}
+
+ctc_status als_mode_mixed(ctc_model *m, uint32_t mode, const ctc_options *o) {
+  const uint32_t R = m->rank, N = m->nmodes;
+  if (R != 32 && R != 64 && R != 128)
+    return fail(CTC_ERR_UNSUPPORTED, "mixed-precision ALS supports rank 32, 64 or 128 (got %u)", R);
+  const uint32_t chunk = mix_chunk_of(o);
+  CTC_TRY(ensure_ciss(m, mode));
+  CTC_TRY(ensure_mix_items(m, mode, chunk));
+  DevCiss &d = m->ciss[mode];
+  CTC_TRY(ensure_scratch(m, std::max<size_t>(1, d.mix_nparts) * ((size_t)R * R + R) * sizeof(float)));
+  hipStream_t st = m->ctx->stream;
+  MixedArgs a{};
+  for (uint32_t s = 0; s + 1 < N; ++s) {
+    const uint32_t md = (mode + 1 + s) % N;
+    if (!m->Fbf[md]) CTC_HIP(hipMalloc(&m->Fbf[md], m->dims[md] * R * sizeof(uint16_t)));
+    CTC_HIP(launch_to_bf16(st, m->F[md], m->Fbf[md], m->dims[md] * R));
+    a.oidx[s] = d.oidx[s];
+    a.fb[s] = m->Fbf[md];
+  }
+  a.vals = d.vals;
+  a.directory = d.directory;
+  a.out = m->F[mode];
+  a.items = d.mix_items;
+  a.nitems = d.mix_nitems;
+  a.part = (float *)m->scratch;
+  a.heavy_slot = d.mix_heavy_slot;
+  a.heavy_pb = d.mix_heavy_pb;
+  a.nheavy = d.mix_nheavy;
+  a.lam = o->lambda;
+  a.refine = o->refine > 0 ? o->refine : 1;
+  a.flag = m->dflag;
+  CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
+  {
+    Prof p(m->ctx, "als_mixed_items");
+    CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
+  }
+  if (a.nheavy) {
+    Prof p(m->ctx, "als_mixed_heavy");
+    CTC_HIP(launch_mixed_heavy(st, a, (int)N, (int)R));
+  }
+  m->resid_valid = false;
+  return CTC_OK;
+}
+
+}  // namespace ctc

@@ -276,10 +276,11 @@ extern "C" ctc_status ctc_context_copy_bandwidth(ctc_context *c, uint64_t bytes,
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+  bytes &= ~(uint64_t)15;
+  (void)launch_copy(c->stream, a, b, bytes);
   (void)hipEventRecord(e0, c->stream);
   const int reps = 10;
-  for (int r = 0; r < reps; ++r) (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+  for (int r = 0; r < reps; ++r) (void)launch_copy(c->stream, a, b, bytes);
   (void)hipEventRecord(e1, c->stream);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -377,6 +378,10 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(d.grp_piece);
     dev_free(d.grp_n);
     dev_free(d.slot_gb);
+    dev_free(d.mix_items);
+    dev_free(d.mix_heavy_slot);
+    dev_free(d.mix_heavy_pb);
+    dev_free(m->Fbf[md]);
     dev_free(m->F[md]);
     dev_free(m->best[md]);
     dev_free(m->resid[md]);
@@ -502,9 +507,8 @@ extern "C" ctc_status ctc_model_prepare(ctc_model *m, const ctc_options *o) {
 // =========================================================================
 static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   const uint32_t R = m->rank;
-  if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "ALS at rank %u is not implemented", R);
-  if (o->precision != CTC_PREC_FP32)
-    return fail(CTC_ERR_UNSUPPORTED, "mixed-precision ALS is not implemented");
+  if (o->precision == CTC_PREC_MIXED) return als_mode_mixed(m, mode, o);
+  if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "FP32 ALS at rank %u is not implemented (use the mixed precision)", R);
   const uint32_t S = span_tiles_of(o);
   CTC_TRY(ensure_ciss(m, mode));
   CTC_TRY(ensure_pieces(m, mode, S));
@@ -530,8 +534,7 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   {
     Prof p(m->ctx, "als_solve");
     CTC_HIP(launch_als_solve(st, pieces, d.slot_pb, d.ngroups ? gsum : nullptr, d.slot_gb, d.directory,
-                             d.dlength, (int)R, o->lambda,
-                             o->refine != 0 || o->precision == CTC_PREC_MIXED, m->F[mode], m->dflag));
+                             d.dlength, (int)R, o->lambda, o->refine != 0, m->F[mode], m->dflag));
   }
   m->resid_valid = false;
   return CTC_OK;
@@ -548,6 +551,11 @@ static ctc_status check_flag(ctc_model *m) {
   }
   return CTC_OK;
 }
+
+namespace ctc {
+ctc_status als_mode_impl(ctc_model *m, uint32_t mode, const ctc_options *o) { return als_mode(m, mode, o); }
+ctc_status check_not_spd(ctc_model *m) { return check_flag(m); }
+}  // namespace ctc
 
 // stride coprime with n near n / golden ratio (1 when spreading is off)
 static uint64_t spread_stride(uint64_t n, bool on) {

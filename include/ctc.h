@@ -153,7 +153,8 @@ ctc_status ctc_context_set_profiling(ctc_context *ctx, int enabled);
 ctc_status ctc_context_kernel_stats(ctc_context *ctx, const char *kernel, uint64_t *launches,
                                     double *total_ms);
 ctc_status ctc_context_reset_stats(ctc_context *ctx);
-/* Measured device-to-device copy bandwidth (GB/s) over `bytes` bytes. */
+/* Measured device-to-device copy bandwidth (GB/s, read + write) of the
+ * library's streaming copy kernel (k_copy) over `bytes` bytes. */
 ctc_status ctc_context_copy_bandwidth(ctc_context *ctx, uint64_t bytes, double *gbps);
 
 /* ------------------------------------------------------------------------ */
@@ -339,6 +340,43 @@ typedef struct ctc_tune_result {
 } ctc_tune_result;
 
 ctc_status ctc_autotune(ctc_model *m, const ctc_options *opt, ctc_tune_result *out);
+
+/* ------------------------------------------------------------------------ */
+/* Batched ALS: many independent completion problems solved together.        */
+/* No counterpart in the reference, which solves one tensor per process      */
+/* (SG/tc.c:40-50); its per-problem rules are those of tc_als.               */
+/* ------------------------------------------------------------------------ */
+typedef struct ctc_batch ctc_batch;
+
+/* Problems (all with the same mode count; rank 8, 16, 32 or 64 each) are
+ * grouped by rank.  The problems of a rank class form one block-diagonal
+ * tensor (problem p's rows follow those of the problems before it), so each
+ * ALS phase is one launch over every problem of the class.  test may be NULL
+ * or hold NULL entries.  The tensors are copied. */
+ctc_status ctc_batch_create(ctc_context *ctx, uint32_t nproblems, const ctc_coo *const *train,
+                            const ctc_coo *const *test, const uint32_t *ranks, ctc_batch **out);
+void ctc_batch_destroy(ctc_batch *b);
+ctc_status ctc_batch_info(const ctc_batch *b, uint32_t *nproblems, uint32_t *nclasses, uint64_t *train_nnz);
+/* rank and problem count of rank class `cls` (classes in increasing rank) */
+ctc_status ctc_batch_class_rank(const ctc_batch *b, uint32_t cls, uint32_t *rank, uint32_t *nproblems);
+/* factor `mode` of problem `problem`, row-major FP32, rank columns and
+ * max(train dims[mode], test dims[mode]) rows */
+ctc_status ctc_batch_set_factor(ctc_batch *b, uint32_t problem, uint32_t mode, const float *host);
+ctc_status ctc_batch_get_factor(ctc_batch *b, uint32_t problem, uint32_t mode, float *host);
+/* Measured (tile_len, block) per rank class for an ALS sweep (ctc_autotune on
+ * each class); results has room for 4 entries.  Later ctc_batch_als calls use
+ * each class's tuned pair in place of opt's.  Factors are left unchanged. */
+ctc_status ctc_batch_autotune(ctc_batch *b, const ctc_options *opt, ctc_tune_result *results,
+                              uint32_t *nclasses);
+/* `iters` ALS iterations (modes 0..N-1 in turn) on every problem, FP32.
+ * use_graph != 0: the run is captured once into a hipGraph (one branch per
+ * rank class, forked from and joined to the context's stream) and replayed
+ * by later calls with the same iters and options; 0: launched directly.
+ * ms receives the device time of the run.  Returns CTC_ERR_NOT_SPD if some
+ * row's normal system was not positive definite (that row becomes 0). */
+ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *opt, uint32_t iters, int32_t use_graph, double *ms);
+/* per-problem train and test RMSE (arrays of nproblems; either may be NULL) */
+ctc_status ctc_batch_rmse(ctc_batch *b, double *rmse_tr, double *rmse_te);
 
 #ifdef __cplusplus
 }

@@ -239,6 +239,27 @@ def als_mode(t: Coo, mode, F, lam, order=None, nthreads=1):
                                   C.c_double(lam), _u32p(perm), _u64p(rowptr), C.c_int(nthreads))
 
 
+def bf16_round(x):
+    """float32 -> bfloat16 -> float32, round to nearest even (scalar)."""
+    f = lib().orc_bf16_round
+    f.restype = C.c_float
+    f.argtypes = [C.c_float]
+    return float(f(float(x)))
+
+
+def als_mode_bf16(t: Coo, mode, F, lam, order=None, nthreads=1):
+    """In-place ALS update of F[mode] (float32 factors) in the arithmetic of the
+    BF16-operand path: BF16-rounded factor rows, BF16-rounded h in the Gram,
+    FP64 accumulation and solve.  Returns the status (0 ok, 7 not SPD)."""
+    perm, rowptr = order if order is not None else mode_order(t, mode)
+    R = F[0].shape[1]
+    assert all(f.dtype == np.float32 for f in F)
+    fn = lib().orc_als_mode_bf16
+    fn.restype = C.c_int
+    return fn(t.c(), C.c_uint32(mode), C.c_uint32(R), _ptrs(F, "f32"), C.c_double(lam), _u32p(perm),
+              _u64p(rowptr), C.c_int(nthreads))
+
+
 def sgd_epoch(t: Coo, F, lr, lam, scale=1.0, order=None):
     R = F[0].shape[1]
     op = _u32p(order) if order is not None else None

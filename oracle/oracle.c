@@ -280,6 +280,76 @@ void orc_cholesky_solve(const double *L, uint32_t n, double *b) {
   }
 }
 
+/* float -> bfloat16 -> float, round to nearest even (finite inputs). */
+float orc_bf16_round(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  float y;
+  memcpy(&y, &u, 4);
+  return y;
+}
+
+/* The BF16-operand ALS update (the mixed-precision path's arithmetic, no
+ * counterpart in the reference): factor rows of the other modes are rounded
+ * to BF16, h is their product (exact in FP32 for up to three BF16 factors),
+ * G = sum bf16(h) bf16(h)^T and m = sum x h are accumulated in FP64 here,
+ * and (G + lambda I) a = m is solved in FP64.  Rows with no nonzero become 0.
+ * Returns 0, or 7 if a system is not positive definite. */
+int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, double lambda,
+                      const uint32_t *perm, const uint64_t *rowptr, int nthreads) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  const int64_t dim = (int64_t)t->dims[mode];
+  int status = 0;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t i = 0; i < dim; ++i) {
+    float *dst = F[mode] + (size_t)i * R;
+    if (rowptr[i] == rowptr[i + 1]) {
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0.f;
+      continue;
+    }
+    double *G = (double *)calloc((size_t)R * R, sizeof(double));
+    double *rhs = (double *)calloc(R, sizeof(double));
+    float h[512], hb[512];
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const uint64_t p = perm[q];
+      for (uint32_t f = 0; f < R; ++f) h[f] = 1.f;
+      for (uint32_t m = 0; m < t->nmodes; ++m) {
+        if (m == mode) continue;
+        const float *row = F[m] + (size_t)t->ind[m][p] * R;
+        for (uint32_t f = 0; f < R; ++f) h[f] *= orc_bf16_round(row[f]);
+      }
+      for (uint32_t f = 0; f < R; ++f) hb[f] = orc_bf16_round(h[f]);
+      const double x = t->vals[p];
+      for (uint32_t a = 0; a < R; ++a) {
+        for (uint32_t b = 0; b <= a; ++b) G[(size_t)a * R + b] += (double)hb[a] * (double)hb[b];
+        rhs[a] += x * (double)h[a];
+      }
+    }
+    for (uint32_t a = 0; a < R; ++a) {
+      for (uint32_t b = 0; b < a; ++b) G[(size_t)b * R + a] = G[(size_t)a * R + b];
+      G[(size_t)a * R + a] += lambda;
+    }
+    if (orc_cholesky(G, R) != 0) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+      status = 7;
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0.f;
+    } else {
+      orc_cholesky_solve(G, R, rhs);
+      for (uint32_t f = 0; f < R; ++f) dst[f] = (float)rhs[f];
+    }
+    free(G);
+    free(rhs);
+  }
+  return status;
+}
+
 int orc_converge(orc_conv *c, double converge_rmse, uint64_t epoch, int *improved) {
   *improved = 0;
   if (converge_rmse - c->best_rmse < -c->tolerance) { /* SG/completion.c:332-341 */

@@ -112,6 +112,14 @@ int orc_cholesky(double *A, uint32_t n);
 /* Solve L L^T x = b in place (b -> x) given the lower factor from orc_cholesky. */
 void orc_cholesky_solve(const double *L, uint32_t n, double *b);
 
+/* ---- BF16-operand ALS (mixed-precision path) ------------------------------ */
+/* float -> bfloat16 -> float, round to nearest even */
+float orc_bf16_round(float x);
+/* ALS mode update with BF16-rounded factor rows and BF16-rounded h in the Gram,
+ * FP64 accumulation and solve; float factors.  Returns 0 or 7 (not SPD). */
+int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, double lambda,
+                      const uint32_t *perm, const uint64_t *rowptr, int nthreads);
+
 /* ---- convergence (SG/completion.c:296-362) -------------------------------- */
 typedef struct orc_conv {
   double best_rmse;

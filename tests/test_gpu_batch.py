@@ -1,0 +1,134 @@
+"""Batched ALS (many independent problems in one block-diagonal launch per
+rank class, optionally replayed from a hipGraph) against the FP64 oracle run
+on each problem alone.
+
+Tolerances: every factor of every problem within 1e-5 relative (Frobenius)
+of the oracle after each run; per-problem RMSE within 1e-6 relative of the
+oracle's; graph replays bit-identical to the direct launches.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cutc_amd as ct
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LAM = 0.05
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ct.Context(0)
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+
+
+def make_problems(n, seed, max_nnz=20000):
+    rng = np.random.default_rng(seed)
+    probs = []
+    for p in range(n):
+        R = (8, 16, 32, 64)[p % 4]
+        dims = tuple(int(x) for x in np.exp(rng.uniform(np.log(40), np.log(400), 3)))
+        nnz = int(min(rng.integers(2000, max_nnz), 0.3 * np.prod(dims)))
+        tr, te = ct.synth(dims, nnz, true_rank=max(1, R // 2), value_map=ct.VALUES_RAW, noise_std=0.1,
+                          test_fraction=0.1, seed=1000 + 17 * seed + p)
+        F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
+        probs.append((tr, te, R, F))
+    return probs
+
+
+def load(b, probs):
+    for p, (_, _, _, F) in enumerate(probs):
+        for m in range(3):
+            b.set_factor(p, m, F[m])
+
+
+def oracle_run(tr, F, iters):
+    ind, vals = tr.arrays()
+    oc = O.Coo(ind, vals, tr.dims)
+    F64 = [f.astype(np.float64) for f in F]
+    for _ in range(iters):
+        for m in range(3):
+            assert O.als_mode(oc, m, F64, LAM) == 0
+    return F64
+
+
+def test_batch_matches_oracle_per_problem(ctx):
+    probs = make_problems(12, seed=1)
+    say(f"[batch_oracle] {len(probs)} problems, ranks {[p[2] for p in probs]}")
+    b = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    assert b.classes() == [(8, 3), (16, 3), (32, 3), (64, 3)]
+    load(b, probs)
+    opt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM)
+    b.als(opt, iters=3, graph=False)
+    rtr, rte = b.rmse()
+    worst = 0.0
+    for p, (tr, te, R, F) in enumerate(probs):
+        F64 = oracle_run(tr, F, 3)
+        for m in range(3):
+            worst = max(worst, relerr(b.factor(p, m), F64[m]))
+        ind, vals = te.arrays()
+        ot = O.Coo(ind, vals, te.dims)
+        ind, vals = tr.arrays()
+        oc = O.Coo(ind, vals, tr.dims)
+        assert rte[p] == pytest.approx(O.rmse(ot, F64), rel=1e-5)
+        assert rtr[p] == pytest.approx(O.rmse(oc, F64), rel=1e-5)
+    say(f"  worst factor rel err {worst:.2e}")
+    assert worst < 1e-5
+
+
+def test_batch_graph_replays_match_direct(ctx):
+    probs = make_problems(9, seed=2)
+    say("[batch_graph]")
+    b = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    opt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM, tile_len=256)
+    load(b, probs)
+    b.als(opt, iters=2, graph=False)
+    ref = [[b.factor(p, m) for m in range(3)] for p in range(len(probs))]
+    for rep in range(3):
+        load(b, probs)
+        # eager work and synchronisation between replays
+        b.rmse()
+        ctx.synchronize()
+        b.als(opt, iters=2, graph=True)
+        for p in range(len(probs)):
+            for m in range(3):
+                assert np.array_equal(b.factor(p, m), ref[p][m]), (rep, p, m)
+    say("  3 replays identical to the direct run")
+
+
+def test_batch_autotune_keeps_results(ctx):
+    probs = make_problems(8, seed=3)
+    say("[batch_autotune]")
+    b = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    opt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM)
+    load(b, probs)
+    res = b.autotune(opt)
+    assert len(res) == 4 and all(r["trials"] > 0 for r in res)
+    # tuning leaves the factors as they were
+    for p, (_, _, _, F) in enumerate(probs):
+        assert np.array_equal(b.factor(p, 0), F[0])
+    b.als(opt, iters=2, graph=True)
+    worst = 0.0
+    for p, (tr, te, R, F) in enumerate(probs):
+        F64 = oracle_run(tr, F, 2)
+        for m in range(3):
+            worst = max(worst, relerr(b.factor(p, m), F64[m]))
+    say(f"  tuned {[(r['tile_len'], r['block']) for r in res]} worst {worst:.2e}")
+    assert worst < 1e-5
+
+
+def test_batch_rejects_bad_input(ctx):
+    tr, te = ct.synth((50, 40, 30), 2000, seed=5)
+    with pytest.raises(ct.CtcError):
+        ct.Batch(ctx, [tr], [te], [12])

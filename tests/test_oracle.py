@@ -311,6 +311,50 @@ def test_als_recovers_exact_rank_one():
 
 
 # --------------------------------------------------------------------------
+# BF16-operand ALS (mixed-precision path)
+# --------------------------------------------------------------------------
+
+def test_bf16_round_matches_torch_round_to_nearest_even():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(0)
+    xs = (rng.standard_normal(3000) * 10.0 ** rng.integers(-6, 6, 3000)).astype(np.float32)
+    xs = np.concatenate([xs, np.float32([1 + 2**-8, 1 + 3 * 2**-8, 1 + 2**-9, -1 - 2**-8, 0.0, 65504.0])])
+    ours = np.array([O.bf16_round(x) for x in xs], np.float32)
+    ref = torch.tensor(xs).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(ours, ref)
+    assert O.bf16_round(1 + 2**-8) == 1.0          # tie -> even
+    assert O.bf16_round(1 + 3 * 2**-8) == 1 + 2**-6  # tie -> even (upwards)
+
+
+def test_als_mode_bf16_equals_fp64_on_bf16_exact_inputs():
+    # factor entries k/8 (3 significant bits): every row and every product h is
+    # exact in BF16, so the BF16-operand update must equal the FP64 one
+    rng = np.random.default_rng(31)
+    dims = (9, 8, 7)
+    t = rand_coo(rng, dims, 200)
+    R, lam = 8, 0.25
+    for mode in range(3):
+        F32 = [(rng.integers(-8, 9, (d, R)) / 8.0).astype(np.float32) for d in dims]
+        F64 = [f.astype(np.float64) for f in F32]
+        assert O.als_mode_bf16(t, mode, F32, lam) == 0
+        assert O.als_mode(t, mode, F64, lam) == 0
+        np.testing.assert_allclose(F32[mode], F64[mode].astype(np.float32), rtol=2e-7, atol=1e-7)
+
+
+def test_als_mode_bf16_rounding_error_is_bf16_sized():
+    rng = np.random.default_rng(32)
+    dims = (10, 9, 8)
+    t = rand_coo(rng, dims, 400)
+    R, lam = 8, 0.1
+    F32 = rand_factors(rng, dims, R, np.float32)
+    F64 = [f.astype(np.float64) for f in F32]
+    assert O.als_mode_bf16(t, 0, F32, lam) == 0
+    assert O.als_mode(t, 0, F64, lam) == 0
+    rel = np.linalg.norm(F32[0] - F64[0]) / np.linalg.norm(F64[0])
+    assert 1e-5 < rel < 5e-2, rel  # rounding to 8 significant bits is visible but small
+
+
+# --------------------------------------------------------------------------
 # SGD
 # --------------------------------------------------------------------------
 
