@@ -40,6 +40,71 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def mixed_leg(ct, ctx, a, log):
+    """ALS at R=128 in the BF16-operand path on a 5M x 5M x 1000 tensor with
+    500M nonzeros (modes 0-1 Zipf(1.1)), values from a rank-32 CP model mapped
+    to [1,5] plus N(0,0.2), 90/10 split, lambda 0.05.  A sample of rows of the
+    first mode update is checked against the FP64 oracle from the same inputs."""
+    import numpy as np
+    R, lam = 128, 0.05
+    dims = (5_000_000, 5_000_000, 1000)
+    t0 = time.time()
+    tr, te = ct.synth(dims, a.mixed_nnz, zipf=(1.1, 1.1, 0.0), true_rank=32, value_map=ct.VALUES_AFFINE_1_5,
+                      noise_std=0.2, test_fraction=0.1, seed=0x5EED0005)
+    log(f"mixed: tensor train {tr.nnz} test {te.nnz} in {time.time() - t0:.1f}s")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    m.init_random(5)
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, precision=ct.PREC_MIXED, refine=1)
+    t0 = time.time()
+    m.prepare(opt)
+    log(f"mixed: CISS build+upload {time.time() - t0:.1f}s")
+    res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam}
+    # sampled FP64 check of the first mode-0 update
+    if not a.no_cpu:
+        import oracle as O
+        F = [m.factor(md) for md in range(3)]
+        ind, vals = tr.arrays()
+        oc = O.Coo(ind, vals, dims)
+        del ind, vals
+        rng = np.random.default_rng(0)
+        rows = np.sort(rng.choice(dims[0], a.mixed_sample, replace=False)).astype(np.uint32)
+        t0 = time.time()
+        ref = O.als_rows(oc, 0, F, lam, rows, nthreads=os.cpu_count() or 1)
+        log(f"mixed: FP64 oracle on {len(rows)} sampled rows in {time.time() - t0:.1f}s")
+        del oc, F
+    ctx.reset_stats()
+    m.als_mode_update(0, opt)
+    if not a.no_cpu:
+        got = m.factor(0)[rows].astype(np.float64)
+        nz = np.linalg.norm(ref, axis=1) > 0
+        rel = np.linalg.norm(got[nz] - ref[nz], axis=1) / np.linalg.norm(ref[nz], axis=1)
+        res["sampled_rows"] = int(nz.sum())
+        res["row_rel_err_vs_fp64_median"] = float(np.median(rel))
+        res["row_rel_err_vs_fp64_max"] = float(rel.max())
+        res["factor_rel_err_vs_fp64_sampled"] = float(np.linalg.norm(got[nz] - ref[nz]) / np.linalg.norm(ref[nz]))
+        log(f"mixed: sampled rows rel err vs FP64 median {np.median(rel):.2e} max {rel.max():.2e}")
+    for md in (1, 2):
+        m.als_mode_update(md, opt)
+    rm = [m.evaluate(lam)["rmse_te"]]
+    ctx.reset_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.mixed_iters):
+        for md in range(3):
+            m.als_mode_update(md, opt)
+        ctx.synchronize()
+        rm.append(m.evaluate(lam)["rmse_te"])  # inside the loop, outside the kernel stats
+    wall = (time.perf_counter() - t0) / a.mixed_iters
+    ni, msi = ctx.kernel_stats("als_mixed_items")
+    nh, msh = ctx.kernel_stats("als_mixed_heavy")
+    res.update({"s_per_iter": wall, "nnz_per_s": tr.nnz / wall,
+                "items_kernel_ms_per_iter": msi / a.mixed_iters, "heavy_kernel_ms_per_iter": msh / a.mixed_iters,
+                "gram_tflops_bf16": 3 * R * (R + 1) * tr.nnz / ((msi + msh) / a.mixed_iters * 1e-3) / 1e12,
+                "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
+    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), rmse_te {rm}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,11 +115,18 @@ def main():
     ap.add_argument("--ccd-epochs", type=int, default=3)
     ap.add_argument("--als-epochs", type=int, default=3)
     ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default")
+    ap.add_argument("--sgd-update", default="auto", choices=["auto", "atomic", "racy", "next_racy"],
+                    help="how concurrent walkers combine updates of shared rows (see ctc_sgd_update)")
+    ap.add_argument("--sgd-hot-rows", type=int, default=0,
+                    help="auto policy: rows with more nonzeros than this are updated racily (0 = default)")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--algs", default="sgd,ccd,als,batch")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--batch-iters", type=int, default=10)
+    ap.add_argument("--mixed-nnz", type=int, default=500_000_000)
+    ap.add_argument("--mixed-iters", type=int, default=3)
+    ap.add_argument("--mixed-sample", type=int, default=1000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -81,9 +153,11 @@ def main():
     rng = np.random.default_rng(1)  # the oracle trajectory fixture starts from this init
     F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
     model.set_factors(F0)
+    policy = {"auto": ct.SGD_AUTO, "atomic": ct.SGD_ATOMIC, "racy": ct.SGD_RACY,
+              "next_racy": ct.SGD_NEXT_RACY}[a.sgd_update]
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3,
                      lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0,
-                     tile_len=a.tile_len or None)
+                     tile_len=a.tile_len or None, sgd_update=policy, sgd_hot_rows=a.sgd_hot_rows)
     t0 = time.time()
     model.prepare(opt)
     log(f"SGD CISS build+upload {time.time() - t0:.1f}s")
@@ -116,9 +190,29 @@ def main():
                        "rmse_te_initial": rm0, "rmse_te_per_epoch": [round(x, 5) for x in traj],
                        "oracle_fp64_rmse_te_per_epoch": [round(x, 5) for x in ref] if ref else None,
                        "max_abs_gap_vs_oracle": gap, "epochs": a.warmup + a.steps,
-                       "tile_len": opt.to_c().tile_len, "policy": "auto", "alg_bytes_per_nnz": alg_bytes_per_nnz}
+                       "tile_len": opt.to_c().tile_len, "policy": a.sgd_update,
+                       "hot_row_threshold": opt.to_c().sgd_hot_rows or 262144, "alg_bytes_per_nnz": alg_bytes_per_nnz}
     log(f"SGD {wall / a.steps * 1e3:.2f} ms/epoch, kernel {kern_ms:.2f} ms, {value / 1e9:.2f} Gnnz/s, "
         f"rmse_te {rm0:.4f} -> {traj[-1]:.4f}; max gap vs FP64 sequential oracle {gap}")
+
+    # -------------------------------------------------------------- MTTKRP
+    if "mttkrp" in algs or "als" in algs:
+        ctx.reset_stats()
+        for md in range(3):
+            model.mttkrp(md)  # warm-up (builds the CISS of each mode)
+        ctx.reset_stats()
+        per_mode = []
+        for md in range(3):
+            for _ in range(3):
+                model.mttkrp(md)
+            n_m, ms_m = ctx.kernel_stats("mttkrp")
+            per_mode.append(ms_m / max(n_m, 1))
+            ctx.reset_stats()
+        mb = 12 + 8 * R  # stream + two gathered rows per nonzero
+        out_algs["mttkrp"] = {"kernel_ms_per_mode": per_mode, "alg_bytes_per_nnz": mb,
+                              "alg_gbs_per_mode": [mb * train.nnz / (t * 1e-3) / 1e9 for t in per_mode]}
+        log(f"MTTKRP kernel ms per mode {[round(t, 3) for t in per_mode]}, algorithmic GB/s "
+            f"{[round(mb * train.nnz / (t * 1e-3) / 1e9) for t in per_mode]}")
 
     # --------------------------------------------------------------- CCD++
     if "ccd" in algs and a.ccd_epochs > 0:
@@ -217,6 +311,10 @@ def main():
             f"{ms_eager:.1f} ms (direct); {bn * a.batch_iters / (ms_graph * 1e-3) / 1e9:.3f} Gnnz/s; "
             f"median rmse_te {np.median(rte):.4f}")
         del batch, probs, Fb
+
+    # ------------------------------------------ mixed-precision ALS, R = 128
+    if "mixed" in algs:
+        out_algs["als_mixed"] = mixed_leg(ct, ctx, a, log)
 
     copy_gbs = ctx.copy_bandwidth(1 << 30)
 

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field, fields
 import numpy as np
 
 from . import _lib
-from ._lib import (ALS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
+from ._lib import (ALS, ALS_AUTO, ALS_ROWS, ALS_SPANS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
                    ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_NEXT_RACY, SGD_RACY,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
@@ -239,6 +239,7 @@ class Options:
     sgd_update: int = SGD_AUTO
     spread: int = 0
     sgd_hot_rows: int = 0
+    als_kernel: int = 0
 
     def to_c(self):
         o = _lib.Options()
@@ -431,11 +432,16 @@ class Batch:
         np_, nc, nnz = C.c_uint32(), C.c_uint32(), C.c_uint64()
         check(lib().ctc_batch_info(self._h, C.byref(np_), C.byref(nc), C.byref(nnz)))
         self.nclasses, self.train_nnz = nc.value, nnz.value
+        _lib.release_at_exit(self)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib.alive():
+    def _release(self):
+        if getattr(self, "_h", None):
             lib().ctc_batch_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        if _lib.alive():
+            self._release()
 
     def set_factor(self, p, mode, F):
         F = np.ascontiguousarray(F, dtype=np.float32)

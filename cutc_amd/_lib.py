@@ -3,6 +3,7 @@ library has not been built: there is no fallback path."""
 from __future__ import annotations
 
 import atexit
+import weakref
 import ctypes as C
 import os
 
@@ -16,6 +17,7 @@ LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
 ORDER_FIXED, ORDER_ROTATE = 0, 1
 SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY = 0, 1, 2, 3
+ALS_AUTO, ALS_SPANS, ALS_ROWS = 0, 1, 2
 VALUES_RAW, VALUES_AFFINE_1_5, VALUES_INT_1_5 = 0, 1, 2
 
 STATUS = {
@@ -98,6 +100,7 @@ class Options(C.Structure):
         ("sgd_update", C.c_int32),
         ("spread", C.c_int32),
         ("sgd_hot_rows", C.c_uint32),
+        ("als_kernel", C.c_int32),
     ]
 
 
@@ -214,10 +217,25 @@ _lib = None
 _shutting_down = False
 
 
+_release_at_exit = weakref.WeakSet()
+
+
+def release_at_exit(obj):
+    """Objects holding HIP graphs are released before the interpreter exits:
+    the HIP runtime aborts if it tears down with an instantiated graph alive."""
+    _release_at_exit.add(obj)
+
+
 def _on_exit():
-    # Objects that outlive interpreter shutdown must not call into the HIP
+    # Graph holders are released first, while the runtime is intact.  Other
+    # objects that outlive interpreter shutdown must not call into the HIP
     # runtime while it is being torn down; the OS reclaims their memory.
     global _shutting_down
+    for obj in list(_release_at_exit):
+        try:
+            obj._release()
+        except Exception:  # noqa: BLE001
+            pass
     _shutting_down = True
 
 

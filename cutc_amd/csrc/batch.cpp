@@ -7,9 +7,10 @@
 // MTTKRP and solve of a row only involve the row's own nonzeros), and each
 // phase of a sweep is one kernel launch over all problems of the class, with
 // the row-segmented reductions keeping light and heavy rows of different
-// problems in the same launch.  Each class runs on its own stream; a run of
-// `iters` iterations is captured once into a hipGraph whose branches (one per
-// class) fork from and join the caller's stream, and replayed.
+// problems in the same launch.  Launched directly, each class runs on its own
+// stream (forked from and joined to the caller's stream); a run of `iters`
+// iterations can instead be captured once into a hipGraph on the caller's
+// stream and replayed.
 //
 // Because mode-0 global ids preserve problem order, each problem's training
 // nonzeros form one contiguous range of the class's mode-0 CISS and its test
@@ -250,6 +251,7 @@ static ctc_options class_options(const RankClass &c, const ctc_options *o) {
 // that the sweep itself only launches kernels
 static ctc_status prepare_class(RankClass &c, const ctc_options *t) {
   ctc_model *m = c.model;
+  if (als_uses_rows(t, c.rank)) return als_rows_prepare(m, t, false);
   const uint32_t S = span_tiles_of(t);
   size_t need = 0;
   for (uint32_t md = 0; md < m->nmodes; ++md) {
@@ -288,19 +290,29 @@ extern "C" ctc_status ctc_batch_class_rank(const ctc_batch *b, uint32_t cls, uin
   return CTC_OK;
 }
 
-// enqueue `iters` sweeps of every class: fork from the caller's stream, one
-// branch per class on the class stream, join back
-static ctc_status enqueue_run(ctc_batch *b, const ctc_options *o, uint32_t iters) {
+// enqueue `iters` sweeps of every class.  Direct launches fork from the
+// caller's stream into one branch per class stream and join back; a graph
+// capture records every class on the caller's stream, one class after the
+// other (each phase is already one launch over all problems of a class).
+static ctc_status enqueue_run(ctc_batch *b, const ctc_options *o, uint32_t iters, bool capture) {
   hipStream_t main = b->ctx->stream;
-  CTC_HIP(hipEventRecord(b->fork, main));
+  if (!capture) CTC_HIP(hipEventRecord(b->fork, main));
   for (auto &c : b->cls) {
     const ctc_options t = class_options(c, o);
-    hipStream_t st = c.ctx->stream;
-    CTC_HIP(hipStreamWaitEvent(st, b->fork, 0));
-    for (uint32_t it = 0; it < iters; ++it)
-      for (uint32_t md = 0; md < b->nmodes; ++md) CTC_TRY(als_mode_impl(c.model, md, &t));
-    CTC_HIP(hipEventRecord(c.join, st));
-    CTC_HIP(hipStreamWaitEvent(main, c.join, 0));
+    hipStream_t own = c.ctx->stream;
+    if (capture)
+      c.ctx->stream = main;  // the class's launches go to the capturing stream
+    else
+      CTC_HIP(hipStreamWaitEvent(own, b->fork, 0));
+    ctc_status s = CTC_OK;
+    for (uint32_t it = 0; it < iters && s == CTC_OK; ++it)
+      for (uint32_t md = 0; md < b->nmodes && s == CTC_OK; ++md) s = als_mode_impl(c.model, md, &t);
+    c.ctx->stream = own;
+    if (s != CTC_OK) return s;
+    if (!capture) {
+      CTC_HIP(hipEventRecord(c.join, own));
+      CTC_HIP(hipStreamWaitEvent(main, c.join, 0));
+    }
   }
   return CTC_OK;
 }
@@ -312,12 +324,15 @@ static std::vector<uintptr_t> graph_key_of(ctc_batch *b, const ctc_options *o, u
     const ctc_options t = class_options(c, o);
     k.push_back(t.tile_len);
     k.push_back(t.block);
+    k.push_back((uintptr_t)t.als_kernel);
     k.push_back((uintptr_t)c.model->scratch);
     for (uint32_t md = 0; md < b->nmodes; ++md) {
       const DevCiss &d = c.model->ciss[md];
       k.push_back((uintptr_t)d.slot_pb);
       k.push_back((uintptr_t)d.span_off);
       k.push_back((uintptr_t)d.slot_gb);
+      k.push_back((uintptr_t)d.mix_items);
+      k.push_back((uintptr_t)d.mix_chunk);
       k.push_back((uintptr_t)c.model->F[md]);
     }
   }
@@ -332,6 +347,8 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
   for (auto &c : b->cls) {
     const ctc_options t = class_options(c, o);
     CTC_TRY(prepare_class(c, &t));
+    // the not-positive-definite flag starts clear (outside any capture)
+    CTC_HIP(hipMemsetAsync(c.model->dflag, 0, sizeof(uint32_t), c.ctx->stream));
     CTC_HIP(hipStreamSynchronize(c.ctx->stream));
   }
   hipStream_t main = b->ctx->stream;
@@ -340,7 +357,7 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
     if (!b->exec || key != b->graph_key) {
       batch_release_graph(b);
       CTC_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
-      ctc_status s = enqueue_run(b, o, iters);
+      ctc_status s = enqueue_run(b, o, iters, true);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(main, &g);
       if (s != CTC_OK) {
@@ -358,7 +375,7 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
     CTC_HIP(hipEventRecord(b->t1, main));
   } else {
     CTC_HIP(hipEventRecord(b->t0, main));
-    CTC_TRY(enqueue_run(b, o, iters));
+    CTC_TRY(enqueue_run(b, o, iters, false));
     CTC_HIP(hipEventRecord(b->t1, main));
   }
   CTC_HIP(hipEventSynchronize(b->t1));
@@ -367,8 +384,8 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
   if (ms) *ms = t;
   ctc_status s = CTC_OK;
   for (auto &c : b->cls) {
-    const ctc_status f = check_not_spd(c.model);
-    if (s == CTC_OK) s = f;
+    if (check_not_spd(c.model) != CTC_OK && s == CTC_OK)
+      s = fail(CTC_ERR_NOT_SPD, "rank-%u class: a normal system (G + lambda I) was not positive definite", c.rank);
   }
   return s;
 }

@@ -14,6 +14,7 @@
 //   --lr-schedule bold|exp|const  --lr-decay G   --inner I (NUM_INNER, :30)
 //   --autotune (replaces the GA of software/autotuning)      --json (summary line)
 //   --dump-factors PREFIX (writes PREFIX<mode>.txt)
+//   --precision fp32|mixed (BF16-operand ALS)  --refine K   --sgd-update P  --sgd-hot-rows N
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,7 +29,9 @@ static void usage() {
           "           [--alg als|sgd|ccd] [--rank R] [--lambda L] [--lr LR] [--sgd-scale S]\n"
           "           [--max-epochs E] [--tol T] [--max-bad-epochs B] [--tile-len N] [--block B]\n"
           "           [--seed S] [--mode-order fixed|rotate] [--lr-schedule bold|exp|const]\n"
-          "           [--lr-decay G] [--inner I] [--autotune] [--json] [--dump-factors PREFIX]\n");
+          "           [--lr-decay G] [--inner I] [--precision fp32|mixed] [--refine K]\n"
+          "           [--sgd-update auto|atomic|racy|next-racy] [--sgd-hot-rows N]\n"
+          "           [--autotune] [--json] [--dump-factors PREFIX]\n");
 }
 
 static int die(ctc_status s, const char *what) {
@@ -91,6 +94,13 @@ int main(int argc, char **argv) {
                                                                                           : CTC_LR_BOLD_DRIVER;
     else if (k == "lr-decay") o.lr_decay = atof(v);
     else if (k == "inner") o.inner_iters = (uint32_t)atoi(v);
+    else if (k == "precision") o.precision = strcmp(v, "mixed") == 0 ? CTC_PREC_MIXED : CTC_PREC_FP32;
+    else if (k == "refine") o.refine = atoi(v);
+    else if (k == "sgd-update")
+      o.sgd_update = strcmp(v, "atomic") == 0 ? CTC_SGD_ATOMIC : strcmp(v, "racy") == 0 ? CTC_SGD_RACY
+                     : strcmp(v, "next-racy") == 0                                       ? CTC_SGD_NEXT_RACY
+                                                                                         : CTC_SGD_AUTO;
+    else if (k == "sgd-hot-rows") o.sgd_hot_rows = (uint32_t)strtoul(v, nullptr, 0);
     else if (k == "dump-factors") dump = v;
     else {
       fprintf(stderr, "ctc: unknown flag --%s\n", k.c_str());

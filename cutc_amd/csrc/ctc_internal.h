@@ -159,8 +159,15 @@ ctc_status ensure_ciss(ctc_model *m, uint32_t mode);
 ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles);
 // one CCD++ epoch on the device (ccd.cpp)
 ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o);
-// one BF16-operand ALS mode update (ranks 32, 64, 128), enqueued on the model's stream
-ctc_status als_mode_mixed(ctc_model *m, uint32_t mode, const ctc_options *o);
+// one row-fused ALS mode update (ranks 32, 64, 128), enqueued on the model's
+// stream: BF16-operand Gram (bf16) or FP64-accumulated Gram from FP32 factors
+ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool bf16);
+// everything als_mode_rows allocates or uploads, for all modes
+ctc_status als_rows_prepare(ctc_model *m, const ctc_options *o, bool bf16);
+// which FP32 ALS kernels a model of rank R uses under these options
+inline bool als_uses_rows(const ctc_options *o, uint32_t R) {
+  return o->precision == CTC_PREC_MIXED || o->als_kernel == CTC_ALS_ROWS || (o->als_kernel == CTC_ALS_AUTO && R >= 32);
+}
 // one ALS mode update, enqueued on the model's stream (no synchronisation, so
 // it can be captured into a graph once the model is prepared)
 ctc_status als_mode_impl(ctc_model *m, uint32_t mode, const ctc_options *o);

@@ -1,4 +1,5 @@
-// kernels_mixed.h — BF16-operand ALS mode update (ranks 32, 64, 128).
+// kernels_mixed.h — row-fused ALS mode update (ranks 32, 64, 128): BF16-operand
+// (mixed precision) or FP64-accumulated (FP32 factors) Gram, FP64 solve.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,15 +13,21 @@ namespace ctc {
 // chunk writes its partial Gram and right-hand side to partial `part`.
 constexpr uint32_t kLightRow = 0xFFFFFFFFu;
 
+// Gram precision: BF16 operands with FP32 accumulation, or FP32 factors with
+// h formed and accumulated in FP64
+constexpr int kPrecBF16 = 0, kPrecF64 = 1;
+
 struct MixedArgs {
+  int prec;                 // kPrecBF16 / kPrecF64
   const uint32_t *oidx[3];  // other-mode ids in CISS order (CISS-relative modes 1..N-1)
   const float *vals;
   const uint32_t *directory;
   const uint16_t *fb[3];  // BF16 mirrors of the other modes' factors (CISS-relative 1..N-1)
+  const float *fr[3];     // the other modes' FP32 factors (kPrecF64)
   float *out;             // factor of the mode being updated, row-major dims x R
   const uint4 *items;
   uint64_t nitems;
-  float *part;                 // partials: (R * R + R) floats each
+  void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])
   uint64_t nheavy;
@@ -32,6 +39,7 @@ struct MixedArgs {
 // items pass (light rows solved, heavy chunks to partials), then heavy rows
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+size_t mixed_partial_bytes(int R, int prec);
 // FP32 -> BF16 (round to nearest even), n elements
 hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n);
 

@@ -60,11 +60,21 @@ constexpr int kHTS = 20;  // dwords per staged feature row: 32 BF16 nonzeros + p
 template <int R>
 struct MixedSmem {
   static constexpr int NW = R / 32, RT = R / 16, NTILE = RT * (RT + 1) / 2;
-  double tiles[NTILE * 256];  // FP64 lower tiles; the Gram phase stages h here
+  static constexpr int TPW = (NTILE + NW - 1) / NW;  // FP64 path: lower tiles per wave
+  static constexpr int HS = R + 2;                   // FP64 path: staged row stride (doubles)
+  static constexpr int NT64 = NTILE * 256 > 32 * HS ? NTILE * 256 : 32 * HS;
+  double tiles[NT64];  // FP64 lower tiles; the Gram phase stages h here
   double vm[R], vx[R], vy[R];
   double contrib[NW][R];
   int bad;
 };
+
+// lower tile u (row-major over I >= J) -> (I, J)
+__device__ inline void tile_ij(int u, int &I, int &J) {
+  I = 0;
+  while ((I + 1) * (I + 2) / 2 <= u) ++I;
+  J = u - I * (I + 1) / 2;
+}
 
 // ------------------------------------------------------------ Gram phase --
 // acc[tt] (tt <= w) += stripe w x stripe tt of sum bf16(h) bf16(h)^T over
@@ -131,10 +141,70 @@ __device__ inline void gram_chunk(const MixedArgs &a, uint32_t p0, uint32_t p1, 
   }
 }
 
-// sums the per-thread m partials of the workgroup into sm.contrib[0][*] (as
-// double); ends with a barrier
-template <int R>
-__device__ inline void reduce_m(float *mp, MixedSmem<R> &sm) {
+// FP64 variant: h = product of the FP32 rows formed in FP64, staged in LDS
+// as FP64 (32 nonzeros x R), and every lower 16x16 tile of G accumulated on
+// v_mfma_f64_16x16x4f64, tiles dealt round-robin to the waves (wave w owns
+// tiles w, w + NW, ...); m accumulated in FP64.
+template <int R, int N>
+__device__ inline void gram_f64(const MixedArgs &a, uint32_t p0, uint32_t p1, f64x4 *acc, double *mp,
+                                double *hs) {
+  constexpr int NW = R / 32, FG8 = R / 8, HS = MixedSmem<R>::HS;
+  constexpr int NTILE = MixedSmem<R>::NTILE, TPW = MixedSmem<R>::TPW;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int fg = tid % FG8, qp = tid / FG8;
+  for (uint32_t base = p0; base < p1; base += 32) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t p = base + 2 * qp + h;
+      double hd[8];
+      if (p < p1) {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) hd[f] = 1.0;
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) {
+          const float *row = a.fr[s] + (size_t)a.oidx[s][p] * R + 8 * fg;
+          const float4 u0 = *(const float4 *)row, u1 = *(const float4 *)(row + 4);
+          hd[0] *= u0.x;
+          hd[1] *= u0.y;
+          hd[2] *= u0.z;
+          hd[3] *= u0.w;
+          hd[4] *= u1.x;
+          hd[5] *= u1.y;
+          hd[6] *= u1.z;
+          hd[7] *= u1.w;
+        }
+        const double x = a.vals[p];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) mp[f] += x * hd[f];
+      } else {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) hd[f] = 0.0;
+      }
+      double *dst = hs + (2 * qp + h) * HS + 8 * fg;
+#pragma unroll
+      for (int f = 0; f < 8; f += 2) *(double2 *)(dst + f) = make_double2(hd[f], hd[f + 1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const double *hq = hs + (4 * s + (l >> 4)) * HS + (l & 15);
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) {
+        const int u = w + NW * j;
+        if (u >= NTILE) continue;  // wave-uniform
+        int I, J;
+        tile_ij(u, I, J);
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(hq[16 * I], hq[16 * J], acc[j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// sums the per-thread m partials of the workgroup into sm.vm (FP64); ends
+// with a barrier
+template <int R, class T>
+__device__ inline void reduce_m(T *mp, MixedSmem<R> &sm) {
   constexpr int NW = R / 32, FG8 = R / 8;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, fg = tid % FG8;
 #pragma unroll
@@ -375,13 +445,12 @@ __device__ inline void fwd_solve(MixedSmem<R> &sm) {
   }
 }
 
-// (G + lambda I) x = m with G in the MFMA accumulators (lower stripes) and m
-// in sm.vm; returns false if the system is not positive definite
+// A = G + lambda I into the lower FP64 tiles, from the BF16 path's stripe
+// accumulators (wave w: 32x32 blocks (w, tt), tt <= w)
 template <int R>
-__device__ inline bool solve_row(MixedSmem<R> &sm, const f32x16 *acc, double lam, int refine) {
-  constexpr int NW = R / 32, RT = R / 16;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  // A = G + lambda I into the lower FP64 tiles
+__device__ inline void tiles_from(MixedSmem<R> &sm, const f32x16 *acc, double lam) {
+  constexpr int NW = R / 32;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int tt = 0; tt < NW; ++tt) {
     if (tt > w) continue;
@@ -392,6 +461,104 @@ __device__ inline bool solve_row(MixedSmem<R> &sm, const f32x16 *acc, double lam
         sm.tiles[tileno(i >> 4, j >> 4) * 256 + tix(i & 15, j & 15)] = (double)acc[tt][v] + (i == j ? lam : 0.0);
     }
   }
+}
+
+// ... from the FP64 path's tile accumulators
+template <int R>
+__device__ inline void tiles_from(MixedSmem<R> &sm, const f64x4 *acc, double lam) {
+  constexpr int NW = R / 32, NTILE = MixedSmem<R>::NTILE, TPW = MixedSmem<R>::TPW;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int u = w + NW * j;
+    if (u >= NTILE) continue;
+    int I, J;
+    tile_ij(u, I, J);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = row16(v, l), c = l & 15;
+      sm.tiles[u * 256 + tix(r, c)] = acc[j][v] + (I == J && r == c ? lam : 0.0);
+    }
+  }
+}
+
+// contrib[w][*] = this wave's share of G x (BF16 path: stripe rows of every
+// lower block plus, for off-diagonal blocks, the transposed block)
+template <int R>
+__device__ inline void gx_share(MixedSmem<R> &sm, const f32x16 *acc) {
+  constexpr int NW = R / 32;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double rs[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) rs[v] = 0.0;
+  for (int k = l; k < R; k += 64) sm.contrib[w][k] = 0.0;
+  wave_sync();
+#pragma unroll
+  for (int tt = 0; tt < NW; ++tt) {
+    if (tt > w) continue;
+    const double xj = sm.vx[32 * tt + (l & 31)];
+    double cs = 0.0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      rs[v] += (double)acc[tt][v] * xj;
+      cs += (double)acc[tt][v] * sm.vx[32 * w + row32(v, l)];
+    }
+    if (tt < w) {
+      cs += __shfl_xor(cs, 32, 64);
+      if (l < 32) sm.contrib[w][32 * tt + l] = cs;
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) rs[v] += __shfl_xor(rs[v], o, 64);
+  if ((l & 31) == 0)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) sm.contrib[w][32 * w + row32(v, l)] = rs[v];
+}
+
+// ... FP64 path: per owned tile (I, J), rows of I and (I != J) rows of J
+template <int R>
+__device__ inline void gx_share(MixedSmem<R> &sm, const f64x4 *acc) {
+  constexpr int NW = R / 32, NTILE = MixedSmem<R>::NTILE, TPW = MixedSmem<R>::TPW;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int k = l; k < R; k += 64) sm.contrib[w][k] = 0.0;
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) {
+    const int u = w + NW * j;
+    if (u >= NTILE) continue;
+    int I, J;
+    tile_ij(u, I, J);
+    double rv[4], cs = 0.0;
+    const double xc = sm.vx[16 * J + (l & 15)];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      rv[v] = acc[j][v] * xc;
+      cs += acc[j][v] * sm.vx[16 * I + row16(v, l)];
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rv[v] += __shfl_xor(rv[v], o, 64);
+    cs += __shfl_xor(cs, 16, 64);
+    cs += __shfl_xor(cs, 32, 64);
+    if ((l & 15) == 0)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sm.contrib[w][16 * I + row16(v, l)] += rv[v];
+    wave_sync();
+    if (I != J && l < 16) sm.contrib[w][16 * J + l] += cs;
+    wave_sync();
+  }
+}
+
+// (G + lambda I) x = m with G in the accumulators and m in sm.vm; returns
+// false if the system is not positive definite
+template <int R, class Acc>
+__device__ inline bool solve_row(MixedSmem<R> &sm, const Acc *acc, double lam, int refine) {
+  constexpr int NW = R / 32, RT = R / 16;
+  const int tid = threadIdx.x, w = tid >> 6;
+  tiles_from<R>(sm, acc, lam);
   if (tid < R) sm.vy[tid] = sm.vm[tid];
   if (tid == 0) sm.bad = 0;
   __syncthreads();
@@ -413,35 +580,8 @@ __device__ inline bool solve_row(MixedSmem<R> &sm, const f32x16 *acc, double lam
   if (sm.bad) return false;
   back_solve(sm, sm.vx);
   for (int it = 0; it < refine; ++it) {
-    // r = m - lambda x - G x; G x from the FP32 accumulators: stripe rows of
-    // every lower block plus, for off-diagonal blocks, the transposed block
-    double rs[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) rs[v] = 0.0;
-    for (int k = l; k < R; k += 64) sm.contrib[w][k] = 0.0;
-    wave_sync();
-#pragma unroll
-    for (int tt = 0; tt < NW; ++tt) {
-      if (tt > w) continue;
-      const double xj = sm.vx[32 * tt + (l & 31)];
-      double cs = 0.0;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        rs[v] += (double)acc[tt][v] * xj;
-        cs += (double)acc[tt][v] * sm.vx[32 * w + row32(v, l)];
-      }
-      if (tt < w) {
-        cs += __shfl_xor(cs, 32, 64);
-        if (l < 32) sm.contrib[w][32 * tt + l] = cs;
-      }
-    }
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) rs[v] += __shfl_xor(rs[v], o, 64);
-    if ((l & 31) == 0)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) sm.contrib[w][32 * w + row32(v, l)] = rs[v];
+    // r = m - lambda x - G x against the unfactored accumulators
+    gx_share<R>(sm, acc);
     __syncthreads();
     if (tid < R) {
       double r = sm.vm[tid] - lam * sm.vx[tid];
@@ -467,56 +607,88 @@ __device__ inline void store_row(const MixedArgs &a, MixedSmem<R> &sm, uint32_t 
 }
 
 // ---------------------------------------------------------------- kernels --
+// Gram + MTTKRP of one item into the accumulators and sm.vm
 template <int R, int N>
-__global__ void __launch_bounds__(2 * R) k_mixed_items(MixedArgs a) {
-  constexpr int NW = R / 32;
-  __shared__ MixedSmem<R> sm;
-  const uint4 it = a.items[blockIdx.x];
-  f32x16 acc[NW];
-#pragma unroll
-  for (int tt = 0; tt < NW; ++tt)
-#pragma unroll
-    for (int v = 0; v < 16; ++v) acc[tt][v] = 0.f;
+__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f32x16 *acc, MixedSmem<R> &sm) {
   float mp[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   gram_chunk<R, N>(a, it.y, it.z, acc, mp, (uint32_t *)sm.tiles);
   reduce_m<R>(mp, sm);
+}
+template <int R, int N>
+__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f64x4 *acc, MixedSmem<R> &sm) {
+  double mp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  gram_f64<R, N>(a, it.y, it.z, acc, mp, sm.tiles);
+  reduce_m<R>(mp, sm);
+}
+
+// accumulator slots of one wave and their place in a partial
+template <int R, int P>
+struct AccShape;
+template <int R>
+struct AccShape<R, kPrecBF16> {  // NW stripes of 32x32 blocks, 16 floats per lane each
+  typedef f32x16 T;
+  typedef float E;
+  static constexpr int NSLOT = R / 32, NV = 16;
+  static constexpr size_t PSZ = (size_t)R * R + R;  // elements per partial
+  __device__ static bool used(int w, int j) { return j <= w; }
+};
+template <int R>
+struct AccShape<R, kPrecF64> {  // round-robin 16x16 tiles, 4 doubles per lane each
+  typedef f64x4 T;
+  typedef double E;
+  static constexpr int NSLOT = MixedSmem<R>::TPW, NV = 4;
+  static constexpr size_t PSZ = (size_t)(R / 32) * MixedSmem<R>::TPW * 256 + R;
+  __device__ static bool used(int w, int j) { return w + (R / 32) * j < MixedSmem<R>::NTILE; }
+};
+
+template <int R, int N, int P>
+__global__ void __launch_bounds__(2 * R, 2) k_mixed_items(MixedArgs a) {
+  typedef AccShape<R, P> S;
+  __shared__ MixedSmem<R> sm;
+  const uint4 it = a.items[blockIdx.x];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  typename S::T acc[S::NSLOT];
+#pragma unroll
+  for (int j = 0; j < S::NSLOT; ++j)
+#pragma unroll
+    for (int v = 0; v < S::NV; ++v) acc[j][v] = 0;
+  gram_item<R, N>(a, it, acc, sm);
   if (it.w != kLightRow) {  // chunk of a heavy row: partial out
-    float *P = a.part + (size_t)it.w * ((size_t)R * R + R);
+    typename S::E *Pp = (typename S::E *)a.part + (size_t)it.w * S::PSZ;
 #pragma unroll
-    for (int tt = 0; tt < NW; ++tt) {
-      if (tt > w) continue;
+    for (int j = 0; j < S::NSLOT; ++j) {
+      if (!S::used(w, j)) continue;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) P[((w * NW + tt) * 16 + v) * 64 + l] = acc[tt][v];
+      for (int v = 0; v < S::NV; ++v) Pp[((w * S::NSLOT + j) * S::NV + v) * 64 + l] = acc[j][v];
     }
-    if (tid < R) P[R * R + tid] = (float)sm.vm[tid];
+    if (tid < R) Pp[S::PSZ - R + tid] = (typename S::E)sm.vm[tid];
     return;
   }
   const bool ok = solve_row<R>(sm, acc, a.lam, a.refine);
   store_row<R>(a, sm, it.x, ok);
 }
 
-template <int R, int N>
-__global__ void __launch_bounds__(2 * R) k_mixed_heavy(MixedArgs a) {
-  constexpr int NW = R / 32;
+template <int R, int N, int P>
+__global__ void __launch_bounds__(2 * R, 2) k_mixed_heavy(MixedArgs a) {
+  typedef AccShape<R, P> S;
   __shared__ MixedSmem<R> sm;
   const uint32_t h = blockIdx.x;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  f32x16 acc[NW];
+  typename S::T acc[S::NSLOT];
 #pragma unroll
-  for (int tt = 0; tt < NW; ++tt)
+  for (int j = 0; j < S::NSLOT; ++j)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[tt][v] = 0.f;
+    for (int v = 0; v < S::NV; ++v) acc[j][v] = 0;
   double mv = 0.0;
   for (uint32_t pi = a.heavy_pb[h]; pi < a.heavy_pb[h + 1]; ++pi) {
-    const float *P = a.part + (size_t)pi * ((size_t)R * R + R);
+    const typename S::E *Pp = (const typename S::E *)a.part + (size_t)pi * S::PSZ;
 #pragma unroll
-    for (int tt = 0; tt < NW; ++tt) {
-      if (tt > w) continue;
+    for (int j = 0; j < S::NSLOT; ++j) {
+      if (!S::used(w, j)) continue;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[tt][v] += P[((w * NW + tt) * 16 + v) * 64 + l];
+      for (int v = 0; v < S::NV; ++v) acc[j][v] += Pp[((w * S::NSLOT + j) * S::NV + v) * 64 + l];
     }
-    if (tid < R) mv += (double)P[R * R + tid];
+    if (tid < R) mv += (double)Pp[S::PSZ - R + tid];
   }
   if (tid < R) sm.vm[tid] = mv;
   __syncthreads();
@@ -529,35 +701,47 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *x, uint16_t *y, ui
     y[p] = f2bf(x[p]);
 }
 
-#define CTC_MIXED_SWITCH(KERNEL, GRID)                                                                    \
-  do {                                                                                                     \
-    const dim3 g((unsigned)(GRID)), b((unsigned)(2 * R));                                                 \
-    if (nmodes == 3) {                                                                                     \
-      if (R == 32) hipLaunchKernelGGL((KERNEL<32, 3>), g, dim3(64), 0, st, a);                            \
-      else if (R == 64) hipLaunchKernelGGL((KERNEL<64, 3>), g, dim3(128), 0, st, a);                      \
-      else if (R == 128) hipLaunchKernelGGL((KERNEL<128, 3>), g, dim3(256), 0, st, a);                    \
-      else return hipErrorInvalidValue;                                                                    \
-    } else if (nmodes == 4) {                                                                              \
-      if (R == 32) hipLaunchKernelGGL((KERNEL<32, 4>), g, dim3(64), 0, st, a);                            \
-      else if (R == 64) hipLaunchKernelGGL((KERNEL<64, 4>), g, dim3(128), 0, st, a);                      \
-      else if (R == 128) hipLaunchKernelGGL((KERNEL<128, 4>), g, dim3(256), 0, st, a);                    \
-      else return hipErrorInvalidValue;                                                                    \
-    } else {                                                                                               \
-      return hipErrorInvalidValue;                                                                         \
-    }                                                                                                      \
-    (void)b;                                                                                               \
+#define CTC_ROW_LAUNCH(KERNEL, RR, NN, GRID)                                                             \
+  do {                                                                                                  \
+    if (a.prec == kPrecBF16)                                                                            \
+      hipLaunchKernelGGL((KERNEL<RR, NN, kPrecBF16>), dim3((unsigned)(GRID)), dim3(2 * RR), 0, st, a); \
+    else                                                                                                \
+      hipLaunchKernelGGL((KERNEL<RR, NN, kPrecF64>), dim3((unsigned)(GRID)), dim3(2 * RR), 0, st, a);  \
+  } while (0)
+
+#define CTC_ROW_SWITCH(KERNEL, GRID)                   \
+  do {                                                 \
+    if (nmodes == 3) {                                 \
+      if (R == 32) CTC_ROW_LAUNCH(KERNEL, 32, 3, GRID);   \
+      else if (R == 64) CTC_ROW_LAUNCH(KERNEL, 64, 3, GRID); \
+      else if (R == 128) CTC_ROW_LAUNCH(KERNEL, 128, 3, GRID); \
+      else return hipErrorInvalidValue;                \
+    } else if (nmodes == 4) {                          \
+      if (R == 32) CTC_ROW_LAUNCH(KERNEL, 32, 4, GRID);   \
+      else if (R == 64) CTC_ROW_LAUNCH(KERNEL, 64, 4, GRID); \
+      else if (R == 128) CTC_ROW_LAUNCH(KERNEL, 128, 4, GRID); \
+      else return hipErrorInvalidValue;                \
+    } else {                                           \
+      return hipErrorInvalidValue;                     \
+    }                                                  \
   } while (0)
 
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
   if (a.nitems == 0) return hipSuccess;
-  CTC_MIXED_SWITCH(k_mixed_items, a.nitems);
+  CTC_ROW_SWITCH(k_mixed_items, a.nitems);
   return hipGetLastError();
 }
 
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
   if (a.nheavy == 0) return hipSuccess;
-  CTC_MIXED_SWITCH(k_mixed_heavy, a.nheavy);
+  CTC_ROW_SWITCH(k_mixed_heavy, a.nheavy);
   return hipGetLastError();
+}
+
+size_t mixed_partial_bytes(int R, int prec) {
+  const size_t NW = (size_t)R / 32, RT = (size_t)R / 16, NTILE = RT * (RT + 1) / 2;
+  const size_t TPW = (NTILE + NW - 1) / NW;
+  return prec == kPrecBF16 ? ((size_t)R * R + R) * sizeof(float) : (NW * TPW * 256 + R) * sizeof(double);
 }
 
 hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n) {

@@ -1,7 +1,8 @@
-// mixed.cpp — host driver of the BF16-operand ALS mode update (kernels in
-// kernels_mixed.hip).
+// mixed.cpp — host driver of the row-fused ALS mode update (kernels in
+// kernels_mixed.hip), in its BF16-operand (mixed precision) and FP64-Gram
+// (FP32 factors) forms.
 //
-// Per mode update: the other modes' factors are rounded to BF16 mirrors, the
+// Per mode update: for BF16 the other modes' factors are rounded to mirrors, the
 // factor being updated is zeroed (rows without nonzeros stay 0, as
 // SG/als.cu:699 does), the items pass solves every row of at most
 // `chunk` nonzeros in one workgroup and writes partial systems for the chunks
@@ -66,30 +67,49 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   return CTC_OK;
 }
 
-ctc_status als_mode_mixed(ctc_model *m, uint32_t mode, const ctc_options *o) {
+ctc_status als_rows_prepare(ctc_model *m, const ctc_options *o, bool bf16) {
+  const uint32_t R = m->rank;
+  size_t need = 1;
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    CTC_TRY(ensure_ciss(m, md));
+    CTC_TRY(ensure_mix_items(m, md, mix_chunk_of(o)));
+    need = std::max(need, std::max<size_t>(1, m->ciss[md].mix_nparts) *
+                              mixed_partial_bytes((int)R, bf16 ? kPrecBF16 : kPrecF64));
+    if (bf16 && !m->Fbf[md]) CTC_HIP(hipMalloc(&m->Fbf[md], m->dims[md] * R * sizeof(uint16_t)));
+  }
+  return ensure_scratch(m, need);
+}
+
+ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool bf16) {
   const uint32_t R = m->rank, N = m->nmodes;
   if (R != 32 && R != 64 && R != 128)
-    return fail(CTC_ERR_UNSUPPORTED, "mixed-precision ALS supports rank 32, 64 or 128 (got %u)", R);
+    return fail(CTC_ERR_UNSUPPORTED, "%s ALS supports rank 32, 64 or 128 (got %u)",
+                bf16 ? "mixed-precision" : "row-fused FP32", R);
   const uint32_t chunk = mix_chunk_of(o);
+  const int prec = bf16 ? kPrecBF16 : kPrecF64;
   CTC_TRY(ensure_ciss(m, mode));
   CTC_TRY(ensure_mix_items(m, mode, chunk));
   DevCiss &d = m->ciss[mode];
-  CTC_TRY(ensure_scratch(m, std::max<size_t>(1, d.mix_nparts) * ((size_t)R * R + R) * sizeof(float)));
+  CTC_TRY(ensure_scratch(m, std::max<size_t>(1, d.mix_nparts) * mixed_partial_bytes((int)R, prec)));
   hipStream_t st = m->ctx->stream;
   MixedArgs a{};
+  a.prec = prec;
   for (uint32_t s = 0; s + 1 < N; ++s) {
     const uint32_t md = (mode + 1 + s) % N;
-    if (!m->Fbf[md]) CTC_HIP(hipMalloc(&m->Fbf[md], m->dims[md] * R * sizeof(uint16_t)));
-    CTC_HIP(launch_to_bf16(st, m->F[md], m->Fbf[md], m->dims[md] * R));
     a.oidx[s] = d.oidx[s];
-    a.fb[s] = m->Fbf[md];
+    a.fr[s] = m->F[md];
+    if (bf16) {
+      if (!m->Fbf[md]) CTC_HIP(hipMalloc(&m->Fbf[md], m->dims[md] * R * sizeof(uint16_t)));
+      CTC_HIP(launch_to_bf16(st, m->F[md], m->Fbf[md], m->dims[md] * R));
+      a.fb[s] = m->Fbf[md];
+    }
   }
   a.vals = d.vals;
   a.directory = d.directory;
   a.out = m->F[mode];
   a.items = d.mix_items;
   a.nitems = d.mix_nitems;
-  a.part = (float *)m->scratch;
+  a.part = m->scratch;
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
   a.nheavy = d.mix_nheavy;
@@ -98,11 +118,11 @@ ctc_status als_mode_mixed(ctc_model *m, uint32_t mode, const ctc_options *o) {
   a.flag = m->dflag;
   CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
   {
-    Prof p(m->ctx, "als_mixed_items");
+    Prof p(m->ctx, bf16 ? "als_mixed_items" : "als_rows_items");
     CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
   }
   if (a.nheavy) {
-    Prof p(m->ctx, "als_mixed_heavy");
+    Prof p(m->ctx, bf16 ? "als_mixed_heavy" : "als_rows_heavy");
     CTC_HIP(launch_mixed_heavy(st, a, (int)N, (int)R));
   }
   m->resid_valid = false;

@@ -316,6 +316,7 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->precision = CTC_PREC_FP32;
   o->inner_iters = 1;                                                  // :30
   o->refine = 0;
+  o->als_kernel = CTC_ALS_AUTO;
   o->eval_every_epoch = 1;
   o->verbose = 0;
   o->sgd_update = CTC_SGD_AUTO;
@@ -444,6 +445,8 @@ extern "C" ctc_status ctc_model_create(ctc_context *ctx, const ctc_coo *train, c
   }
   if ((s = dev_alloc(&m->dstat, 4096)) != CTC_OK) return bail(s);
   if ((s = dev_alloc(&m->dflag, 16)) != CTC_OK) return bail(s);
+  if (hipMemsetAsync(m->dflag, 0, 16 * sizeof(uint32_t), st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    return bail(fail(CTC_ERR_HIP, "status flag initialisation"));
   if (hipHostMalloc((void **)&m->hstat, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return bail(fail(CTC_ERR_NOMEM, "pinned host buffer"));
   *out = m;
@@ -495,6 +498,8 @@ extern "C" ctc_status ctc_model_prepare(ctc_model *m, const ctc_options *o) {
   if (!m || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_model_prepare");
   CTC_HIP(hipSetDevice(m->ctx->device));
   if (o->algorithm == CTC_SGD) return ensure_ciss(m, 0);
+  if (o->algorithm == CTC_ALS && als_uses_rows(o, m->rank))
+    return als_rows_prepare(m, o, o->precision == CTC_PREC_MIXED);
   for (uint32_t md = 0; md < m->nmodes; ++md) {
     CTC_TRY(ensure_ciss(m, md));
     if (o->algorithm == CTC_ALS) CTC_TRY(ensure_pieces(m, md, span_tiles_of(o)));
@@ -507,8 +512,8 @@ extern "C" ctc_status ctc_model_prepare(ctc_model *m, const ctc_options *o) {
 // =========================================================================
 static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   const uint32_t R = m->rank;
-  if (o->precision == CTC_PREC_MIXED) return als_mode_mixed(m, mode, o);
-  if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "FP32 ALS at rank %u is not implemented (use the mixed precision)", R);
+  if (als_uses_rows(o, R)) return als_mode_rows(m, mode, o, o->precision == CTC_PREC_MIXED);
+  if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "the span ALS kernels support ranks up to 64 (got %u)", R);
   const uint32_t S = span_tiles_of(o);
   CTC_TRY(ensure_ciss(m, mode));
   CTC_TRY(ensure_pieces(m, mode, S));
@@ -1033,6 +1038,9 @@ extern "C" ctc_status ctc_autotune(ctc_model *m, const ctc_options *o, ctc_tune_
     ctc_status s2 = ctc_model_set_factor(m, md, saved[md].data());
     if (s == CTC_OK) s = s2;
   }
+  // the trial sweeps' status flags do not describe the restored model
+  (void)hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), st);
+  (void)hipStreamSynchronize(st);
   r.ms = best;
   *out = r;
   return s;

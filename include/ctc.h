@@ -177,6 +177,13 @@ typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mo
  * stored after every nonzero (concurrent writers overwrite each other),
  * "atomic" rows are re-read every nonzero and every update is added with a
  * float atomic (the reference's atomicAdd, SG/sgd.cu:96-98). */
+/* FP32 ALS kernels.  SPANS: waves walk fixed spans of the CISS and write
+ * per-(span, row) FP64 Gram pieces, a second kernel sums a row's pieces and
+ * solves (ranks 8-64).  ROWS: one workgroup per row forms its Gram on FP64
+ * matrix cores and solves it in place with a blocked FP64 Cholesky, long
+ * rows cut into chunks (ranks 32-128).  AUTO: ROWS for rank >= 32. */
+typedef enum ctc_als_kernel { CTC_ALS_AUTO = 0, CTC_ALS_SPANS = 1, CTC_ALS_ROWS = 2 } ctc_als_kernel;
+
 typedef enum ctc_sgd_update {
   CTC_SGD_AUTO = 0,      /* walked-mode rows racy; a row of another mode is racy when
                             it has more than sgd_hot_rows nonzeros (its atomics would
@@ -204,13 +211,14 @@ typedef struct ctc_options {
   int32_t mode_order;      /* ctc_mode_order; SG/als.cu:684-685 rotates at random */
   int32_t precision;       /* ctc_precision */
   uint32_t inner_iters;    /* CCD++ inner iterations, NUM_INNER SG/base.h:30 */
-  int32_t refine;          /* extra iterative-refinement step in the ALS solve */
+  int32_t refine;          /* iterative-refinement steps in the ALS solve (FP32 path: 0/1; mixed: at least 1) */
   int32_t eval_every_epoch;/* evaluate RMSE after every epoch (convergence needs it) */
   int32_t verbose;         /* print the reference's progress line per epoch */
   int32_t sgd_update;      /* ctc_sgd_update */
   int32_t spread;          /* 1: walkers visit spans in a strided order that
                               spreads a hot row's spans over the sweep */
   uint32_t sgd_hot_rows;   /* CTC_SGD_AUTO threshold (nonzeros per row; 0 = 262144) */
+  int32_t als_kernel;      /* ctc_als_kernel (FP32 ALS) */
 } ctc_options;
 
 /* Reference defaults for an algorithm (SG/base.h, SG/completion.c:460-480). */

@@ -350,6 +350,59 @@ int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, do
   return status;
 }
 
+/* The FP64 ALS update of selected rows only (same rule as orc_als_mode_f64,
+ * SG/als.cu:691-716), from float factors: out[r * R + f] for rows[r].  For
+ * checking a full-size update on a sample of rows.  Returns 0 or 7. */
+int orc_als_rows_f64(const orc_coo *t, uint32_t mode, uint32_t R, float *const *F, double lambda,
+                     const uint32_t *perm, const uint64_t *rowptr, const uint32_t *rows, uint64_t nrows,
+                     double *out, int nthreads) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  int status = 0;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t r = 0; r < (int64_t)nrows; ++r) {
+    const uint32_t i = rows[r];
+    double *dst = out + (size_t)r * R;
+    double *G = (double *)calloc((size_t)R * R, sizeof(double));
+    double *rhs = (double *)calloc(R, sizeof(double));
+    double h[512];
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const uint64_t p = perm[q];
+      for (uint32_t f = 0; f < R; ++f) h[f] = 1.0;
+      for (uint32_t m = 0; m < t->nmodes; ++m) {
+        if (m == mode) continue;
+        const float *row = F[m] + (size_t)t->ind[m][p] * R;
+        for (uint32_t f = 0; f < R; ++f) h[f] *= (double)row[f];
+      }
+      for (uint32_t a = 0; a < R; ++a) {
+        for (uint32_t b = 0; b <= a; ++b) G[(size_t)a * R + b] += h[a] * h[b];
+        rhs[a] += t->vals[p] * h[a];
+      }
+    }
+    for (uint32_t a = 0; a < R; ++a) {
+      for (uint32_t b = 0; b < a; ++b) G[(size_t)b * R + a] = G[(size_t)a * R + b];
+      G[(size_t)a * R + a] += lambda;
+    }
+    if (rowptr[i] == rowptr[i + 1]) {
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0.0;
+    } else if (orc_cholesky(G, R) != 0) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+      status = 7;
+      for (uint32_t f = 0; f < R; ++f) dst[f] = 0.0;
+    } else {
+      orc_cholesky_solve(G, R, rhs);
+      for (uint32_t f = 0; f < R; ++f) dst[f] = rhs[f];
+    }
+    free(G);
+    free(rhs);
+  }
+  return status;
+}
+
 int orc_converge(orc_conv *c, double converge_rmse, uint64_t epoch, int *improved) {
   *improved = 0;
   if (converge_rmse - c->best_rmse < -c->tolerance) { /* SG/completion.c:332-341 */

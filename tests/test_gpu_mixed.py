@@ -86,7 +86,10 @@ def test_mixed_four_modes(ctx):
 
 
 def test_mixed_als_converges_like_fp64(ctx):
-    """Five full sweeps at R=32: test RMSE within 1e-3 of the FP64 oracle's."""
+    """Six full sweeps at R=32.  Every sweep's test RMSE within 2e-4 of the
+    BF16-operand oracle's; the last within 1e-3 of the FP64 oracle's.  (The
+    BF16 rounding itself moves sweep 1 by 2.3e-3 on this tensor while the RMSE
+    is still near 0.9; from sweep 2 on the gap stays below 2e-4.)"""
     tr, te = problem((500, 400, 60), 60000, seed=70)
     R = 32
     m = ct.Model(ctx, tr, None, te, rank=R)
@@ -98,14 +101,18 @@ def test_mixed_als_converges_like_fp64(ctx):
     ind, vals = te.arrays()
     ot = O.Coo(ind, vals, te.dims)
     F64 = [f.astype(np.float64) for f in F]
+    Fb = [f.copy() for f in F]
     opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.5, precision=ct.PREC_MIXED)
-    for it in range(5):
+    for it in range(6):
         for mode in range(3):
             m.als_mode_update(mode, opt)
             assert O.als_mode(oc, mode, F64, 0.5) == 0
-        g, o = m.evaluate()["rmse_te"], O.rmse(ot, F64)
-        say(f"  sweep {it}: rmse_te mixed {g:.6f} fp64 {o:.6f}")
-        assert abs(g - o) < 1e-3
+            assert O.als_mode_bf16(oc, mode, Fb, 0.5) == 0
+        g = m.evaluate()["rmse_te"]
+        o64, ob = O.rmse(ot, F64), O.rmse(ot, [f.astype(np.float64) for f in Fb])
+        say(f"  sweep {it}: rmse_te mixed {g:.6f} bf16-oracle {ob:.6f} fp64 {o64:.6f}")
+        assert abs(g - ob) < 2e-4
+    assert abs(g - o64) < 1e-3
 
 
 def test_mixed_rejects_unsupported_rank(ctx):

@@ -82,17 +82,23 @@ def test_evaluate(ctx, nmodes):
     assert ev["frob"] == pytest.approx(O.frob_sq(tr.dims, F64, 0.5), rel=1e-6)
 
 
-@pytest.mark.parametrize("R", [8, 16, 32, 64])
+ALS_CASES = ([(R, ct.ALS_SPANS) for R in (8, 16, 32, 64)] +
+             [(R, ct.ALS_ROWS) for R in (32, 64, 128)])
+
+
+@pytest.mark.parametrize("R,kernel", ALS_CASES)
 @pytest.mark.parametrize("tile_len", [64, 256, 4096])
-def test_als_mode_update(ctx, R, tile_len):
+def test_als_mode_update(ctx, R, kernel, tile_len):
+    """SPANS: tile_len nonzeros per span; ROWS: rows longer than tile_len are
+    cut into chunks (64 makes most rows chunked)."""
     tr, te = small_problem(3, nnz=30000, seed=20 + R)
-    say(f"[als_mode] R={R} tile_len={tile_len}")
+    say(f"[als_mode] R={R} kernel={kernel} tile_len={tile_len}")
     m = ct.Model(ctx, tr, None, te, rank=R)
     F = init_factors(tr.dims, R, 5)
     m.set_factors(F)
     oc = ocoo(tr)
     F64 = [f.astype(np.float64) for f in F]
-    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, tile_len=tile_len)
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, tile_len=tile_len, als_kernel=kernel)
     for mode in range(3):
         m.als_mode_update(mode, opt)
         assert O.als_mode(oc, mode, F64, 0.05) == 0

@@ -341,6 +341,19 @@ def test_als_mode_bf16_equals_fp64_on_bf16_exact_inputs():
         np.testing.assert_allclose(F32[mode], F64[mode].astype(np.float32), rtol=2e-7, atol=1e-7)
 
 
+def test_als_rows_equals_full_mode_update():
+    rng = np.random.default_rng(33)
+    dims = (30, 20, 10)
+    t = rand_coo(rng, dims, 1500)
+    R = 8
+    F32 = rand_factors(rng, dims, R, np.float32)
+    rows = np.array([0, 5, 7, 29])
+    sub = O.als_rows(t, 0, F32, 0.1, rows)
+    F64 = [f.astype(np.float64) for f in F32]
+    assert O.als_mode(t, 0, F64, 0.1) == 0
+    np.testing.assert_array_equal(sub, F64[0][rows])
+
+
 def test_als_mode_bf16_rounding_error_is_bf16_sized():
     rng = np.random.default_rng(32)
     dims = (10, 9, 8)
