@@ -102,6 +102,21 @@ def mixed_leg(ct, ctx, a, log):
                 "gram_tflops_bf16": 3 * R * (R + 1) * tr.nnz / ((msi + msh) / a.mixed_iters * 1e-3) / 1e12,
                 "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
     log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), rmse_te {rm}")
+    # the plain FP32 path (FP64-accumulated Gram, same solve) on the same tensor, for the speed ratio
+    fopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1)
+    m.init_random(5)
+    m.prepare(fopt)
+    for md in range(3):
+        m.als_mode_update(md, fopt)  # warm-up sweep
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for md in range(3):
+        m.als_mode_update(md, fopt)
+    ctx.synchronize()
+    w32 = time.perf_counter() - t0
+    res.update({"fp32_s_per_iter": w32, "fp32_nnz_per_s": tr.nnz / w32, "speedup_vs_fp32": w32 / wall,
+                "fp32_rmse_te_after_2": m.evaluate(lam)["rmse_te"]})
+    log(f"FP32 row path R=128: {w32:.3f} s/iter; mixed speed-up {w32 / wall:.2f}x")
     return res
 
 
@@ -119,7 +134,8 @@ def main():
                     help="how concurrent walkers combine updates of shared rows (see ctc_sgd_update)")
     ap.add_argument("--sgd-hot-rows", type=int, default=0,
                     help="auto policy: rows with more nonzeros than this are updated racily (0 = default)")
-    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="training nonzeros in the CPU sample (0 = all)")
+    ap.add_argument("--cpu-epochs", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--algs", default="sgd,ccd,als,batch")
     ap.add_argument("--batch-problems", type=int, default=512)
@@ -243,31 +259,42 @@ def main():
 
     # ----------------------------------------------------------------- ALS
     if "als" in algs and a.als_epochs > 0:
-        aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=a.tile_len or None)
-        model.set_factors(F0)
-        t0 = time.time()
-        model.prepare(aopt)
-        log(f"ALS CISS build+upload {time.time() - t0:.1f}s")
-        for md in range(3):
-            model.als_mode_update(md, aopt)  # warm-up
-        ctx.reset_stats()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.als_epochs):
+        res_als = {}
+        for kname, kern in (("rows", ct.ALS_ROWS), ("spans", ct.ALS_SPANS)):
+            aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=a.tile_len or None,
+                              als_kernel=kern)
+            model.set_factors(F0)
+            t0 = time.time()
+            model.prepare(aopt)
+            log(f"ALS ({kname}) set-up {time.time() - t0:.1f}s")
             for md in range(3):
-                model.als_mode_update(md, aopt)
-        ctx.synchronize()
-        w = (time.perf_counter() - t0) / a.als_epochs
-        n_g, ms_g = ctx.kernel_stats("als_gram")
-        n_s, ms_s = ctx.kernel_stats("als_solve")
-        ev = model.evaluate(lam)
-        gram_ms = ms_g / max(n_g, 1)
-        out_algs["als"] = {"s_per_epoch": w, "nnz_per_s": train.nnz / w,
-                           "gram_kernel_ms_avg": gram_ms, "solve_kernel_ms_avg": ms_s / max(n_s, 1),
-                           "gram_tflops_fp64": R * (R + 1) * train.nnz / (gram_ms * 1e-3) / 1e12,
-                           "rmse_te": ev["rmse_te"], "epochs": 1 + a.als_epochs}
-        log(f"ALS {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), gram {gram_ms:.2f} ms, "
-            f"solve {ms_s / max(n_s, 1):.2f} ms, rmse_te {ev['rmse_te']:.4f}")
+                model.als_mode_update(md, aopt)  # warm-up
+            ctx.reset_stats()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.als_epochs):
+                for md in range(3):
+                    model.als_mode_update(md, aopt)
+            ctx.synchronize()
+            w = (time.perf_counter() - t0) / a.als_epochs
+            ev = model.evaluate(lam)
+            r = {"s_per_epoch": w, "nnz_per_s": train.nnz / w, "rmse_te": ev["rmse_te"], "epochs": 1 + a.als_epochs}
+            if kern == ct.ALS_ROWS:
+                n_i, ms_i = ctx.kernel_stats("als_rows_items")
+                n_h, ms_h = ctx.kernel_stats("als_rows_heavy")
+                r.update({"items_kernel_ms_per_mode": ms_i / max(n_i, 1), "heavy_kernel_ms_per_mode": ms_h / max(n_i, 1)})
+                log(f"ALS rows {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), items {ms_i / max(n_i, 1):.2f} ms"
+                    f" + heavy {ms_h / max(n_i, 1):.2f} ms per mode, rmse_te {ev['rmse_te']:.4f}")
+            else:
+                n_g, ms_g = ctx.kernel_stats("als_gram")
+                n_s, ms_s = ctx.kernel_stats("als_solve")
+                gram_ms = ms_g / max(n_g, 1)
+                r.update({"gram_kernel_ms_avg": gram_ms, "solve_kernel_ms_avg": ms_s / max(n_s, 1),
+                          "gram_tflops_fp64": R * (R + 1) * train.nnz / (gram_ms * 1e-3) / 1e12 if gram_ms else None})
+                log(f"ALS spans {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), gram {gram_ms:.2f} ms, "
+                    f"solve {ms_s / max(n_s, 1):.2f} ms, rmse_te {ev['rmse_te']:.4f}")
+            res_als[kname] = r
+        out_algs["als"] = res_als
 
     # ------------------------------------------------------- batched ALS
     if "batch" in algs and a.batch_problems > 0:
@@ -322,19 +349,23 @@ def main():
     cpu = None
     if not a.no_cpu:
         import oracle as O
-        n = min(a.cpu_sample, train.nnz)
+        n = min(a.cpu_sample or train.nnz, train.nnz)
         ind, vals = train.arrays()
         sub = O.Coo([x[:n] for x in ind], vals[:n], dims)
         rng = np.random.default_rng(1)
         F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
         order = np.arange(n, dtype=np.uint32)  # the training stream is in mode-0 order
+        lr = 1e-3
         t0 = time.perf_counter()
-        O.sgd_epoch(sub, F, 1e-3, lam, 1.0, order=order)
+        for _ in range(a.cpu_epochs):
+            O.sgd_epoch(sub, F, lr, lam, 1.0, order=order)
+            lr *= 0.95
         dt = time.perf_counter() - t0
-        cpu = {"value": n / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
-               "sample": f"one sequential FP32 SGD epoch (oracle) over the first {n} training "
-                         f"nonzeros in mode-0 order, R={R}; {dt:.1f} s"}
-        log(f"CPU baseline {n / dt / 1e6:.2f} Mnnz/s over {n} nnz ({dt:.1f}s)")
+        cpu = {"value": n * a.cpu_epochs / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
+               "sample": f"{a.cpu_epochs} sequential FP32 SGD epochs (the oracle's FP32 twin, one core) over "
+                         f"{'all' if n == train.nnz else 'the first'} {n} training nonzeros in mode-0 order, "
+                         f"R={R}; {dt:.1f} s"}
+        log(f"CPU baseline {n * a.cpu_epochs / dt / 1e6:.2f} Mnnz/s over {a.cpu_epochs} x {n} nnz ({dt:.1f}s)")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "sgd_traffic.json")

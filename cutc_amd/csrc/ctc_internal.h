@@ -166,7 +166,7 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
 ctc_status als_rows_prepare(ctc_model *m, const ctc_options *o, bool bf16);
 // which FP32 ALS kernels a model of rank R uses under these options
 inline bool als_uses_rows(const ctc_options *o, uint32_t R) {
-  return o->precision == CTC_PREC_MIXED || o->als_kernel == CTC_ALS_ROWS || (o->als_kernel == CTC_ALS_AUTO && R >= 32);
+  return o->precision == CTC_PREC_MIXED || o->als_kernel == CTC_ALS_ROWS || (o->als_kernel == CTC_ALS_AUTO && R >= 64);
 }
 // one ALS mode update, enqueued on the model's stream (no synchronisation, so
 // it can be captured into a graph once the model is prepared)
