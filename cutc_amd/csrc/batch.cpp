@@ -7,10 +7,9 @@
 // MTTKRP and solve of a row only involve the row's own nonzeros), and each
 // phase of a sweep is one kernel launch over all problems of the class, with
 // the row-segmented reductions keeping light and heavy rows of different
-// problems in the same launch.  Launched directly, each class runs on its own
-// stream (forked from and joined to the caller's stream); a run of `iters`
-// iterations can instead be captured once into a hipGraph on the caller's
-// stream and replayed.
+// problems in the same launch.  Each class runs on its own stream, forked
+// from and joined to the caller's stream; a run of `iters` iterations can be
+// captured once into a hipGraph with one branch per class and replayed.
 //
 // Because mode-0 global ids preserve problem order, each problem's training
 // nonzeros form one contiguous range of the class's mode-0 CISS and its test
@@ -290,29 +289,20 @@ extern "C" ctc_status ctc_batch_class_rank(const ctc_batch *b, uint32_t cls, uin
   return CTC_OK;
 }
 
-// enqueue `iters` sweeps of every class.  Direct launches fork from the
-// caller's stream into one branch per class stream and join back; a graph
-// capture records every class on the caller's stream, one class after the
-// other (each phase is already one launch over all problems of a class).
-static ctc_status enqueue_run(ctc_batch *b, const ctc_options *o, uint32_t iters, bool capture) {
+// enqueue `iters` sweeps of every class: fork from the caller's stream into
+// one branch per class stream, join back.  Under capture the same calls
+// record a graph whose class branches are concurrent.
+static ctc_status enqueue_run(ctc_batch *b, const ctc_options *o, uint32_t iters) {
   hipStream_t main = b->ctx->stream;
-  if (!capture) CTC_HIP(hipEventRecord(b->fork, main));
+  CTC_HIP(hipEventRecord(b->fork, main));
   for (auto &c : b->cls) {
     const ctc_options t = class_options(c, o);
-    hipStream_t own = c.ctx->stream;
-    if (capture)
-      c.ctx->stream = main;  // the class's launches go to the capturing stream
-    else
-      CTC_HIP(hipStreamWaitEvent(own, b->fork, 0));
-    ctc_status s = CTC_OK;
-    for (uint32_t it = 0; it < iters && s == CTC_OK; ++it)
-      for (uint32_t md = 0; md < b->nmodes && s == CTC_OK; ++md) s = als_mode_impl(c.model, md, &t);
-    c.ctx->stream = own;
-    if (s != CTC_OK) return s;
-    if (!capture) {
-      CTC_HIP(hipEventRecord(c.join, own));
-      CTC_HIP(hipStreamWaitEvent(main, c.join, 0));
-    }
+    hipStream_t st = c.ctx->stream;
+    CTC_HIP(hipStreamWaitEvent(st, b->fork, 0));
+    for (uint32_t it = 0; it < iters; ++it)
+      for (uint32_t md = 0; md < b->nmodes; ++md) CTC_TRY(als_mode_impl(c.model, md, &t));
+    CTC_HIP(hipEventRecord(c.join, st));
+    CTC_HIP(hipStreamWaitEvent(main, c.join, 0));
   }
   return CTC_OK;
 }
@@ -357,7 +347,7 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
     if (!b->exec || key != b->graph_key) {
       batch_release_graph(b);
       CTC_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
-      ctc_status s = enqueue_run(b, o, iters, true);
+      ctc_status s = enqueue_run(b, o, iters);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(main, &g);
       if (s != CTC_OK) {
@@ -375,7 +365,7 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
     CTC_HIP(hipEventRecord(b->t1, main));
   } else {
     CTC_HIP(hipEventRecord(b->t0, main));
-    CTC_TRY(enqueue_run(b, o, iters, false));
+    CTC_TRY(enqueue_run(b, o, iters));
     CTC_HIP(hipEventRecord(b->t1, main));
   }
   CTC_HIP(hipEventSynchronize(b->t1));
