@@ -225,8 +225,20 @@ __device__ inline void reduce_m(T *mp, MixedSmem<R> &sm) {
 }
 
 // ---------------------------------------------------------- solve phase --
+// 1/x to full FP64 precision: hardware reciprocal estimate + two Newton steps
+__device__ inline double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
 // wave-level: factor tile (K, K) (Cholesky), replace it by the inverse of its
-// lower factor; with fold, also y_K = inverse * vy_K in place
+// lower factor; with fold, also y_K = inverse * vy_K in place.
+// The factorisation eliminates in place in LDS without square roots (after
+// step k, a_ij for i > j holds l_ij d_j and a_jj the pivot d_j of A = L D L^T),
+// so each of its 16 dependent steps is one LDS round trip and one reciprocal;
+// the Cholesky factor C = L D^1/2 is formed at the end.  The inverse of C is
+// then formed column by column, one lane per column, by forward substitution.
 template <int R>
 __device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
   double *T = sm.tiles + tileno(K, K) * 256;
@@ -236,64 +248,67 @@ __device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
   for (int e = 0; e < 4; ++e) a[e] = T[tix(4 * q + e, c)];
   bool ok = true;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const double akk = __shfl(a[k & 3], k + 16 * (k >> 2), 64);
+  for (int k = 0; k < 15; ++k) {
+    const double akk = T[tix(k, k)];
     ok = ok && (akk > 0.0);
-    const double lkk = sqrt(akk > 0.0 ? akk : 1.0);
-    const double inv = 1.0 / lkk;
-    double cik[4];
+    const double rk = rcp_f64(akk > 0.0 ? akk : 1.0);
+    const double ack = T[tix(c, k)] * rk;  // l_ck
+    double aik[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) cik[e] = __shfl(a[e], k + 16 * q, 64) * inv;
-    const double t0 = __shfl(a[0], k + 16 * (c >> 2), 64);
-    const double t1 = __shfl(a[1], k + 16 * (c >> 2), 64);
-    const double t2 = __shfl(a[2], k + 16 * (c >> 2), 64);
-    const double t3 = __shfl(a[3], k + 16 * (c >> 2), 64);
-    const double cck = ((c & 3) == 0 ? t0 : (c & 3) == 1 ? t1 : (c & 3) == 2 ? t2 : t3) * inv;
+    for (int e = 0; e < 4; ++e) aik[e] = T[tix(4 * q + e, k)];
+    wave_sync();  // every lane has read step k's column before any write
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int i = 4 * q + e;
-      if (c == k) {
-        if (i == k)
-          a[e] = lkk;
-        else if (i > k)
-          a[e] = cik[e];
-      } else if (c > k && i > k) {
-        a[e] -= cik[e] * cck;
+      if (i >= c && c > k) {
+        a[e] = __builtin_fma(-aik[e], ack, a[e]);
+        T[tix(i, c)] = a[e];
       }
     }
-  }
-#pragma unroll
-  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = (4 * q + e >= c) ? a[e] : 0.0;
-  wave_sync();
-  // X = L^-1, column c: lane (c, q) holds X[4q + e][c]
-  double x[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-      if (4 * q + e < i) s += T[tix(i, 4 * q + e)] * x[e];
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
-    const double lii = T[tix(i, i)];
-    if (q == (i >> 2)) x[i & 3] = ((i == c ? 1.0 : 0.0) - s) / lii;
-  }
-  wave_sync();
-#pragma unroll
-  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = x[e];
-  if (!ok && l == 0) sm.bad = 1;
-  if (fold) {
-    double pv[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) pv[e] = x[e] * sm.vy[16 * K + c];
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) pv[e] += __shfl_xor(pv[e], o, 64);
     wave_sync();
-    if (c == 0)
+  }
+  ok = ok && (T[tix(15, 15)] > 0.0);
+  // C = L D^1/2: c_ic = a_ic / sqrt(d_c), c_cc = sqrt(d_c)
+  const double dc = T[tix(c, c)];
+  const double sdc = sqrt(dc > 0.0 ? dc : 1.0), rsd = rcp_f64(sdc);
+  wave_sync();
 #pragma unroll
-      for (int e = 0; e < 4; ++e) sm.vy[16 * K + 4 * q + e] = pv[e];
+  for (int e = 0; e < 4; ++e) {
+    const int i = 4 * q + e;
+    T[tix(i, c)] = i > c ? a[e] * rsd : (i == c ? sdc : 0.0);
+  }
+  wave_sync();
+  // X = C^-1, lane c < 16 forms column c: x_i = (delta_ic - sum_{j<i} c_ij x_j) / c_ii
+  double x[16];
+  if (l < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < i; ++j) {
+        if (j & 1)
+          s1 = __builtin_fma(-T[tix(i, j)], x[j], s1);
+        else
+          s0 = __builtin_fma(-T[tix(i, j)], x[j], s0);
+      }
+      x[i] = (s0 + s1) * rcp_f64(T[tix(i, i)]);
+    }
+  }
+  wave_sync();
+  if (l < 16)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) T[tix(i, c)] = i >= c ? x[i] : 0.0;
+  if (!ok && l == 0) sm.bad = 1;
+  wave_sync();
+  if (fold) {
+    // y_K = X b_K: lane i < 16 takes row i of X
+    double y = 0.0;
+    if (l < 16) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) y = __builtin_fma(T[tix(l, j)], sm.vy[16 * K + j], y);
+    }
+    wave_sync();
+    if (l < 16) sm.vy[16 * K + l] = y;
   }
   wave_sync();
 }
