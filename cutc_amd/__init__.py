@@ -95,6 +95,10 @@ class Coo:
         t.vals[:] = vals
         return t
 
+    def write_binary(self, path, base=1):
+        """SPLATT binary coordinate file (read back by Coo.read for *.bin)."""
+        check(lib().ctc_coo_write_binary(self.c, path.encode(), int(base)))
+
     def write(self, path, base=1, decimals=-1):
         check(lib().ctc_coo_write(self._p, str(path).encode(), base, decimals))
 

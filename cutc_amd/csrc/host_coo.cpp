@@ -166,8 +166,150 @@ struct LineChunk {
 
 }  // namespace
 
+// -------------------------------------------------------------------------
+// SPLATT binary coordinate files (SG/io.h:79-84, SG/io.c:116-189,369-513):
+//   int32 magic (0 = coordinate), uint64 index width (4 or 8), uint64 value
+//   width (4 or 8), then in the index width: nmodes, dims[nmodes], nnz, the
+//   nmodes index arrays of nnz entries each; then nnz values.
+// Indices are stored as the writer held them (1-based for the reference's
+// text-read tensors); the reader applies the text reader's base rule.
+// -------------------------------------------------------------------------
+namespace {
+
+bool has_suffix(const char *path, const char *suf) {
+  const size_t n = strlen(path), k = strlen(suf);
+  return n >= k && strcmp(path + n - k, suf) == 0;
+}
+
+ctc_status read_words(FILE *f, uint64_t width, uint64_t count, uint64_t *dst, const char *path) {
+  if (width == 8) {
+    if (count && fread(dst, 8, count, f) != count) return fail(CTC_ERR_PARSE, "'%s': truncated file", path);
+    return CTC_OK;
+  }
+  std::vector<uint32_t> buf(std::min<uint64_t>(count, 1u << 20));
+  for (uint64_t n = 0; n < count;) {
+    const uint64_t k = std::min<uint64_t>(buf.size(), count - n);
+    if (fread(buf.data(), 4, k, f) != k) return fail(CTC_ERR_PARSE, "'%s': truncated file", path);
+    for (uint64_t i = 0; i < k; ++i) dst[n + i] = buf[i];
+    n += k;
+  }
+  return CTC_OK;
+}
+
+ctc_status read_binary(const char *path, ctc_coo **out) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(CTC_ERR_IO, "failed to open '%s'", path);
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+  int32_t magic = -1;
+  uint64_t iw = 0, vw = 0;
+  if (fread(&magic, 4, 1, f) != 1 || fread(&iw, 8, 1, f) != 1 || fread(&vw, 8, 1, f) != 1)
+    return fail(CTC_ERR_PARSE, "'%s': truncated binary header", path);
+  if (magic != 0) return fail(CTC_ERR_PARSE, "'%s': not a binary coordinate tensor (magic %d)", path, magic);
+  if ((iw != 4 && iw != 8) || (vw != 4 && vw != 8))
+    return fail(CTC_ERR_PARSE, "'%s': index width %llu / value width %llu", path, (unsigned long long)iw,
+                (unsigned long long)vw);
+  uint64_t hdr[1 + CTC_MAX_MODES + 1];
+  CTC_TRY(read_words(f, iw, 1, hdr, path));
+  const uint64_t nmodes = hdr[0];
+  if (nmodes < 2 || nmodes > CTC_MAX_MODES) return fail(CTC_ERR_UNSUPPORTED, "'%s': %llu modes", path, (unsigned long long)nmodes);
+  CTC_TRY(read_words(f, iw, nmodes + 1, hdr + 1, path));
+  const uint64_t nnz = hdr[1 + nmodes];
+  uint64_t dims[CTC_MAX_MODES] = {0, 0, 0, 0};
+  for (uint64_t m = 0; m < nmodes; ++m) dims[m] = hdr[1 + m];
+  ctc_coo *t = nullptr;
+  CTC_TRY(ctc_coo_alloc((uint32_t)nmodes, nnz, dims, &t));
+  std::unique_ptr<ctc_coo, void (*)(ctc_coo *)> tg(t, ctc_coo_free);
+  std::vector<uint64_t> idx(std::min<uint64_t>(nnz, 1u << 22));
+  for (uint64_t m = 0; m < nmodes; ++m) {
+    uint64_t lo = ~0ull, hi = 0;
+    for (uint64_t n = 0; n < nnz;) {
+      const uint64_t k = std::min<uint64_t>(idx.size(), nnz - n);
+      CTC_TRY(read_words(f, iw, k, idx.data(), path));
+      for (uint64_t i = 0; i < k; ++i) {
+        lo = std::min(lo, idx[i]);
+        hi = std::max(hi, idx[i]);
+        t->ind[m][n + i] = (uint32_t)idx[i];
+      }
+      n += k;
+    }
+    // the text reader's rule (SG/io.c:263-300): base 0 if some index is 0, else 1
+    const uint32_t base = nnz && lo == 0 ? 0u : 1u;
+    if (hi - base >= 0xFFFFFFFFull) return fail(CTC_ERR_UNSUPPORTED, "'%s': index beyond 32 bits", path);
+    if (base)
+      for (uint64_t n = 0; n < nnz; ++n) t->ind[m][n] -= 1;
+    t->base[m] = base;
+    if (nnz && hi + 1 - base > t->dims[m]) t->dims[m] = hi + 1 - base;
+  }
+  if (vw == 8) {
+    if (nnz && fread(t->vals, 8, nnz, f) != nnz) return fail(CTC_ERR_PARSE, "'%s': truncated values", path);
+  } else {
+    std::vector<float> fb(std::min<uint64_t>(nnz, 1u << 20));
+    for (uint64_t n = 0; n < nnz;) {
+      const uint64_t k = std::min<uint64_t>(fb.size(), nnz - n);
+      if (fread(fb.data(), 4, k, f) != k) return fail(CTC_ERR_PARSE, "'%s': truncated values", path);
+      for (uint64_t i = 0; i < k; ++i) t->vals[n + i] = fb[i];
+      n += k;
+    }
+  }
+  *out = tg.release();
+  return CTC_OK;
+}
+
+}  // namespace
+
+extern "C" ctc_status ctc_coo_write_binary(const ctc_coo *t, const char *path, uint32_t base) {
+  if (!t || !path || base > 1) return fail(CTC_ERR_INVALID_ARG, "ctc_coo_write_binary");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(CTC_ERR_IO, "failed to open '%s' for writing", path);
+  std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+  // narrowest exact widths (SG/io.c:116-152 picks them the same way)
+  uint64_t iw = 4, vw = 4;
+  if (t->nnz >= 0xFFFFFFFFull) iw = 8;
+  for (uint32_t m = 0; m < t->nmodes; ++m)
+    if (t->dims[m] + base >= 0xFFFFFFFFull) iw = 8;
+  for (uint64_t p = 0; p < t->nnz && vw == 4; ++p)
+    if ((double)(float)t->vals[p] != t->vals[p]) vw = 8;
+  const int32_t magic = 0;
+  bool ok = fwrite(&magic, 4, 1, f) == 1 && fwrite(&iw, 8, 1, f) == 1 && fwrite(&vw, 8, 1, f) == 1;
+  auto put = [&](uint64_t v) {
+    if (iw == 8) {
+      ok = ok && fwrite(&v, 8, 1, f) == 1;
+    } else {
+      const uint32_t w = (uint32_t)v;
+      ok = ok && fwrite(&w, 4, 1, f) == 1;
+    }
+  };
+  put(t->nmodes);
+  for (uint32_t m = 0; m < t->nmodes; ++m) put(t->dims[m]);
+  put(t->nnz);
+  std::vector<uint32_t> b32;
+  std::vector<uint64_t> b64;
+  for (uint32_t m = 0; m < t->nmodes && ok; ++m) {
+    if (iw == 8) {
+      b64.resize(t->nnz);
+      for (uint64_t p = 0; p < t->nnz; ++p) b64[p] = (uint64_t)t->ind[m][p] + base;
+      ok = ok && fwrite(b64.data(), 8, t->nnz, f) == t->nnz;
+    } else {
+      b32.resize(t->nnz);
+      for (uint64_t p = 0; p < t->nnz; ++p) b32[p] = t->ind[m][p] + base;
+      ok = ok && fwrite(b32.data(), 4, t->nnz, f) == t->nnz;
+    }
+  }
+  if (vw == 8) {
+    ok = ok && fwrite(t->vals, 8, t->nnz, f) == t->nnz;
+  } else {
+    std::vector<float> fv(t->nnz);
+    for (uint64_t p = 0; p < t->nnz; ++p) fv[p] = (float)t->vals[p];
+    ok = ok && fwrite(fv.data(), 4, t->nnz, f) == t->nnz;
+  }
+  if (!ok) return fail(CTC_ERR_IO, "write error on '%s'", path);
+  return CTC_OK;
+}
+
 extern "C" ctc_status ctc_coo_read(const char *path, ctc_coo **out) {
   if (!path || !out) return fail(CTC_ERR_INVALID_ARG, "ctc_coo_read: null argument");
+  // file type by extension, as SG/io.c:21-52 decides (.bin: binary coordinates)
+  if (has_suffix(path, ".bin")) return read_binary(path, out);
   FILE *f = fopen(path, "rb");
   if (!f) return fail(CTC_ERR_IO, "failed to open '%s'", path);
   fseek(f, 0, SEEK_END);

@@ -12,6 +12,8 @@ namespace ctc {
 // kLightRow: the whole row, Gram and solve in one workgroup; otherwise the
 // chunk writes its partial Gram and right-hand side to partial `part`.
 constexpr uint32_t kLightRow = 0xFFFFFFFFu;
+// rows with at most this many nonzeros are solved in the dual (n x n) form
+constexpr int kThinRow = 32;
 
 // Gram precision: BF16 operands with FP32 accumulation, or FP32 factors with
 // h formed and accumulated in FP64
@@ -27,6 +29,8 @@ struct MixedArgs {
   float *out;             // factor of the mode being updated, row-major dims x R
   const uint4 *items;
   uint64_t nitems;
+  const uint4 *thin;  // thin rows {slot, first, end, -}: n <= min(kThinRow, R)
+  uint64_t nthin;
   void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])
@@ -39,6 +43,7 @@ struct MixedArgs {
 // items pass (light rows solved, heavy chunks to partials), then heavy rows
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 size_t mixed_partial_bytes(int R, int prec);
 // FP32 -> BF16 (round to nearest even), n elements
 hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n);

@@ -4,8 +4,8 @@
 // SG/als.cu:467-491,715-716 (G + lambda I) a = m per row), in the precision
 // split of the mixed path: h = product of the BF16 factor rows (exact in
 // FP32), G = sum bf16(h) bf16(h)^T on BF16 MFMA with FP32 accumulation,
-// m = sum x h in FP32, and the R x R system factored and solved in FP64 with
-// iterative refinement against the FP32 Gram.
+// m = sum x bf16(h) in FP32, and the R x R system factored and solved in FP64
+// with iterative refinement against the FP32 Gram.
 //
 // One workgroup of 2R threads (R/32 waves) owns one row.  The row-major
 // per-row Gram never touches HBM: wave w accumulates the 32-row stripe w of
@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels_mixed.h"
 
@@ -110,8 +111,8 @@ __device__ inline void gram_chunk(const MixedArgs &a, uint32_t p0, uint32_t p1, 
         const float x = a.vals[p];
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          mp[f] += x * hf[f];
           hv[f][h] = f2bf(hf[f]);
+          mp[f] += x * __uint_as_float(hv[f][h] << 16);
         }
       } else {
 #pragma unroll
@@ -621,6 +622,137 @@ __device__ inline void store_row(const MixedArgs &a, MixedSmem<R> &sm, uint32_t 
   if (!ok && tid == 0) atomicOr(a.flag, 1u);
 }
 
+// ------------------------------------------------------------- thin rows --
+// A row with n <= kThinRow nonzeros is solved in its dual form,
+//   a = H^T (H H^T + lambda I)^-1 x,
+// which equals (H^T H + lambda I)^-1 H^T x exactly (H = the row's n x R
+// matrix of h vectors, x its values): an n x n FP64 system in place of the
+// R x R one.  One wave per row: h staged in LDS, K = H H^T on FP64 matrix
+// cores, K + lambda I eliminated in place (L D L^T, no square roots), two
+// triangular solves, a = H^T z.  All in FP64 (the BF16 path stages the
+// BF16-rounded h), so no refinement step is needed.
+template <int R, int P>
+struct ThinSmem {
+  typedef typename std::conditional<P == kPrecBF16, float, double>::type HT;
+  static constexpr int HS = R + 1;  // staged h stride (odd: spreads LDS banks)
+  static constexpr int KS = kThinRow + 1;
+  HT H[kThinRow * HS];
+  double K[kThinRow * KS];
+  double x[kThinRow];
+  int bad;
+};
+
+template <int R, int N, int P>
+__global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
+  typedef ThinSmem<R, P> TS;
+  __shared__ TS sm;
+  const uint4 it = a.thin[blockIdx.x];
+  const int l = threadIdx.x;
+  const int n = (int)(it.z - it.y);
+  const int nt = n <= 16 ? 16 : 32;  // system size handled (zero-padded)
+  // 1. stage h for the row's nonzeros (rows >= n zero) and x
+  for (int idx = l; idx < nt * R; idx += 64) {
+    const int p = idx / R, f = idx % R;
+    typename TS::HT h = 0;
+    if (p < n) {
+      const uint32_t q = it.y + p;
+      if (P == kPrecBF16) {
+        float hf = 1.f;
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) hf *= __uint_as_float((uint32_t)a.fb[s][(size_t)a.oidx[s][q] * R + f] << 16);
+        h = __uint_as_float((uint32_t)f2bf(hf) << 16);
+      } else {
+        double hd = 1.0;
+#pragma unroll
+        for (int s = 0; s < N - 1; ++s) hd *= (double)a.fr[s][(size_t)a.oidx[s][q] * R + f];
+        h = hd;
+      }
+    }
+    sm.H[p * TS::HS + f] = h;
+  }
+  if (l < kThinRow) sm.x[l] = l < n ? (double)a.vals[it.y + l] : 0.0;
+  if (l == 0) sm.bad = 0;
+  wave_sync();
+  // 2. K = H H^T (lower 16x16 tiles) on v_mfma_f64_16x16x4f64
+  f64x4 k00 = {0.0, 0.0, 0.0, 0.0}, k10 = k00, k11 = k00;
+  for (int f0 = 0; f0 < R; f0 += 4) {
+    const int f = f0 + (l >> 4);
+    const double h0 = (double)sm.H[(l & 15) * TS::HS + f];
+    k00 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0, h0, k00, 0, 0, 0);
+    if (nt == 32) {
+      const double h1 = (double)sm.H[(16 + (l & 15)) * TS::HS + f];
+      k10 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1, h0, k10, 0, 0, 0);
+      k11 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1, h1, k11, 0, 0, 0);
+    }
+  }
+  const double lam = a.lam;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = row16(v, l), c = l & 15;
+    // padded rows/columns (>= n) become the identity
+    auto put = [&](int rr, int cc, double val) {
+      const bool pad = rr >= n || cc >= n;
+      sm.K[rr * TS::KS + cc] = pad ? (rr == cc ? 1.0 : 0.0) : val + (rr == cc ? lam : 0.0);
+    };
+    put(r, c, k00[v]);
+    if (nt == 32) {
+      put(16 + r, c, k10[v]);
+      put(16 + r, 16 + c, k11[v]);
+    }
+  }
+  wave_sync();
+  // 3. K = L D L^T in place: after step k, K[i][j] (i > j) = l_ij d_j and
+  // K[j][j] = d_j.  Lane l updates column c = l % 32 for rows of its half.
+  {
+    const int c = l & 31, half = l >> 5;
+    bool ok = true;
+    for (int k = 0; k < nt - 1; ++k) {
+      const double dk = sm.K[k * TS::KS + k];
+      ok = ok && dk > 0.0;
+      const double lck = sm.K[c * TS::KS + k] * rcp_f64(dk > 0.0 ? dk : 1.0);
+      double aik[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) aik[e] = sm.K[(16 * half + e) * TS::KS + k];
+      wave_sync();
+      if (c > k && c < nt) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int i = 16 * half + e;
+          if (i >= c && i < nt) sm.K[i * TS::KS + c] = __builtin_fma(-aik[e], lck, sm.K[i * TS::KS + c]);
+        }
+      }
+      wave_sync();
+    }
+    ok = ok && sm.K[(nt - 1) * TS::KS + nt - 1] > 0.0;
+    if (!ok && l == 0) sm.bad = 1;
+  }
+  wave_sync();
+  // 4. z = (L D L^T)^-1 x: lane i < nt holds x_i
+  double xi = l < nt ? sm.x[l] : 0.0;
+  const double di = l < nt ? sm.K[l * TS::KS + l] : 1.0;
+  for (int j = 0; j < nt - 1; ++j) {  // forward, unit lower L
+    const double yj = __shfl(xi, j, 64);
+    if (l > j && l < nt) xi = __builtin_fma(-sm.K[l * TS::KS + j] * rcp_f64(sm.K[j * TS::KS + j]), yj, xi);
+  }
+  xi *= rcp_f64(di);
+  for (int j = nt - 1; j > 0; --j) {  // backward, L^T
+    const double zj = __shfl(xi, j, 64);
+    if (l < j) xi = __builtin_fma(-sm.K[j * TS::KS + l] * rcp_f64(di), zj, xi);
+  }
+  wave_sync();
+  if (l < kThinRow) sm.x[l] = l < nt ? xi : 0.0;
+  wave_sync();
+  // 5. a = H^T z
+  float *dst = a.out + (size_t)a.directory[it.x] * R;
+  const bool ok = sm.bad == 0;
+  for (int f = l; f < R; f += 64) {
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc = __builtin_fma((double)sm.H[p * TS::HS + f], sm.x[p], acc);
+    dst[f] = ok ? (float)acc : 0.f;
+  }
+  if (!ok && l == 0) atomicOr(a.flag, 1u);
+}
+
 // ---------------------------------------------------------------- kernels --
 // Gram + MTTKRP of one item into the accumulators and sm.vm
 template <int R, int N>
@@ -750,6 +882,32 @@ hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, in
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
   if (a.nheavy == 0) return hipSuccess;
   CTC_ROW_SWITCH(k_mixed_heavy, a.nheavy);
+  return hipGetLastError();
+}
+
+#define CTC_THIN_LAUNCH(RR, NN)                                                                         \
+  do {                                                                                                 \
+    if (a.prec == kPrecBF16)                                                                           \
+      hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecBF16>), dim3((unsigned)a.nthin), dim3(64), 0, st, a); \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecF64>), dim3((unsigned)a.nthin), dim3(64), 0, st, a);  \
+  } while (0)
+
+hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
+  if (a.nthin == 0) return hipSuccess;
+  if (nmodes == 3) {
+    if (R == 32) CTC_THIN_LAUNCH(32, 3);
+    else if (R == 64) CTC_THIN_LAUNCH(64, 3);
+    else if (R == 128) CTC_THIN_LAUNCH(128, 3);
+    else return hipErrorInvalidValue;
+  } else if (nmodes == 4) {
+    if (R == 32) CTC_THIN_LAUNCH(32, 4);
+    else if (R == 64) CTC_THIN_LAUNCH(64, 4);
+    else if (R == 128) CTC_THIN_LAUNCH(128, 4);
+    else return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -6,8 +6,9 @@
 // factor being updated is zeroed (rows without nonzeros stay 0, as
 // SG/als.cu:699 does), the items pass solves every row of at most
 // `chunk` nonzeros in one workgroup and writes partial systems for the chunks
-// of longer rows, and the heavy pass sums each long row's partials and
-// solves it.
+// of longer rows, the heavy pass sums each long row's partials and solves it,
+// and the thin pass solves the rows of at most min(32, R) nonzeros in their
+// dual n x n form, one wave each.
 #include <algorithm>
 #include <vector>
 
@@ -27,11 +28,16 @@ static uint32_t mix_chunk_of(const ctc_options *o) {
 static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) {
   DevCiss &d = m->ciss[mode];
   if (d.mix_items && d.mix_chunk == chunk) return CTC_OK;
-  std::vector<uint4> chunks, light;
+  const uint32_t thin = std::min<uint32_t>((uint32_t)kThinRow, m->rank);
+  std::vector<uint4> chunks, light, thins;
   std::vector<uint32_t> hslot, hpb{0};
   uint32_t nparts = 0;
   for (uint64_t s = 0; s < d.dlength; ++s) {
     const uint32_t p0 = d.h_rowptr[s], p1 = d.h_rowptr[s + 1];
+    if (p1 - p0 <= thin) {
+      thins.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
+      continue;
+    }
     if (p1 - p0 <= chunk) {
       light.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
       continue;
@@ -40,11 +46,13 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
     hslot.push_back((uint32_t)s);
     hpb.push_back(nparts);
   }
-  // the long rows' chunks first, so the longest work starts earliest
+  // the long rows' chunks first, so the longest work starts earliest; the
+  // thin rows (dual-form kernel) last
   std::vector<uint4> items;
-  items.reserve(chunks.size() + light.size());
+  items.reserve(chunks.size() + light.size() + thins.size());
   items.insert(items.end(), chunks.begin(), chunks.end());
   items.insert(items.end(), light.begin(), light.end());
+  items.insert(items.end(), thins.begin(), thins.end());
   for (void *p : {(void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb})
     if (p) CTC_HIP(hipFree(p));
   d.mix_items = nullptr;
@@ -61,7 +69,8 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   CTC_HIP(hipMemcpyAsync(d.mix_heavy_pb, hpb.data(), hpb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   CTC_HIP(hipStreamSynchronize(st));
   d.mix_chunk = chunk;
-  d.mix_nitems = items.size();
+  d.mix_nitems = items.size() - thins.size();
+  d.mix_nthin = thins.size();
   d.mix_nheavy = hslot.size();
   d.mix_nparts = nparts;
   return CTC_OK;
@@ -109,6 +118,8 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.out = m->F[mode];
   a.items = d.mix_items;
   a.nitems = d.mix_nitems;
+  a.thin = d.mix_items + d.mix_nitems;
+  a.nthin = d.mix_nthin;
   a.part = m->scratch;
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
@@ -124,6 +135,10 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   if (a.nheavy) {
     Prof p(m->ctx, bf16 ? "als_mixed_heavy" : "als_rows_heavy");
     CTC_HIP(launch_mixed_heavy(st, a, (int)N, (int)R));
+  }
+  if (a.nthin) {
+    Prof p(m->ctx, bf16 ? "als_mixed_thin" : "als_rows_thin");
+    CTC_HIP(launch_rows_thin(st, a, (int)N, (int)R));
   }
   m->resid_valid = false;
   return CTC_OK;

@@ -70,6 +70,12 @@ ctc_status ctc_coo_read(const char *path, ctc_coo **out);
  * values with %.17g, otherwise with %.<decimals>f (software/scripts/tvtensor.py:76-82
  * uses %.1f). */
 ctc_status ctc_coo_write(const ctc_coo *t, const char *path, uint32_t base, int decimals);
+/* Writes a SPLATT binary coordinate file (tt_write_binary_file, SG/io.c:369-420;
+ * header SG/io.c:116-152): 4-byte indices and values where they are exact,
+ * 8-byte otherwise; indices stored with the given base.  ctc_coo_read reads
+ * files ending in .bin in this format (SG/io.c:162-189, 430-513, with the
+ * narrow-index path that the reference cannot take fixed). */
+ctc_status ctc_coo_write_binary(const ctc_coo *t, const char *path, uint32_t base);
 void ctc_coo_free(ctc_coo *t);
 
 /* ------------------------------------------------------------------------ */

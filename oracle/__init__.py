@@ -249,7 +249,7 @@ def bf16_round(x):
 
 def als_mode_bf16(t: Coo, mode, F, lam, order=None, nthreads=1):
     """In-place ALS update of F[mode] (float32 factors) in the arithmetic of the
-    BF16-operand path: BF16-rounded factor rows, BF16-rounded h in the Gram,
+    BF16-operand path: BF16-rounded factor rows, BF16-rounded h in the Gram and the right-hand side,
     FP64 accumulation and solve.  Returns the status (0 ok, 7 not SPD)."""
     perm, rowptr = order if order is not None else mode_order(t, mode)
     R = F[0].shape[1]

@@ -294,9 +294,9 @@ float orc_bf16_round(float x) {
 /* The BF16-operand ALS update (the mixed-precision path's arithmetic, no
  * counterpart in the reference): factor rows of the other modes are rounded
  * to BF16, h is their product (exact in FP32 for up to three BF16 factors),
- * G = sum bf16(h) bf16(h)^T and m = sum x h are accumulated in FP64 here,
- * and (G + lambda I) a = m is solved in FP64.  Rows with no nonzero become 0.
- * Returns 0, or 7 if a system is not positive definite. */
+ * G = sum bf16(h) bf16(h)^T and m = sum x bf16(h) are accumulated in FP64
+ * here, and (G + lambda I) a = m is solved in FP64.  Rows with no nonzero
+ * become 0.  Returns 0, or 7 if a system is not positive definite. */
 int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, double lambda,
                       const uint32_t *perm, const uint64_t *rowptr, int nthreads) {
   if (mode >= t->nmodes || R == 0 || R > 512) return 1;
@@ -327,7 +327,7 @@ int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, do
       const double x = t->vals[p];
       for (uint32_t a = 0; a < R; ++a) {
         for (uint32_t b = 0; b <= a; ++b) G[(size_t)a * R + b] += (double)hb[a] * (double)hb[b];
-        rhs[a] += x * (double)h[a];
+        rhs[a] += x * (double)hb[a];
       }
     }
     for (uint32_t a = 0; a < R; ++a) {

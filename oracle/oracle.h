@@ -115,7 +115,7 @@ void orc_cholesky_solve(const double *L, uint32_t n, double *b);
 /* ---- BF16-operand ALS (mixed-precision path) ------------------------------ */
 /* float -> bfloat16 -> float, round to nearest even */
 float orc_bf16_round(float x);
-/* ALS mode update with BF16-rounded factor rows and BF16-rounded h in the Gram,
+/* ALS mode update with BF16-rounded factor rows and BF16-rounded h in the Gram and m,
  * FP64 accumulation and solve; float factors.  Returns 0 or 7 (not SPD). */
 int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, double lambda,
                       const uint32_t *perm, const uint64_t *rowptr, int nthreads);

@@ -1,7 +1,7 @@
 """BF16-operand ALS (mixed precision) against the oracle.
 
 The oracle's `als_mode_bf16` restates the path's arithmetic: BF16-rounded
-factor rows, BF16-rounded h in the Gram, then an FP64 solve; it accumulates
+factor rows, BF16-rounded h in the Gram and in m, then an FP64 solve; it accumulates
 G and m in FP64 where the GPU accumulates in FP32 (MFMA), so the bar is 1e-4
 relative per factor (FP32 accumulation of up to thousands of terms, amplified
 by the condition number of G + lambda I).  The distance to the plain FP64

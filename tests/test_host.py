@@ -79,6 +79,76 @@ def test_tns_write_roundtrip(tmp_path):
     assert np.allclose(c.vals, np.round(tr.vals, 1), atol=1e-12)
 
 
+def _splatt_bin(path, iw, vw, dims, ind, vals):
+    """A SPLATT binary coordinate file packed by hand from its published layout
+    (int32 magic 0, uint64 index width, uint64 value width, then nmodes, dims,
+    nnz, the index arrays and the values)."""
+    import struct
+    fi, fv = ("<I" if iw == 4 else "<Q"), ("<f" if vw == 4 else "<d")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iQQ", 0, iw, vw))
+        for v in [len(dims), *dims, len(vals)]:
+            f.write(struct.pack(fi, v))
+        for arr in ind:
+            for v in arr:
+                f.write(struct.pack(fi, v))
+        for v in vals:
+            f.write(struct.pack(fv, v))
+
+
+@pytest.mark.parametrize("iw,vw", [(4, 4), (8, 8), (4, 8), (8, 4)])
+def test_binary_reader_known_answer(tmp_path, iw, vw):
+    path = str(tmp_path / "t.bin")
+    ind = [[1, 3, 3], [2, 1, 2], [1, 2, 2]]  # 1-based, as the reference stores them
+    vals = [1.5, -2.25, 4.0]
+    _splatt_bin(path, iw, vw, (3, 2, 2), ind, vals)
+    t = ct.Coo.read(path)
+    got_ind, got_vals = t.arrays()
+    assert t.dims == (3, 2, 2) and t.nnz == 3 and t.base == (1, 1, 1)
+    for m in range(3):
+        assert list(got_ind[m]) == [v - 1 for v in ind[m]]
+    assert list(got_vals) == vals
+
+
+def test_binary_reader_zero_based_and_errors(tmp_path):
+    path = str(tmp_path / "z.bin")
+    _splatt_bin(path, 8, 8, (4, 3, 2), [[0, 3], [2, 0], [1, 0]], [0.1, 0.2])
+    t = ct.Coo.read(path)
+    assert t.base == (0, 0, 0) and t.dims == (4, 3, 2)
+    assert [list(x) for x in t.arrays()[0]] == [[0, 3], [2, 0], [1, 0]]
+    with open(path, "r+b") as f:  # truncate the values
+        f.truncate(os.path.getsize(path) - 4)
+    with pytest.raises(ct.CtcError):
+        ct.Coo.read(path)
+    bad = str(tmp_path / "bad.bin")
+    with open(bad, "wb") as f:
+        f.write(b"\x01\x00\x00\x00" + b"\x00" * 16)
+    with pytest.raises(ct.CtcError):
+        ct.Coo.read(bad)
+
+
+def test_binary_roundtrip_widths(tmp_path):
+    tr, _ = ct.synth((50, 40, 30), 2000, test_fraction=0.0, seed=8)
+    path = str(tmp_path / "r.bin")
+    tr.write_binary(path)
+    back = ct.Coo.read(path)
+    a, b = tr.arrays(), back.arrays()
+    assert back.dims == tr.dims
+    for m in range(3):
+        assert np.array_equal(a[0][m], b[0][m])
+    assert np.array_equal(a[1], b[1])
+    # 4-byte indices and, for float-exact values, 4-byte values
+    assert struct_header(path) == (0, 4, 8)
+    ct.Coo.from_arrays(a[0], np.round(a[1] * 4) / 4, tr.dims).write_binary(path)
+    assert struct_header(path) == (0, 4, 4)
+
+
+def struct_header(path):
+    import struct
+    with open(path, "rb") as f:
+        return struct.unpack("<iQQ", f.read(20))
+
+
 def test_synth_is_seeded_distinct_and_exact():
     a, at = ct.synth((200, 150, 30), 20000, zipf=(1.0, 1.0, 0.0), true_rank=4,
                      value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.05, seed=7)
