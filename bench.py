@@ -87,36 +87,43 @@ def mixed_leg(ct, ctx, a, log):
         m.als_mode_update(md, opt)
     rm = [m.evaluate(lam)["rmse_te"]]
     ctx.reset_stats()
-    ctx.synchronize()
-    t0 = time.perf_counter()
+    wall = 0.0
     for _ in range(a.mixed_iters):
+        ctx.synchronize()
+        t0 = time.perf_counter()
         for md in range(3):
             m.als_mode_update(md, opt)
         ctx.synchronize()
-        rm.append(m.evaluate(lam)["rmse_te"])  # inside the loop, outside the kernel stats
-    wall = (time.perf_counter() - t0) / a.mixed_iters
+        wall += time.perf_counter() - t0
+        rm.append(m.evaluate(lam)["rmse_te"])  # untimed
+    wall /= a.mixed_iters
     ni, msi = ctx.kernel_stats("als_mixed_items")
     nh, msh = ctx.kernel_stats("als_mixed_heavy")
+    nt, mst = ctx.kernel_stats("als_mixed_thin")
     res.update({"s_per_iter": wall, "nnz_per_s": tr.nnz / wall,
                 "items_kernel_ms_per_iter": msi / a.mixed_iters, "heavy_kernel_ms_per_iter": msh / a.mixed_iters,
-                "gram_tflops_bf16": 3 * R * (R + 1) * tr.nnz / ((msi + msh) / a.mixed_iters * 1e-3) / 1e12,
+                "thin_kernel_ms_per_iter": mst / a.mixed_iters,
                 "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
-    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), rmse_te {rm}")
-    # the plain FP32 path (FP64-accumulated Gram, same solve) on the same tensor, for the speed ratio
+    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), items {msi / a.mixed_iters:.0f} ms, "
+        f"heavy {msh / a.mixed_iters:.0f} ms, thin {mst / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
+    # the plain FP32 path (FP64-accumulated Gram, same solves) from the same init
     fopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1)
     m.init_random(5)
     m.prepare(fopt)
-    for md in range(3):
-        m.als_mode_update(md, fopt)  # warm-up sweep
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for md in range(3):
-        m.als_mode_update(md, fopt)
-    ctx.synchronize()
-    w32 = time.perf_counter() - t0
+    w32, r32 = 0.0, []
+    for _ in range(1 + a.mixed_iters):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for md in range(3):
+            m.als_mode_update(md, fopt)
+        ctx.synchronize()
+        w32 += time.perf_counter() - t0
+        r32.append(m.evaluate(lam)["rmse_te"])
+    w32 /= 1 + a.mixed_iters
     res.update({"fp32_s_per_iter": w32, "fp32_nnz_per_s": tr.nnz / w32, "speedup_vs_fp32": w32 / wall,
-                "fp32_rmse_te_after_2": m.evaluate(lam)["rmse_te"]})
-    log(f"FP32 row path R=128: {w32:.3f} s/iter; mixed speed-up {w32 / wall:.2f}x")
+                "fp32_rmse_te_per_iter": [round(x, 5) for x in r32],
+                "rmse_te_gap_vs_fp32_path": [round(abs(x - y), 6) for x, y in zip(rm, r32)]})
+    log(f"FP32 row path R=128: {w32:.3f} s/iter; mixed speed-up {w32 / wall:.2f}x; rmse_te {r32}")
     return res
 
 
@@ -141,7 +148,7 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--batch-iters", type=int, default=10)
     ap.add_argument("--mixed-nnz", type=int, default=500_000_000)
-    ap.add_argument("--mixed-iters", type=int, default=3)
+    ap.add_argument("--mixed-iters", type=int, default=5)
     ap.add_argument("--mixed-sample", type=int, default=1000)
     a = ap.parse_args()
     if a.gpus != 1:

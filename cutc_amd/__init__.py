@@ -95,6 +95,19 @@ class Coo:
         t.vals[:] = vals
         return t
 
+    def sorted(self, keys):
+        """Stable lexicographic sort by the given modes (stensor.py's orders)."""
+        k = (C.c_uint32 * len(keys))(*keys)
+        out = C.POINTER(_lib.Coo)()
+        check(lib().ctc_coo_sort(self.c, k, len(keys), C.byref(out)))
+        return Coo(out)
+
+    def split(self, seed=0):
+        """Seeded 80/10/10 split -> (train, validation, test) (tvtensor.py)."""
+        tr, va, te = C.POINTER(_lib.Coo)(), C.POINTER(_lib.Coo)(), C.POINTER(_lib.Coo)()
+        check(lib().ctc_coo_split(self.c, int(seed), C.byref(tr), C.byref(va), C.byref(te)))
+        return Coo(tr), Coo(va), Coo(te)
+
     def write_binary(self, path, base=1):
         """SPLATT binary coordinate file (read back by Coo.read for *.bin)."""
         check(lib().ctc_coo_write_binary(self.c, path.encode(), int(base)))

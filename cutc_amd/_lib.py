@@ -159,6 +159,8 @@ SIGNATURES = {
     "ctc_coo_alloc": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_uint64), P(P(Coo))]),
     "ctc_coo_read": (C.c_int, [C.c_char_p, P(P(Coo))]),
     "ctc_coo_write_binary": (C.c_int, [P(Coo), C.c_char_p, C.c_uint32]),
+    "ctc_coo_sort": (C.c_int, [P(Coo), P(C.c_uint32), C.c_uint32, P(P(Coo))]),
+    "ctc_coo_split": (C.c_int, [P(Coo), C.c_uint64, P(P(Coo)), P(P(Coo)), P(P(Coo))]),
     "ctc_coo_write": (C.c_int, [P(Coo), C.c_char_p, C.c_uint32, C.c_int]),
     "ctc_coo_free": (None, [P(Coo)]),
     "ctc_synth_generate": (C.c_int, [P(SynthSpec), P(P(Coo)), P(P(Coo))]),

@@ -15,6 +15,8 @@
 //   --autotune (replaces the GA of software/autotuning)      --json (summary line)
 //   --dump-factors PREFIX (writes PREFIX<mode>.txt)
 //   --precision fp32|mixed (BF16-operand ALS)  --refine K   --sgd-update P  --sgd-hot-rows N
+//
+//   ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]  (data preparation)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -39,7 +41,51 @@ static int die(ctc_status s, const char *what) {
   return (int)s;
 }
 
+// `ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]`: the
+// reference's data preparation in one step.  Splits the tensor 80/10/10
+// (software/scripts/tvtensor.py) into <prefix>_tr, _v and _te, and writes the
+// training tensor also sorted by modes (1, 2) and (2, 0) as <prefix>_tr2 and
+// _tr3 (software/scripts/stensor.py), as .tns text with %.<D>f values
+// (default 1, as the scripts) or as .bin files.
+static int prep(int argc, char **argv) {
+  if (argc < 4) {
+    fprintf(stderr, "usage: ctc prep <tensor.tns|.bin> <out_prefix> [--seed S] [--decimals D] [--binary]\n");
+    return 1;
+  }
+  uint64_t seed = 0;
+  int decimals = 1;
+  bool binary = false;
+  for (int i = 4; i < argc; ++i) {
+    if (!strcmp(argv[i], "--seed") && i + 1 < argc) seed = strtoull(argv[++i], nullptr, 0);
+    else if (!strcmp(argv[i], "--decimals") && i + 1 < argc) decimals = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--binary")) binary = true;
+    else {
+      fprintf(stderr, "ctc prep: unknown argument %s\n", argv[i]);
+      return 1;
+    }
+  }
+  ctc_coo *t = nullptr, *tr = nullptr, *va = nullptr, *te = nullptr, *tr2 = nullptr, *tr3 = nullptr;
+  ctc_status s;
+  if ((s = ctc_coo_read(argv[2], &t)) != CTC_OK) return die(s, argv[2]);
+  if ((s = ctc_coo_split(t, seed, &tr, &va, &te)) != CTC_OK) return die(s, "split");
+  const uint32_t k12[2] = {1, 2}, k20[2] = {2, 0};
+  if ((s = ctc_coo_sort(tr, k12, 2, &tr2)) != CTC_OK || (s = ctc_coo_sort(tr, k20, 2, &tr3)) != CTC_OK)
+    return die(s, "sort");
+  const char *names[5] = {"_tr", "_v", "_te", "_tr2", "_tr3"};
+  ctc_coo *parts[5] = {tr, va, te, tr2, tr3};
+  for (int k = 0; k < 5; ++k) {
+    std::string path = std::string(argv[3]) + names[k] + (binary ? ".bin" : ".tns");
+    s = binary ? ctc_coo_write_binary(parts[k], path.c_str(), t->base[0])
+               : ctc_coo_write(parts[k], path.c_str(), t->base[0], decimals);
+    if (s != CTC_OK) return die(s, path.c_str());
+    printf("%s: %llu nonzeros\n", path.c_str(), (unsigned long long)parts[k]->nnz);
+  }
+  for (ctc_coo *c : {t, tr, va, te, tr2, tr3}) ctc_coo_free(c);
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 1 && !strcmp(argv[1], "prep")) return prep(argc, argv);
   std::vector<const char *> pos;
   int alg = CTC_SGD;  // SG/tc.c:22 default
   std::vector<std::pair<std::string, std::string>> kv;

@@ -721,3 +721,67 @@ extern "C" ctc_status ctc_synth_generate(const ctc_synth_spec *sp, ctc_coo **tra
   if (test) *test = te;
   return CTC_OK;
 }
+
+// -------------------------------------------------------------------------
+// data preparation (software/scripts/tvtensor.py, stensor.py)
+// -------------------------------------------------------------------------
+namespace {
+
+// gathers the nonzeros of t in the given order into a new tensor
+ctc_status gather(const ctc_coo *t, const uint32_t *order, uint64_t n, ctc_coo **out) {
+  CTC_TRY(ctc_coo_alloc(t->nmodes, n, t->dims, out));
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    (*out)->base[m] = t->base[m];
+    for (uint64_t p = 0; p < n; ++p) (*out)->ind[m][p] = t->ind[m][order[p]];
+  }
+  for (uint64_t p = 0; p < n; ++p) (*out)->vals[p] = t->vals[order[p]];
+  return CTC_OK;
+}
+
+// stable lexicographic order of the nonzeros by keys[0], keys[1], ...
+ctc_status sorted_order(const ctc_coo *t, const uint32_t *keys, uint32_t nkeys, std::vector<uint32_t> &order) {
+  order.resize(t->nnz);
+  std::iota(order.begin(), order.end(), 0u);
+  std::vector<uint32_t> tmp;
+  for (uint32_t k = nkeys; k-- > 0;) {  // least significant key first
+    if (keys[k] >= t->nmodes) return fail(CTC_ERR_INVALID_ARG, "sort key %u of a %u-mode tensor", keys[k], t->nmodes);
+    ctc::counting_sort(t->ind[keys[k]], t->dims[keys[k]], order, tmp);
+  }
+  return CTC_OK;
+}
+
+}  // namespace
+
+extern "C" ctc_status ctc_coo_sort(const ctc_coo *t, const uint32_t *keys, uint32_t nkeys, ctc_coo **out) {
+  if (!t || !out || (nkeys && !keys) || nkeys > CTC_MAX_MODES) return fail(CTC_ERR_INVALID_ARG, "ctc_coo_sort");
+  std::vector<uint32_t> order;
+  CTC_TRY(sorted_order(t, keys, nkeys, order));
+  return gather(t, order.data(), t->nnz, out);
+}
+
+extern "C" ctc_status ctc_coo_split(const ctc_coo *t, uint64_t seed, ctc_coo **train, ctc_coo **validation,
+                                    ctc_coo **test) {
+  if (!t || !train || !validation || !test || t->nmodes < 2) return fail(CTC_ERR_INVALID_ARG, "ctc_coo_split");
+  *train = *validation = *test = nullptr;
+  // nonzeros in (mode 0, mode 1) order (tvtensor.py:64), then one draw from
+  // 0..9 each: 0-7 train, 8 test, 9 validation (tvtensor.py:73-81)
+  const uint32_t keys[2] = {0, 1};
+  std::vector<uint32_t> order;
+  CTC_TRY(sorted_order(t, keys, 2, order));
+  std::vector<uint32_t> parts[3];
+  Xoshiro rng(seed ^ 0x5EED5917ull);
+  for (uint32_t p : order) {
+    const uint64_t d = ((rng.next() >> 32) * 10) >> 32;
+    parts[d < 8 ? 0 : (d == 8 ? 2 : 1)].push_back(p);
+  }
+  ctc_status s = gather(t, parts[0].data(), parts[0].size(), train);
+  if (s == CTC_OK) s = gather(t, parts[1].data(), parts[1].size(), validation);
+  if (s == CTC_OK) s = gather(t, parts[2].data(), parts[2].size(), test);
+  if (s != CTC_OK) {
+    ctc_coo_free(*train);
+    ctc_coo_free(*validation);
+    ctc_coo_free(*test);
+    *train = *validation = *test = nullptr;
+  }
+  return s;
+}

@@ -76,6 +76,16 @@ ctc_status ctc_coo_write(const ctc_coo *t, const char *path, uint32_t base, int 
  * files ending in .bin in this format (SG/io.c:162-189, 430-513, with the
  * narrow-index path that the reference cannot take fixed). */
 ctc_status ctc_coo_write_binary(const ctc_coo *t, const char *path, uint32_t base);
+/* Data preparation, replacing software/scripts/{tvtensor,stensor}.py:
+ * ctc_coo_sort — stable lexicographic sort by the given modes (first key most
+ * significant); stensor.py:36-37 writes the training tensor sorted by (1, 2)
+ * and (2, 0), tvtensor.py:64 sorts by (0, 1).
+ * ctc_coo_split — seeded 80/10/10 split of the nonzeros taken in (0, 1)
+ * order, one draw from 0..9 per nonzero: 0-7 train, 8 test, 9 validation
+ * (tvtensor.py:73-81, which draws from an unseeded generator). */
+ctc_status ctc_coo_sort(const ctc_coo *t, const uint32_t *keys, uint32_t nkeys, ctc_coo **out);
+ctc_status ctc_coo_split(const ctc_coo *t, uint64_t seed, ctc_coo **train, ctc_coo **validation,
+                         ctc_coo **test);
 void ctc_coo_free(ctc_coo *t);
 
 /* ------------------------------------------------------------------------ */

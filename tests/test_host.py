@@ -149,6 +149,56 @@ def struct_header(path):
         return struct.unpack("<iQQ", f.read(20))
 
 
+def test_coo_sort_is_stable_lexicographic():
+    rng = np.random.default_rng(3)
+    n = 5000
+    ind = [rng.integers(0, d, n) for d in (40, 30, 20)]
+    vals = np.arange(n, dtype=np.float64)  # records the input position
+    t = ct.Coo.from_arrays(ind, vals, (40, 30, 20))
+    for keys in ((1, 2), (2, 0), (0, 1), (2,)):
+        got_ind, got_vals = t.sorted(keys).arrays()
+        # numpy's lexsort is stable; its last key is the most significant
+        order = np.lexsort([ind[k] for k in reversed(keys)])
+        assert np.array_equal(got_vals, vals[order])
+        for m in range(3):
+            assert np.array_equal(got_ind[m], ind[m][order])
+
+
+def test_coo_split_fractions_cover_and_seed():
+    tr, _ = ct.synth((300, 200, 100), 100000, test_fraction=0.0, seed=4)
+    a, b, c = tr.split(seed=1)
+    n = tr.nnz
+    assert a.nnz + b.nnz + c.nnz == n
+    assert abs(a.nnz / n - 0.8) < 0.01 and abs(b.nnz / n - 0.1) < 0.01 and abs(c.nnz / n - 0.1) < 0.01
+    key = lambda t: set(zip(*[x.tolist() for x in t.arrays()[0]]))
+    assert key(a) | key(b) | key(c) == key(tr)
+    for part in (a, b, c):  # each part keeps the (mode 0, mode 1) order
+        i0, i1 = part.arrays()[0][:2]
+        assert np.all((np.diff(i0.astype(np.int64)) > 0) | ((np.diff(i0.astype(np.int64)) == 0) & (np.diff(i1.astype(np.int64)) >= 0)))
+    a2, _, _ = tr.split(seed=1)
+    a3, _, _ = tr.split(seed=2)
+    assert np.array_equal(a.arrays()[1], a2.arrays()[1])  # same seed, same split
+    same = a3.nnz == a.nnz and np.array_equal(a.arrays()[1], a3.arrays()[1])
+    assert not same  # another seed, another split
+
+
+def test_cli_prep_writes_reference_file_set(tmp_path):
+    import subprocess
+    src = str(tmp_path / "x.tns")
+    tr, _ = ct.synth((60, 50, 40), 3000, test_fraction=0.0, value_map=ct.VALUES_INT_1_5, seed=6)
+    tr.write(src, base=1, decimals=1)
+    cli = os.path.join(os.path.dirname(ct.__file__), "ctc")
+    r = subprocess.run([cli, "prep", src, str(tmp_path / "x"), "--seed", "3"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    parts = {k: ct.Coo.read(str(tmp_path / f"x_{k}.tns")) for k in ("tr", "v", "te", "tr2", "tr3")}
+    assert parts["tr"].nnz + parts["v"].nnz + parts["te"].nnz == 3000
+    assert parts["tr2"].nnz == parts["tr"].nnz == parts["tr3"].nnz
+    i = parts["tr2"].arrays()[0]
+    assert np.all(np.diff(i[1].astype(np.int64) * 100 + i[2]) >= 0)  # sorted by (1, 2)
+    with open(str(tmp_path / "x_tr.tns")) as f:
+        assert re.match(r"^\d+ \d+ \d+ \d\.\d$", f.readline().strip())  # %.1f values
+
+
 def test_synth_is_seeded_distinct_and_exact():
     a, at = ct.synth((200, 150, 30), 20000, zipf=(1.0, 1.0, 0.0), true_rank=4,
                      value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.05, seed=7)
