@@ -98,14 +98,16 @@ def mixed_leg(ct, ctx, a, log):
         rm.append(m.evaluate(lam)["rmse_te"])  # untimed
     wall /= a.mixed_iters
     ni, msi = ctx.kernel_stats("als_mixed_items")
+    nc, msc = ctx.kernel_stats("als_mixed_chunks")
     nh, msh = ctx.kernel_stats("als_mixed_heavy")
     nt, mst = ctx.kernel_stats("als_mixed_thin")
     res.update({"s_per_iter": wall, "nnz_per_s": tr.nnz / wall,
-                "items_kernel_ms_per_iter": msi / a.mixed_iters, "heavy_kernel_ms_per_iter": msh / a.mixed_iters,
-                "thin_kernel_ms_per_iter": mst / a.mixed_iters,
+                "rows_kernel_ms_per_iter": msi / a.mixed_iters, "chunks_kernel_ms_per_iter": msc / a.mixed_iters,
+                "heavy_kernel_ms_per_iter": msh / a.mixed_iters, "thin_kernel_ms_per_iter": mst / a.mixed_iters,
                 "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
-    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), items {msi / a.mixed_iters:.0f} ms, "
-        f"heavy {msh / a.mixed_iters:.0f} ms, thin {mst / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
+    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), long-row chunks {msc / a.mixed_iters:.0f} ms, "
+        f"rows {msi / a.mixed_iters:.0f} ms, long-row solves {msh / a.mixed_iters:.0f} ms, thin rows "
+        f"{mst / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
     # the plain FP32 path (FP64-accumulated Gram, same solves) from the same init
     fopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1)
     m.init_random(5)
@@ -289,9 +291,14 @@ def main():
             if kern == ct.ALS_ROWS:
                 n_i, ms_i = ctx.kernel_stats("als_rows_items")
                 n_h, ms_h = ctx.kernel_stats("als_rows_heavy")
-                r.update({"items_kernel_ms_per_mode": ms_i / max(n_i, 1), "heavy_kernel_ms_per_mode": ms_h / max(n_i, 1)})
-                log(f"ALS rows {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), items {ms_i / max(n_i, 1):.2f} ms"
-                    f" + heavy {ms_h / max(n_i, 1):.2f} ms per mode, rmse_te {ev['rmse_te']:.4f}")
+                n_c, ms_c = ctx.kernel_stats("als_rows_chunks")
+                n_t, ms_t = ctx.kernel_stats("als_rows_thin")
+                nm = 3 * a.als_epochs
+                r.update({"rows_kernel_ms_per_mode": ms_i / nm, "chunks_kernel_ms_per_mode": ms_c / nm,
+                          "heavy_kernel_ms_per_mode": ms_h / nm, "thin_kernel_ms_per_mode": ms_t / nm})
+                log(f"ALS rows {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), per mode: chunks "
+                    f"{ms_c / nm:.2f} ms, rows {ms_i / nm:.2f} ms, long-row solves {ms_h / nm:.2f} ms, thin rows "
+                    f"{ms_t / nm:.2f} ms; rmse_te {ev['rmse_te']:.4f}")
             else:
                 n_g, ms_g = ctx.kernel_stats("als_gram")
                 n_s, ms_s = ctx.kernel_stats("als_solve")

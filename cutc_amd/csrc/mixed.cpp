@@ -128,8 +128,18 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.refine = o->refine > 0 ? o->refine : 1;
   a.flag = m->dflag;
   CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
-  {
+  // chunks of long rows and whole rows are the same kernel, launched apart so
+  // that their times are separate
+  const uint64_t nchunk = d.mix_nparts, nall = a.nitems;
+  if (nchunk) {
+    Prof p(m->ctx, bf16 ? "als_mixed_chunks" : "als_rows_chunks");
+    a.nitems = nchunk;
+    CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
+  }
+  if (nall > nchunk) {
     Prof p(m->ctx, bf16 ? "als_mixed_items" : "als_rows_items");
+    a.items = d.mix_items + nchunk;
+    a.nitems = nall - nchunk;
     CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
   }
   if (a.nheavy) {
