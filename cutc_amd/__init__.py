@@ -245,7 +245,7 @@ class Options:
     tolerance: float = 1e-4
     max_bad_epochs: int = 3
     tile_len: int = None
-    block: int = 256
+    block: int = None
     seed: int = 0
     mode_order: int = ORDER_FIXED
     precision: int = PREC_FP32

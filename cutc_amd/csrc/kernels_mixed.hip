@@ -87,18 +87,51 @@ __device__ inline void gram_chunk(const MixedArgs &a, uint32_t p0, uint32_t p1, 
   constexpr int NW = R / 32, FG8 = R / 8;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int fg = tid % FG8, qp = tid / FG8;  // features 8fg..8fg+7, nonzeros 2qp, 2qp+1
-  for (uint32_t base = p0; base < p1; base += 32) {
-    uint32_t hv[8][2];
+  // software pipeline: ids two steps ahead, factor rows one step ahead, so
+  // the gathers overlap the staging and MFMA work of the current step
+  uint32_t id_n[2][N - 1];
+  float x_n[2], x_c[2];
+  uint4 u_n[2][N - 1], u_c[2][N - 1];
+  auto load_ids = [&](uint32_t base) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t p = base + 2 * qp + h;
-      if (p < p1) {
+      const bool ok = p < p1;
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) id_n[h][s] = ok ? a.oidx[s][p] : 0u;
+      x_n[h] = ok ? a.vals[p] : 0.f;
+    }
+  };
+  auto load_rows = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) u_n[h][s] = *(const uint4 *)(a.fb[s] + (size_t)id_n[h][s] * R + 8 * fg);
+  };
+  load_ids(p0);
+  load_rows();
+  float x_r[2] = {x_n[0], x_n[1]};
+  load_ids(p0 + 32);
+  for (uint32_t base = p0; base < p1; base += 32) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      x_c[h] = x_r[h];
+      x_r[h] = x_n[h];
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) u_c[h][s] = u_n[h][s];
+    }
+    load_rows();           // rows of step base + 32
+    load_ids(base + 64);   // ids of step base + 64
+    uint32_t hv[8][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (base + 2 * qp + h < p1) {
         float hf[8];
 #pragma unroll
         for (int f = 0; f < 8; ++f) hf[f] = 1.f;
 #pragma unroll
         for (int s = 0; s < N - 1; ++s) {
-          const uint4 u = *(const uint4 *)(a.fb[s] + (size_t)a.oidx[s][p] * R + 8 * fg);
+          const uint4 u = u_c[h][s];
           hf[0] *= bf_lo(u.x);
           hf[1] *= bf_hi(u.x);
           hf[2] *= bf_lo(u.y);
@@ -108,11 +141,10 @@ __device__ inline void gram_chunk(const MixedArgs &a, uint32_t p0, uint32_t p1, 
           hf[6] *= bf_lo(u.w);
           hf[7] *= bf_hi(u.w);
         }
-        const float x = a.vals[p];
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
           hv[f][h] = f2bf(hf[f]);
-          mp[f] += x * __uint_as_float(hv[f][h] << 16);
+          mp[f] += x_c[h] * __uint_as_float(hv[f][h] << 16);
         }
       } else {
 #pragma unroll
@@ -153,7 +185,46 @@ __device__ inline void gram_f64(const MixedArgs &a, uint32_t p0, uint32_t p1, f6
   constexpr int NTILE = MixedSmem<R>::NTILE, TPW = MixedSmem<R>::TPW;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int fg = tid % FG8, qp = tid / FG8;
+  // software pipeline as in gram_chunk: ids two steps ahead, rows one ahead
+  uint32_t id_n[2][N - 1];
+  float x_n[2], x_r[2];
+  float4 u_n[2][N - 1][2], u_c[2][N - 1][2];
+  auto load_ids = [&](uint32_t base) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t p = base + 2 * qp + h;
+      const bool ok = p < p1;
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) id_n[h][s] = ok ? a.oidx[s][p] : 0u;
+      x_n[h] = ok ? a.vals[p] : 0.f;
+    }
+  };
+  auto load_rows = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) {
+        const float *row = a.fr[s] + (size_t)id_n[h][s] * R + 8 * fg;
+        u_n[h][s][0] = *(const float4 *)row;
+        u_n[h][s][1] = *(const float4 *)(row + 4);
+      }
+  };
+  load_ids(p0);
+  load_rows();
+  x_r[0] = x_n[0];
+  x_r[1] = x_n[1];
+  load_ids(p0 + 32);
   for (uint32_t base = p0; base < p1; base += 32) {
+    float x_c[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      x_c[h] = x_r[h];
+      x_r[h] = x_n[h];
+#pragma unroll
+      for (int s = 0; s < N - 1; ++s) u_c[h][s][0] = u_n[h][s][0], u_c[h][s][1] = u_n[h][s][1];
+    }
+    load_rows();           // rows of step base + 32
+    load_ids(base + 64);   // ids of step base + 64
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const uint32_t p = base + 2 * qp + h;
@@ -163,8 +234,7 @@ __device__ inline void gram_f64(const MixedArgs &a, uint32_t p0, uint32_t p1, f6
         for (int f = 0; f < 8; ++f) hd[f] = 1.0;
 #pragma unroll
         for (int s = 0; s < N - 1; ++s) {
-          const float *row = a.fr[s] + (size_t)a.oidx[s][p] * R + 8 * fg;
-          const float4 u0 = *(const float4 *)row, u1 = *(const float4 *)(row + 4);
+          const float4 u0 = u_c[h][s][0], u1 = u_c[h][s][1];
           hd[0] *= u0.x;
           hd[1] *= u0.y;
           hd[2] *= u0.z;
@@ -174,7 +244,7 @@ __device__ inline void gram_f64(const MixedArgs &a, uint32_t p0, uint32_t p1, f6
           hd[6] *= u1.z;
           hd[7] *= u1.w;
         }
-        const double x = a.vals[p];
+        const double x = x_c[h];
 #pragma unroll
         for (int f = 0; f < 8; ++f) mp[f] += x * hd[f];
       } else {
@@ -650,27 +720,40 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
   const int l = threadIdx.x;
   const int n = (int)(it.z - it.y);
   const int nt = n <= 16 ? 16 : 32;  // system size handled (zero-padded)
-  // 1. stage h for the row's nonzeros (rows >= n zero) and x
-  for (int idx = l; idx < nt * R; idx += 64) {
-    const int p = idx / R, f = idx % R;
-    typename TS::HT h = 0;
-    if (p < n) {
-      const uint32_t q = it.y + p;
-      if (P == kPrecBF16) {
-        float hf = 1.f;
+  // 1. stage h for the row's nonzeros (rows >= n zero) and x.  Lane p < n
+  // loads nonzero p's ids once; the gathers of all nonzeros are then issued
+  // back to back (independent loads, one memory latency for the row).
+  constexpr int FPL = R / 64 > 0 ? R / 64 : 1;  // features per lane
+  uint32_t myid[N - 1];
 #pragma unroll
-        for (int s = 0; s < N - 1; ++s) hf *= __uint_as_float((uint32_t)a.fb[s][(size_t)a.oidx[s][q] * R + f] << 16);
-        h = __uint_as_float((uint32_t)f2bf(hf) << 16);
-      } else {
-        double hd = 1.0;
-#pragma unroll
-        for (int s = 0; s < N - 1; ++s) hd *= (double)a.fr[s][(size_t)a.oidx[s][q] * R + f];
-        h = hd;
-      }
-    }
-    sm.H[p * TS::HS + f] = h;
-  }
+  for (int s = 0; s < N - 1; ++s) myid[s] = l < n ? a.oidx[s][it.y + l] : 0u;
   if (l < kThinRow) sm.x[l] = l < n ? (double)a.vals[it.y + l] : 0.0;
+#pragma unroll 8
+  for (int p = 0; p < nt; ++p) {
+    uint32_t id[N - 1];
+#pragma unroll
+    for (int s = 0; s < N - 1; ++s) id[s] = (uint32_t)__shfl((int)myid[s], p, 64);
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+      const int f = l + 64 * j;
+      if (f >= R) continue;
+      typename TS::HT h = 0;
+      if (p < n) {
+        if (P == kPrecBF16) {
+          float hf = 1.f;
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) hf *= __uint_as_float((uint32_t)a.fb[s][(size_t)id[s] * R + f] << 16);
+          h = __uint_as_float((uint32_t)f2bf(hf) << 16);
+        } else {
+          double hd = 1.0;
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) hd *= (double)a.fr[s][(size_t)id[s] * R + f];
+          h = hd;
+        }
+      }
+      sm.H[p * TS::HS + f] = h;
+    }
+  }
   if (l == 0) sm.bad = 0;
   wave_sync();
   // 2. K = H H^T (lower 16x16 tiles) on v_mfma_f64_16x16x4f64
@@ -730,14 +813,15 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
   // 4. z = (L D L^T)^-1 x: lane i < nt holds x_i
   double xi = l < nt ? sm.x[l] : 0.0;
   const double di = l < nt ? sm.K[l * TS::KS + l] : 1.0;
-  for (int j = 0; j < nt - 1; ++j) {  // forward, unit lower L
-    const double yj = __shfl(xi, j, 64);
-    if (l > j && l < nt) xi = __builtin_fma(-sm.K[l * TS::KS + j] * rcp_f64(sm.K[j * TS::KS + j]), yj, xi);
+  const double rdi = rcp_f64(di > 0.0 ? di : 1.0);
+  for (int j = 0; j < nt - 1; ++j) {  // forward, unit lower L: l_ij = K[i][j] / d_j
+    const double yj = __shfl(xi * rdi, j, 64);  // y_j / d_j
+    if (l > j && l < nt) xi = __builtin_fma(-sm.K[l * TS::KS + j], yj, xi);
   }
-  xi *= rcp_f64(di);
-  for (int j = nt - 1; j > 0; --j) {  // backward, L^T
+  xi *= rdi;
+  for (int j = nt - 1; j > 0; --j) {  // backward, L^T: l_ji = K[j][i] / d_i
     const double zj = __shfl(xi, j, 64);
-    if (l < j) xi = __builtin_fma(-sm.K[j * TS::KS + l] * rcp_f64(di), zj, xi);
+    if (l < j) xi = __builtin_fma(-sm.K[j * TS::KS + l] * rdi, zj, xi);
   }
   wave_sync();
   if (l < kThinRow) sm.x[l] = l < nt ? xi : 0.0;

@@ -309,8 +309,10 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->max_epochs = algorithm == CTC_SGD ? 999 : 1000;                   // :22
   o->tolerance = 1e-4;                                                 // :36
   o->max_bad_epochs = 3;                                               // :32
-  o->tile_len = algorithm == CTC_SGD ? 4096 : algorithm == CTC_ALS ? 1024 : 256;
-  o->block = 256;
+  // work-unit shapes measured on the 100M-nonzero benchmark tensor (the
+  // reference's GA searched the same two knobs, GA/fitness.c:67-79)
+  o->tile_len = algorithm == CTC_SGD ? 2048 : algorithm == CTC_ALS ? 1024 : 256;
+  o->block = algorithm == CTC_SGD ? 64 : 256;
   o->seed = 0;
   o->mode_order = CTC_ORDER_FIXED;
   o->precision = CTC_PREC_FP32;
