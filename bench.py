@@ -44,7 +44,9 @@ def mixed_leg(ct, ctx, a, log):
     """ALS at R=128 in the BF16-operand path on a 5M x 5M x 1000 tensor with
     500M nonzeros (modes 0-1 Zipf(1.1)), values from a rank-32 CP model mapped
     to [1,5] plus N(0,0.2), 90/10 split, lambda 0.05.  A sample of rows of the
-    first mode update is checked against the FP64 oracle from the same inputs."""
+    first mode update is checked against an FP64 numpy solve of the same
+    normal equations (Gram of the other modes' rows + lambda I) from the same
+    inputs."""
     import numpy as np
     R, lam = 128, 0.05
     dims = (5_000_000, 5_000_000, 1000)
@@ -61,17 +63,22 @@ def mixed_leg(ct, ctx, a, log):
     res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam}
     # sampled FP64 check of the first mode-0 update
     if not a.no_cpu:
-        import oracle as O
+        t0 = time.time()
         F = [m.factor(md) for md in range(3)]
-        ind, vals = tr.arrays()
-        oc = O.Coo(ind, vals, dims)
-        del ind, vals
+        ind, vals = tr.arrays()  # the training tensor is in mode-0 order
         rng = np.random.default_rng(0)
         rows = np.sort(rng.choice(dims[0], a.mixed_sample, replace=False)).astype(np.uint32)
-        t0 = time.time()
-        ref = O.als_rows(oc, 0, F, lam, rows, nthreads=os.cpu_count() or 1)
-        log(f"mixed: FP64 oracle on {len(rows)} sampled rows in {time.time() - t0:.1f}s")
-        del oc, F
+        lo = np.searchsorted(ind[0], rows, "left")
+        hi = np.searchsorted(ind[0], rows, "right")
+        ref = np.zeros((len(rows), R))
+        for k in range(len(rows)):
+            if hi[k] == lo[k]:
+                continue  # a row without nonzeros stays 0
+            q = slice(lo[k], hi[k])
+            H = F[1][ind[1][q]].astype(np.float64) * F[2][ind[2][q]].astype(np.float64)
+            ref[k] = np.linalg.solve(H.T @ H + lam * np.eye(R), H.T @ vals[q])
+        log(f"mixed: FP64 solve of {len(rows)} sampled rows in {time.time() - t0:.1f}s")
+        del ind, vals, F
     ctx.reset_stats()
     m.als_mode_update(0, opt)
     if not a.no_cpu:
@@ -129,6 +136,89 @@ def mixed_leg(ct, ctx, a, log):
     return res
 
 
+def adjoint_leg(ct, ctx, a, log):
+    """The loss adjoint behind torch.autograd on the 4-way tensor 1M x 500K x
+    1000 x 24 with 140M nonzeros (modes 0-1 Zipf(1.0)), values from a rank-16
+    CP model plus N(0, 0.2), R = 32: full-gradient steps with torch's Adam
+    through `completion_loss` (the adjoint kernel returns the loss and every
+    factor gradient in one walk per mode), and a central finite-difference
+    check (eps = 1e-3, the loss in FP64 numpy) of the gradient on a
+    1000-nonzero subsample."""
+    import numpy as np
+    import torch
+    from cutc_amd.autograd import completion_loss
+    R, lam = 32, 0.05
+    dims = (1_000_000, 500_000, 1000, 24)
+    t0 = time.time()
+    tr, te = ct.synth(dims, a.adjoint_nnz, zipf=(1.0, 1.0, 0.0, 0.0), true_rank=16, value_map=ct.VALUES_RAW,
+                      noise_std=0.2, test_fraction=0.01, seed=0x5EED0003)
+    log(f"adjoint: tensor train {tr.nnz} test {te.nnz} in {time.time() - t0:.1f}s")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    F = [((torch.rand(d, R, device="cuda", generator=g) * 2 - 1) * 0.5).requires_grad_() for d in dims]
+    opt = torch.optim.Adam(F, lr=a.adjoint_lr)
+    loss = completion_loss(m, F, lam)  # first call builds the per-mode CISS copies
+    ctx.reset_stats()
+    losses, times = [], []
+    for _ in range(a.adjoint_steps):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        opt.zero_grad()
+        loss = completion_loss(m, F, lam)
+        loss.backward()
+        opt.step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        losses.append(float(loss.detach()))
+    n_a, ms_a = ctx.kernel_stats("adjoint")
+    per_mode_ms = ms_a / max(n_a, 1)
+    alg = 16 + 12 * R  # per nonzero per mode: three indices, the value, three gathered rows
+    res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam, "optimizer": f"Adam lr {a.adjoint_lr}",
+           "step_ms": 1e3 * float(np.median(times)), "adjoint_ms_per_mode": per_mode_ms,
+           "adjoint_alg_gbs": alg * tr.nnz / (per_mode_ms * 1e-3) / 1e9,
+           "nnz_per_s": tr.nnz / float(np.median(times)), "loss_per_step": [round(x, 2) for x in losses]}
+    log(f"adjoint: step {res['step_ms']:.1f} ms, adjoint {per_mode_ms:.2f} ms per mode "
+        f"({res['adjoint_alg_gbs']:.0f} GB/s algorithmic), loss {losses[0]:.4g} -> {losses[-1]:.4g}")
+    # gradient of the loss on a 1000-nonzero subsample against central finite
+    # differences of the same loss evaluated in FP64 with numpy
+    ind, vals = tr.arrays()
+    rng = np.random.default_rng(0)
+    pick = np.sort(rng.choice(tr.nnz, 1000, replace=False))
+    sub_ind = [np.ascontiguousarray(x[pick]) for x in ind]
+    sub_vals = vals[pick]
+    del ind, vals
+    ms = ct.Model(ctx, ct.Coo.from_arrays(sub_ind, sub_vals, dims), None, None, rank=R)
+    Fd = [f.detach().clone().requires_grad_() for f in F]
+    completion_loss(ms, Fd, lam).backward()
+    # the subsample's rows of each factor in FP64; the loss's data term only
+    # reads these
+    H = [f.detach()[torch.from_numpy(ix.astype(np.int64)).cuda()].double().cpu().numpy() for f, ix in zip(Fd, sub_ind)]
+
+    def data_loss(H):
+        return 0.5 * float(np.sum((sub_vals - np.prod(np.stack(H), axis=0).sum(axis=1)) ** 2))
+
+    eps, errs = 1e-3, []
+    for k in range(48):
+        md, q, f = k % 4, int(rng.integers(1000)), int(rng.integers(R))
+        row = int(sub_ind[md][q])
+        same = sub_ind[md] == row  # every subsample nonzero that reads this row
+        old = H[md][q, f]
+        H[md][same, f] = old + eps
+        lp = data_loss(H)
+        H[md][same, f] = old - eps
+        lm = data_loss(H)
+        H[md][same, f] = old
+        fdd = (lp - lm) / (2 * eps)  # exact up to rounding: the loss is quadratic in one entry
+        fd = fdd + lam * old  # + the per-row L2 term's derivative
+        gg = float(Fd[md].grad[row, f])
+        errs.append(abs(gg - fd) / (abs(fdd) + abs(lam * old) + 1e-12))
+    res["fd_check_entries"] = len(errs)
+    res["fd_max_rel_err"] = float(max(errs))
+    log(f"adjoint: central differences (eps 1e-3, FP64) on {len(errs)} gradient entries of a 1000-nonzero "
+        f"subsample, max rel err {max(errs):.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +229,7 @@ def main():
     ap.add_argument("--ccd-epochs", type=int, default=3)
     ap.add_argument("--als-epochs", type=int, default=3)
     ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default")
-    ap.add_argument("--sgd-update", default="auto", choices=["auto", "atomic", "racy", "next_racy"],
+    ap.add_argument("--sgd-update", default="auto", choices=["auto", "atomic", "racy", "next_racy", "hot_atomic"],
                     help="how concurrent walkers combine updates of shared rows (see ctc_sgd_update)")
     ap.add_argument("--sgd-hot-rows", type=int, default=0,
                     help="auto policy: rows with more nonzeros than this are updated racily (0 = default)")
@@ -152,6 +242,9 @@ def main():
     ap.add_argument("--mixed-nnz", type=int, default=500_000_000)
     ap.add_argument("--mixed-iters", type=int, default=5)
     ap.add_argument("--mixed-sample", type=int, default=1000)
+    ap.add_argument("--adjoint-nnz", type=int, default=140_000_000)
+    ap.add_argument("--adjoint-steps", type=int, default=10)
+    ap.add_argument("--adjoint-lr", type=float, default=0.01)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -179,7 +272,7 @@ def main():
     F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
     model.set_factors(F0)
     policy = {"auto": ct.SGD_AUTO, "atomic": ct.SGD_ATOMIC, "racy": ct.SGD_RACY,
-              "next_racy": ct.SGD_NEXT_RACY}[a.sgd_update]
+              "next_racy": ct.SGD_NEXT_RACY, "hot_atomic": ct.SGD_HOT_ATOMIC}[a.sgd_update]
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3,
                      lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0,
                      tile_len=a.tile_len or None, sgd_update=policy, sgd_hot_rows=a.sgd_hot_rows)
@@ -352,6 +445,10 @@ def main():
             f"{ms_eager:.1f} ms (direct); {bn * a.batch_iters / (ms_graph * 1e-3) / 1e9:.3f} Gnnz/s; "
             f"median rmse_te {np.median(rte):.4f}")
         del batch, probs, Fb
+
+    # ----------------------------------------- adjoint (4-way, autograd)
+    if "adjoint" in algs:
+        out_algs["adjoint"] = adjoint_leg(ct, ctx, a, log)
 
     # ------------------------------------------ mixed-precision ALS, R = 128
     if "mixed" in algs:

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALS, ALS_AUTO, ALS_ROWS, ALS_SPANS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
-                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_NEXT_RACY, SGD_RACY,
+                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_HOT_ATOMIC, SGD_NEXT_RACY, SGD_RACY,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
 __all__ = ["Coo", "Ciss", "Context", "Model", "Batch", "Options", "synth", "tt_read", "ciss_alloc",

@@ -32,7 +32,7 @@ static void usage() {
           "           [--max-epochs E] [--tol T] [--max-bad-epochs B] [--tile-len N] [--block B]\n"
           "           [--seed S] [--mode-order fixed|rotate] [--lr-schedule bold|exp|const]\n"
           "           [--lr-decay G] [--inner I] [--precision fp32|mixed] [--refine K]\n"
-          "           [--sgd-update auto|atomic|racy|next-racy] [--sgd-hot-rows N]\n"
+          "           [--sgd-update auto|atomic|racy|next-racy|hot-atomic] [--sgd-hot-rows N]\n"
           "           [--autotune] [--json] [--dump-factors PREFIX]\n");
 }
 
@@ -145,6 +145,7 @@ int main(int argc, char **argv) {
     else if (k == "sgd-update")
       o.sgd_update = strcmp(v, "atomic") == 0 ? CTC_SGD_ATOMIC : strcmp(v, "racy") == 0 ? CTC_SGD_RACY
                      : strcmp(v, "next-racy") == 0                                       ? CTC_SGD_NEXT_RACY
+                     : strcmp(v, "hot-atomic") == 0                                      ? CTC_SGD_HOT_ATOMIC
                                                                                          : CTC_SGD_AUTO;
     else if (k == "sgd-hot-rows") o.sgd_hot_rows = (uint32_t)strtoul(v, nullptr, 0);
     else if (k == "dump-factors") dump = v;

@@ -174,15 +174,18 @@ ctc_status als_mode_impl(ctc_model *m, uint32_t mode, const ctc_options *o);
 // reads and clears the model's not-positive-definite flag (synchronises)
 ctc_status check_not_spd(ctc_model *m);
 // profiled launch bracketing
-void prof_begin(ctc_context *c, const char *name, hipEvent_t *a, hipEvent_t *b);
-void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b);
+void prof_begin(ctc_context *c, const char *name, hipEvent_t *a, hipEvent_t *b, hipStream_t s);
+void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b, hipStream_t s);
 ctc_status prof_flush(ctc_context *c);
 // RAII bracket of one profiled launch (no-op unless profiling is enabled)
 struct Prof {
   ctc_context *c;
   const char *name;
   hipEvent_t a, b;
-  Prof(ctc_context *c_, const char *n) : c(c_), name(n) { prof_begin(c, n, &a, &b); }
-  ~Prof() { prof_end(c, name, a, b); }
+  hipStream_t s;  // the stream the timed launches go to (null: the context's)
+  Prof(ctc_context *c_, const char *n, hipStream_t s_ = nullptr) : c(c_), name(n), s(s_) {
+    prof_begin(c, n, &a, &b, s);
+  }
+  ~Prof() { prof_end(c, name, a, b, s); }
 };
 }  // namespace ctc

@@ -40,9 +40,21 @@ namespace ctc {
 //   P_HOT      per row: P_RACY if the row's bit is set in the mode's hot-row
 //              bitmap (rows with more updates per sweep than a threshold,
 //              whose atomics would serialise on one L2 channel), else P_ATOMIC.
+//   P_HOTRUN   per row: P_RACY for a hot row, else P_RUN with runs cut
+//              after kRunCap nonzeros (the row is then added and re-read, so
+//              a walker's copy is never more than kRunCap own updates away
+//              from the shared row: uncut runs of thousands of nonzeros on a
+//              row many walkers share add up deltas taken from one stale copy
+//              and diverge).  In mode-0 order the nonzeros of one mode-0 row
+//              come sorted by the next mode, so runs of one next-mode row are
+//              common and each run costs one atomic row update instead of one
+//              per nonzero.
+// The float atomics are the kernel's limit (the L2 completes about one
+// 4-byte atomic per channel per clock), so fewer atomic rows is what pays.
 // Walker w processes span (w * stride) mod nspans (stride 1 = natural order).
 // ==========================================================================
-enum { P_RACY = 0, P_ATOMIC = 1, P_RUN = 2, P_HOT = 3 };
+enum { P_RACY = 0, P_ATOMIC = 1, P_RUN = 2, P_HOT = 3, P_HOTRUN = 4 };
+constexpr int kRunCap = 8;
 
 template <int R, int N, int PA, int PB, int PC>
 __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows hot, uint32_t span,
@@ -77,10 +89,17 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
   uint32_t slot = 0;
 
   auto pol = [](int s) { return s == 0 ? PA : (s == 1 ? PB : PC); };
+  auto bitmapped = [&](int s) { return pol(s) == P_HOT || pol(s) == P_HOTRUN; };
   auto flush_run = [&](int s) {
 #pragma unroll
     for (int q2 = 0; q2 < CPL; ++q2) atomic_add_f32(Fm[s] + (size_t)cur[s] * R + wl + q2 * W, del[s][q2]);
   };
+  // the cached row of slot s is a P_RUN row whose delta is not yet added, and
+  // the nonzeros its run has taken so far
+  bool pend[N];
+  int runlen[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) pend[s] = false, runlen[s] = 0;
 
   for (uint32_t ch = 0; ch < span; ch += W) {
     const uint64_t el = s0 + ch + wl;
@@ -98,12 +117,19 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
       for (int s = 0; s < N - 1; ++s) il[s] = 0;
     }
     const uint64_t hm = __ballot(vl && hl);
+    // hot-row bits of this lane's nonzero, one bit per slot, looked up once
+    // per chunk rather than on every nonzero's dependency chain
+    int hotl = 0;
+#pragma unroll
+    for (int s = 1; s < N; ++s)
+      if (bitmapped(s) && vl) hotl |= (int)((hot.bits[s][il[s - 1] >> 5] >> (il[s - 1] & 31)) & 1u) << s;
     for (int q = 0; q < W; ++q) {
       const int src = sub * W + q;
       uint32_t idx[N];
 #pragma unroll
       for (int s = 0; s < N - 1; ++s) idx[1 + s] = (uint32_t)__shfl((int)il[s], src, 64);
       const float x = __shfl(xl, src, 64);
+      const int hotq = __shfl(hotl, src, 64);
       const uint64_t e = s0 + ch + q;
       if (e < s1) {
         const bool head = (hm >> src) & 1ull;
@@ -117,14 +143,18 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
 #pragma unroll
         for (int s = 0; s < N; ++s) {
           int P = pol(s);
-          if (P == P_HOT) P = ((hot.bits[s][idx[s] >> 5] >> (idx[s] & 31)) & 1u) ? P_RACY : P_ATOMIC;
+          if (P == P_HOT) P = ((hotq >> s) & 1) ? P_RACY : P_ATOMIC;
+          if (P == P_HOTRUN) P = ((hotq >> s) & 1) ? P_RACY : P_RUN;
           pr[s] = P;
         }
 #pragma unroll
         for (int s = 0; s < N; ++s) {
           const int P = pr[s];
-          if (P == P_ATOMIC || idx[s] != cur[s]) {
-            if (P == P_RUN && cur[s] != NONE && idx[s] != cur[s]) flush_run(s);
+          const bool capped = pol(s) == P_HOTRUN && pend[s] && runlen[s] >= kRunCap;
+          if (P == P_ATOMIC || idx[s] != cur[s] || capped) {
+            if (pend[s]) flush_run(s);  // the cached row's run ends here
+            pend[s] = P == P_RUN;
+            runlen[s] = 0;
             // an atomic row is re-read at every nonzero: forget it as the cached row
             cur[s] = P == P_ATOMIC ? NONE : idx[s];
 #pragma unroll
@@ -171,12 +201,14 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
             }
           }
         }
+#pragma unroll
+        for (int s = 0; s < N; ++s) runlen[s] += 1;
       }
     }
   }
 #pragma unroll
   for (int s = 0; s < N; ++s)
-    if (pol(s) == P_RUN && cur[s] != NONE) flush_run(s);
+    if (pend[s]) flush_run(s);
 }
 
 // ==========================================================================
@@ -749,18 +781,19 @@ static inline unsigned grid_for(uint64_t waves, unsigned block) {
   return (unsigned)((waves + wpb - 1) / wpb);
 }
 
-// policy code = PA * 16 + PB * 4 + PC
+// policy code = PA * 64 + PB * 8 + PC
 template <int N>
 static hipError_t launch_sgd_n(hipStream_t st, const CissArgs &c, const FactorArgs &f, const HotRows &hot, int R,
                                uint32_t span,
                                uint64_t nspans, uint64_t stride, int code, float step, float lam, unsigned grid,
                                unsigned block) {
 #define CTC_SGD_P(PA, PB, PC) \
-  case PA * 16 + PB * 4 + PC:  \
+  case PA * 64 + PB * 8 + PC:  \
     CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd<RR, N, PA, PB, PC>), dim3(grid), dim3(block), 0, st, c, f, hot, span, nspans, stride, step, lam)); \
     break;
   switch (code) {
     CTC_SGD_P(P_RACY, P_HOT, P_HOT)
+    CTC_SGD_P(P_RACY, P_HOTRUN, P_HOTRUN)
     CTC_SGD_P(P_RACY, P_ATOMIC, P_ATOMIC)
     CTC_SGD_P(P_RACY, P_RACY, P_RACY)
     CTC_SGD_P(P_RACY, P_RACY, P_ATOMIC)

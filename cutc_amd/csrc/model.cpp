@@ -50,7 +50,7 @@ double median(std::vector<double> v) {
 namespace ctc {
 
 // ------------------------------------------------------------- profiling --
-void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b) {
+void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b, hipStream_t s) {
   *a = *b = nullptr;
   if (!c->profiling) return;
   if (c->event_pool.size() < 2) {
@@ -63,19 +63,19 @@ void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b) {
   c->event_pool.pop_back();
   *b = c->event_pool.back();
   c->event_pool.pop_back();
-  (void)hipEventRecord(*a, c->stream);
+  (void)hipEventRecord(*a, s ? s : c->stream);
 }
 
-void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b) {
+void prof_end(ctc_context *c, const char *name, hipEvent_t a, hipEvent_t b, hipStream_t s) {
   if (!c->profiling || !a) return;
-  (void)hipEventRecord(b, c->stream);
+  (void)hipEventRecord(b, s ? s : c->stream);
   c->pending.push_back({name, a, b});
 }
 
 ctc_status prof_flush(ctc_context *c) {
   if (c->pending.empty()) return CTC_OK;
-  CTC_HIP(hipStreamSynchronize(c->stream));
   for (auto &p : c->pending) {
+    CTC_HIP(hipEventSynchronize(p.b));
     float ms = 0.f;
     CTC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
     auto &s = c->stats[p.name];
@@ -582,12 +582,13 @@ static uint64_t spread_stride(uint64_t n, bool on) {
 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   CTC_TRY(ensure_ciss(m, 0));
-  // policy codes of k_sgd: PA * 16 + PB * 4 + PC with 0 racy, 1 atomic, 3 hot-row bitmap
-  static const int code_of[4] = {0 * 16 + 3 * 4 + 3, 0 * 16 + 1 * 4 + 1, 0, 0 * 16 + 0 * 4 + 1};
-  if (o->sgd_update < 0 || o->sgd_update > 3) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
+  // policy codes of k_sgd: PA * 64 + PB * 8 + PC with 0 racy, 1 atomic, 3 hot
+  // bitmap (racy / atomic), 4 hot bitmap (racy / run-accumulated atomic)
+  static const int code_of[5] = {0 * 64 + 4 * 8 + 4, 0 * 64 + 1 * 8 + 1, 0, 0 * 64 + 0 * 8 + 1, 0 * 64 + 3 * 8 + 3};
+  if (o->sgd_update < 0 || o->sgd_update > 4) return fail(CTC_ERR_INVALID_ARG, "sgd_update %d", o->sgd_update);
   const int code = code_of[o->sgd_update];
   HotRows hot{};
-  if (o->sgd_update == CTC_SGD_AUTO) {
+  if (o->sgd_update == CTC_SGD_AUTO || o->sgd_update == CTC_SGD_HOT_ATOMIC) {
     const uint64_t thr = o->sgd_hot_rows ? o->sgd_hot_rows : 262144;
     if (thr != m->hot_threshold || !m->hotbits[0]) {
       const ctc_coo *t = m->train.get();
@@ -731,7 +732,7 @@ extern "C" ctc_status ctc_grad(ctc_model *m, const float *const *factors, float 
     CTC_HIP(launch_fill(st, grads[md], m->dims[md] * R, 0.f));
     unsigned grid = 0;
     {
-      Prof p(m->ctx, "adjoint");
+      Prof p(m->ctx, "adjoint", st);
       CTC_HIP(launch_mttkrp(st, ciss_args(m->ciss[md]), rel_factors(F, N, md), (int)N, (int)R, span, true,
                             grads[md], part + (size_t)md * nb, &grid, 256));
     }

@@ -202,12 +202,16 @@ typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mo
 typedef enum ctc_als_kernel { CTC_ALS_AUTO = 0, CTC_ALS_SPANS = 1, CTC_ALS_ROWS = 2 } ctc_als_kernel;
 
 typedef enum ctc_sgd_update {
-  CTC_SGD_AUTO = 0,      /* walked-mode rows racy; a row of another mode is racy when
-                            it has more than sgd_hot_rows nonzeros (its atomics would
-                            serialise on one L2 channel), atomic otherwise */
-  CTC_SGD_ATOMIC = 1,    /* walked-mode rows racy, every other row atomic */
-  CTC_SGD_RACY = 2,      /* every row racy (fastest; loses updates on shared rows) */
-  CTC_SGD_NEXT_RACY = 3  /* walked mode and the next mode racy, other modes atomic */
+  CTC_SGD_AUTO = 0,       /* walked-mode rows racy; a row of another mode is racy when
+                             it has more than sgd_hot_rows nonzeros (its atomics would
+                             serialise on one L2 channel); otherwise a walker applies its
+                             updates to its register copy of the row and adds their sum
+                             with atomics when its run of nonzeros on that row ends */
+  CTC_SGD_ATOMIC = 1,     /* walked-mode rows racy, every other row atomic at every nonzero */
+  CTC_SGD_RACY = 2,       /* every row racy (fastest; loses updates on shared rows) */
+  CTC_SGD_NEXT_RACY = 3,  /* walked mode and the next mode racy, other modes atomic */
+  CTC_SGD_HOT_ATOMIC = 4  /* as AUTO, but a row that is not hot is re-read and updated
+                             with atomics at every nonzero */
 } ctc_sgd_update;
 
 typedef struct ctc_options {

@@ -170,7 +170,8 @@ def test_als_not_spd_is_reported(ctx):
 
 
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("policy,tile_len", [(ct.SGD_AUTO, None), (ct.SGD_AUTO, 1024), (ct.SGD_ATOMIC, 256)])
+@pytest.mark.parametrize("policy,tile_len", [(ct.SGD_AUTO, None), (ct.SGD_AUTO, 1024), (ct.SGD_ATOMIC, 256),
+                                             (ct.SGD_HOT_ATOMIC, None)])
 def test_sgd_test_rmse_trajectory(ctx, policy, tile_len):
     """SGD (lr 1e-3, x0.95 per epoch, lambda 0.05, scale 1) test-RMSE per
     epoch within 1e-3 of the sequential FP64 oracle."""
@@ -198,7 +199,7 @@ def test_sgd_test_rmse_trajectory(ctx, policy, tile_len):
 
 
 @pytest.mark.parametrize("R", [8, 16, 64, 128])
-@pytest.mark.parametrize("policy", [ct.SGD_AUTO, ct.SGD_ATOMIC, ct.SGD_RACY])
+@pytest.mark.parametrize("policy", [ct.SGD_AUTO, ct.SGD_ATOMIC, ct.SGD_RACY, ct.SGD_HOT_ATOMIC])
 def test_sgd_single_walker_matches_sequential(ctx, R, policy):
     """With one tile the GPU walk is one sequential walker: factors match the
     sequential oracle to FP32 rounding."""
