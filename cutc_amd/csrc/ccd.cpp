@@ -6,6 +6,8 @@
 // sum h^2) over each row; residual -= component f (new values).  The
 // subtraction of column f is deferred into the pass of column f+1 (or the next
 // epoch's column 0) so each (column, mode) costs one pass over the tensor.
+// At the start of column f every mode's per-row records are packed from the
+// column-major mirror: (pending column's new value, f saved, f current).
 #include <vector>
 
 #include "ctc_internal.h"
@@ -20,24 +22,17 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
   const uint32_t S = span_tiles_of(o);
   const uint32_t inner = o->inner_iters ? o->inner_iters : 1;
   hipStream_t st = m->ctx->stream;
-  size_t total_rows = 0, max_pieces = 0;
-  std::vector<size_t> off(N);
+  size_t max_pieces = 0;
   for (uint32_t md = 0; md < N; ++md) {
     CTC_TRY(ensure_ciss(m, md));
     CTC_TRY(ensure_pieces(m, md, S));
-    off[md] = total_rows;
-    total_rows += m->dims[md];
     max_pieces = std::max<size_t>(max_pieces, m->ciss[md].npieces);
     if (!m->FT[md]) CTC_HIP(hipMalloc(&m->FT[md], m->dims[md] * R * sizeof(float)));
+    if (!m->ccdx[md]) CTC_HIP(hipMalloc(&m->ccdx[md], std::max<uint64_t>(1, m->dims[md]) * 4 * sizeof(float)));
     if (!m->resid[md]) {
       CTC_HIP(hipMalloc(&m->resid[md], m->ciss[md].nnz * sizeof(float)));
       m->resid_valid = false;
     }
-  }
-  if (m->colsave_elems < total_rows) {
-    if (m->colsave) CTC_HIP(hipFree(m->colsave));
-    CTC_HIP(hipMalloc(&m->colsave, total_rows * sizeof(float)));
-    m->colsave_elems = total_rows;
   }
   CTC_TRY(ensure_scratch(m, max_pieces * 2 * sizeof(double)));
   double *pieces = (double *)m->scratch;
@@ -55,20 +50,13 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
   const unsigned block = block_of(o);
   for (uint32_t f = 0; f < R; ++f) {
     for (uint32_t md = 0; md < N; ++md)
-      CTC_HIP(hipMemcpyAsync(m->colsave + off[md], m->FT[md] + (size_t)f * m->dims[md],
-                             m->dims[md] * sizeof(float), hipMemcpyDeviceToDevice, st));
+      CTC_HIP(launch_ccd_pack(st, m->FT[md], m->dims[md], (int)f, m->ccd_pending, (float4 *)m->ccdx[md]));
     for (uint32_t it = 0; it < inner; ++it) {
       for (uint32_t md = 0; md < N; ++md) {
         const DevCiss &d = m->ciss[md];
         CcdColArgs a{};
         a.c = ciss_args(d);
-        for (uint32_t s = 0; s < N; ++s) {
-          const uint32_t q = (md + s) % N;
-          a.ft[s] = m->FT[q];
-          a.dims[s] = m->dims[q];
-          a.saved[s] = m->colsave + off[q];
-        }
-        a.col = (int)f;
+        for (uint32_t s = 0; s < N; ++s) a.x[s] = (const float4 *)m->ccdx[(md + s) % N];
         a.sub_col = it == 0 ? m->ccd_pending : -1;
         a.add_back = it == 0 ? 1 : 0;
         a.resid = m->resid[md];
@@ -84,7 +72,8 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
         CTC_HIP(launch_fill(st, col, m->dims[md], 0.f));  // rows with no nonzero -> 0
         {
           Prof p(m->ctx, "ccd_finish");
-          CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, o->lambda, col));
+          CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, o->lambda, col,
+                                    (float4 *)m->ccdx[md]));
         }
       }
     }

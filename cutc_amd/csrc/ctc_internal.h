@@ -141,8 +141,9 @@ struct ctc_model {
   // CCD++ residuals per mode (in that mode's CISS order)
   float *resid[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   bool resid_valid = false;
-  float *colsave = nullptr;  // CCD++ saved columns
-  size_t colsave_elems = 0;
+  // CCD++ per-row records of the column being updated, 4 floats per row:
+  // (pending column's new value, saved value, current value, unused)
+  float *ccdx[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   float *FT[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};  // column-major mirrors
   int ccd_pending = -1;  // column whose new component the residual copies still hold
   // SGD hot-row bitmaps (rows with more nonzeros than hot_threshold)

@@ -6,12 +6,16 @@
 //
 // Layout.  Every mode's CISS carries its own copy of the residual in its own
 // nonzero order, so each column pass streams it coalesced.  The factors are
-// mirrored column-major (R x dims) during an epoch so a column is a
-// contiguous, L2-resident vector and the per-nonzero gathers touch 4 bytes.
-// A copy stores "x minus every component except column `pending`"; the pass
-// of column f over mode m first completes the previous column (subtracting
-// its new component), then adds column f back with its pre-update values
-// (saved before the column's first mode update), and sums num/den.
+// mirrored column-major (R x dims) during an epoch.  A copy stores "x minus
+// every component except column `pending`"; the pass of column f over mode m
+// first completes the previous column (subtracting its new component), then
+// adds column f back with its pre-update values (saved before the column's
+// first mode update), and sums num/den.  The three values a pass needs of a
+// row (pending column's new value, column f saved, column f current) sit in
+// one 16-byte record per row, built at the start of each column: a nonzero
+// costs one gather per mode instead of three.  Divergent gathers are what
+// the pass is limited by (the texture addresser is the busy unit), so the
+// number of gather instructions, not their width, sets its time.
 //
 // Reductions.  One wave walks a span of tiles, one lane per nonzero.  Row
 // sums inside a tile use a segmented inclusive scan (DPP row shifts and
@@ -56,14 +60,7 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
   const uint64_t t0 = w * a.span_tiles;
   const uint64_t t1 = t0 + a.span_tiles < c.ntiles ? t0 + a.span_tiles : c.ntiles;
   const uint32_t poff = a.span_off[w];
-  // column pointers, CISS-relative (0 = own mode)
-  const float *cur[N], *sub[N], *old[N];
-#pragma unroll
-  for (int s = 0; s < N; ++s) {
-    cur[s] = a.ft[s] + (size_t)a.col * a.dims[s];
-    sub[s] = a.sub_col >= 0 ? a.ft[s] + (size_t)a.sub_col * a.dims[s] : nullptr;
-    old[s] = a.saved[s];
-  }
+  const bool fix = a.sub_col >= 0 || a.add_back;  // this pass rewrites the residual
   double carry_n = 0.0, carry_d = 0.0;
   uint32_t carry_slot = 0;
   bool have_carry = false;
@@ -80,29 +77,29 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
     if (valid) {
       const uint64_t p = base + lane;
       myslot = tile_lane_slot(first_slot, mask, lane);
-      uint32_t ix[N];
-      ix[0] = c.directory[myslot];
+      float4 q[N];
 #pragma unroll
-      for (int s = 0; s < N - 1; ++s) ix[1 + s] = c.oidx[s][p];
+      for (int s = 0; s < N - 1; ++s) q[1 + s] = a.x[1 + s][c.oidx[s][p]];
       float e = a.resid[p];
-      if (a.sub_col >= 0 || a.add_back) {
+      if (fix) {
+        q[0] = a.x[0][c.directory[myslot]];  // the own mode's row: pending and saved values
         if (a.sub_col >= 0) {
-          float v = sub[0][ix[0]];
+          float v = q[0].x;
 #pragma unroll
-          for (int s = 1; s < N; ++s) v *= sub[s][ix[s]];
+          for (int s = 1; s < N; ++s) v *= q[s].x;
           e -= v;
         }
         if (a.add_back) {
-          float v = old[0][ix[0]];
+          float v = q[0].y;
 #pragma unroll
-          for (int s = 1; s < N; ++s) v *= old[s][ix[s]];
+          for (int s = 1; s < N; ++s) v *= q[s].y;
           e += v;
         }
         a.resid[p] = e;
       }
-      float h = cur[1][ix[1]];
+      float h = q[1].z;
 #pragma unroll
-      for (int s = 2; s < N; ++s) h *= cur[s][ix[s]];
+      for (int s = 2; s < N; ++s) h *= q[s].z;
       num = e * h;
       den = h * h;
     }
@@ -158,7 +155,7 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
 // thousands of pieces of a heavy row).
 __global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const uint32_t *slot_pb,
                                                     const uint32_t *directory, uint64_t dlength,
-                                                    double lam, float *col) {
+                                                    double lam, float *col, float4 *x) {
   const uint64_t s = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= dlength) return;
   const int lane = lane_id();
@@ -171,7 +168,22 @@ __global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const 
     n += __shfl_xor(n, o, 64);
     d += __shfl_xor(d, o, 64);
   }
-  if (lane == 0) col[directory[s]] = (float)(n / (lam + d));
+  if (lane == 0) {
+    const uint32_t row = directory[s];
+    const float v = (float)(n / (lam + d));
+    col[row] = v;
+    x[row].z = v;  // the current value later modes' passes gather
+  }
+}
+
+// per-row records of column `col` at the start of its update
+__global__ void __launch_bounds__(256) k_ccd_pack(const float *ft, uint64_t rows, int col, int sub_col, float4 *x) {
+  const float *cur = ft + (size_t)col * rows;
+  const float *sub = sub_col >= 0 ? ft + (size_t)sub_col * rows : nullptr;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (uint64_t)gridDim.x * blockDim.x) {
+    const float v = cur[i];
+    x[i] = make_float4(sub ? sub[i] : 0.f, v, v, 0.f);
+  }
 }
 
 // residual copy of a mode: r = x - <rows>, row-major factors in CISS-relative order
@@ -226,10 +238,18 @@ hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsig
 }
 
 hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
-                             const uint32_t *directory, uint64_t dlength, double lam, float *col) {
+                             const uint32_t *directory, uint64_t dlength, double lam, float *col, float4 *x) {
   if (dlength == 0) return hipSuccess;
   hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 3) / 4)), dim3(256), 0, st, pieces,
-                     slot_pb, directory, dlength, lam, col);
+                     slot_pb, directory, dlength, lam, col, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_ccd_pack(hipStream_t st, const float *ft, uint64_t rows, int col, int sub_col, float4 *x) {
+  if (rows == 0) return hipSuccess;
+  uint64_t blocks = (rows + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_ccd_pack, dim3((unsigned)blocks), dim3(256), 0, st, ft, rows, col, sub_col, x);
   return hipGetLastError();
 }
 

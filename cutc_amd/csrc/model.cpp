@@ -389,6 +389,7 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->best[md]);
     dev_free(m->resid[md]);
     dev_free(m->FT[md]);
+    dev_free(m->ccdx[md]);
     dev_free(m->hotbits[md]);
   }
   for (DevCoo *c : {&m->valid, &m->test}) {
@@ -398,7 +399,6 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
   dev_free(m->scratch);
   dev_free(m->dstat);
   dev_free(m->dflag);
-  dev_free(m->colsave);
   if (m->hstat) (void)hipHostFree(m->hstat);
   delete m;
 }
