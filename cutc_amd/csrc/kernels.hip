@@ -599,6 +599,96 @@ __global__ void __launch_bounds__(64) k_als_solve(const double *pieces, const ui
   if (bad && lane == 0) atomicOr(flag, 1u);
 }
 
+// Same solve without refinement for R <= 32, one half-wave (32 lanes) per
+// slot, eight slots per 256-thread workgroup, no workgroup barriers.  Lane c
+// keeps column c of G in registers (rows i >= c; the upper part of the
+// register column holds don't-care values).  Root-free right-looking LDL^T:
+// at step k lane k publishes its column and right-hand-side entry to a
+// per-slot LDS buffer; d_k = buf[k] + lambda (lambda only ever sits on the
+// diagonal, so it is added when the pivot is read); every lane c > k
+// subtracts buf[i] * buf[c] / d_k from its column and l_ck z_k from its
+// right-hand-side entry, so the forward solve L z = m rides along.  Masking is
+// done by zero coefficients, not branches.  Then x = L^-T (D^-1 z), with
+// l_kc = g[k] / d_c read from lane c's own registers.  Instruction count per
+// system is about R^2 FMAs plus R reciprocals; the kernel is issue-bound, so
+// that count is its time.
+template <int R>
+__global__ void __launch_bounds__(256) k_als_solve_h(const double *pieces, const uint32_t *slot_pb,
+                                                     const double *gsum, const uint32_t *slot_gb,
+                                                     const uint32_t *directory, uint64_t dlength, double lam,
+                                                     float *A, uint32_t *flag) {
+  static_assert(R <= 32, "one half-wave per slot");
+  constexpr size_t PSZ = (size_t)R * R + R;
+  constexpr int CB = R + 2;  // column k, then z_k / x_k; even, so every half's buffer is 16-byte aligned
+  __shared__ __align__(16) double cbuf[8][CB];
+  const int lane = lane_id();
+  const int c = lane & 31;
+  const int cc = c < R ? c : R - 1;  // clamped column for lanes past the rank
+  const int hw = (int)(threadIdx.x >> 5);
+  const uint64_t s = (uint64_t)blockIdx.x * 8 + (uint64_t)hw;
+  const bool live = s < dlength;
+  double *buf = cbuf[hw];
+
+  double g[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) g[i] = 0.0;
+  double b = 0.0;
+  if (live && c < R) {
+    uint32_t p0 = slot_pb[s], p1 = slot_pb[s + 1];
+    const double *src = pieces;
+    if (slot_gb && slot_gb[s + 1] > slot_gb[s]) {  // heavy slot: its group sums
+      p0 = slot_gb[s];
+      p1 = slot_gb[s + 1];
+      src = gsum;
+    }
+    for (uint32_t p = p0; p < p1; ++p) {
+      const double *P = src + (size_t)p * PSZ;
+      // rows i < c are above the diagonal: don't-care (inside the piece, never used)
+#pragma unroll
+      for (int i = 0; i < R; ++i) g[i] += P[i * R + c];
+      b += P[R * R + c];
+    }
+  }
+
+  bool bad = false;
+  double rd = 1.0;  // lane c: 1 / d_c
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (c == k) {
+#pragma unroll
+      for (int i = k; i < R; ++i) buf[i] = g[i];
+      buf[R] = b;
+    }
+    wave_sync();
+    double d = buf[k] + lam;
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
+    }
+    const double rk = rcp_f64(d);
+    const double zk = buf[R];
+    const double lck = c > k ? buf[cc] * rk : 0.0;  // l_ck for the lanes still to factor, else 0
+#pragma unroll
+    for (int i = k + 1; i < R; ++i) g[i] = __builtin_fma(-buf[i], lck, g[i]);
+    b = __builtin_fma(-lck, zk, b);
+    if (c == k) rd = rk;
+    wave_sync();
+  }
+  // w = D^-1 z, then L^T x = w
+  b *= rd;
+#pragma unroll
+  for (int k = R - 1; k >= 0; --k) {
+    if (c == k) buf[R] = b;
+    wave_sync();
+    const double xk = buf[R];
+    const double lkc = c < k ? g[k] * rd : 0.0;  // l_kc (unit lower factor) for lanes before k
+    b = __builtin_fma(-lkc, xk, b);
+    wave_sync();
+  }
+  if (live && c < R) A[(size_t)directory[s] * R + c] = bad ? 0.f : (float)b;
+  if (live && bad && c == 0) atomicOr(flag, 1u);
+}
+
 // ==========================================================================
 // Evaluation: sum of squared errors over a COO (validation / test) or over a
 // mode's CISS (training), FP64 accumulation, one partial per block (fixed
@@ -891,9 +981,21 @@ hipError_t launch_piece_reduce(hipStream_t st, const double *pieces, const uint3
 
 hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb, const double *gsum,
                             const uint32_t *slot_gb, const uint32_t *directory, uint64_t dlength, int R,
-                            double lam, bool refine, float *A, uint32_t *flag) {
+                            double lam, bool refine, bool light, float *A, uint32_t *flag) {
   if (dlength == 0) return hipSuccess;
   const unsigned grid = (unsigned)dlength;
+  // light: few pieces per slot (short rows), where the per-slot factorisation
+  // dominates; slots that sum many pieces keep the 64-lane summation
+  if (!refine && light && R <= 32) {
+    const unsigned g8 = (unsigned)((dlength + 7) / 8);
+    switch (R) {
+      case 8: hipLaunchKernelGGL((k_als_solve_h<8>), dim3(g8), dim3(256), 0, st, pieces, slot_pb, gsum, slot_gb, directory, dlength, lam, A, flag); break;
+      case 16: hipLaunchKernelGGL((k_als_solve_h<16>), dim3(g8), dim3(256), 0, st, pieces, slot_pb, gsum, slot_gb, directory, dlength, lam, A, flag); break;
+      case 32: hipLaunchKernelGGL((k_als_solve_h<32>), dim3(g8), dim3(256), 0, st, pieces, slot_pb, gsum, slot_gb, directory, dlength, lam, A, flag); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
 #define CTC_SOLVE(RR)                                                                                      \
   do {                                                                                                      \
     if (refine)                                                                                             \

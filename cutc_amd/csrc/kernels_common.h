@@ -45,6 +45,20 @@ struct HotRows {
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
 
+// 1/x to full FP64 precision: hardware reciprocal estimate + two Newton steps
+__device__ inline double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+// LDS ordering between lanes of one wave (no workgroup barrier)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum across groups of W consecutive lanes (W in {1,2,...,64}); every lane of
 // the group receives the total.  DPP quad permutes for distances 1 and 2,
 // ds_swizzle (xor mode, within 32 lanes) for 4..16, ds_bpermute across halves.

@@ -21,7 +21,7 @@ hipError_t launch_piece_reduce(hipStream_t st, const double *pieces, const uint3
                                const uint32_t *grp_n, uint64_t ngroups, int R, double *gsum);
 hipError_t launch_als_solve(hipStream_t st, const double *pieces, const uint32_t *slot_pb, const double *gsum,
                             const uint32_t *slot_gb, const uint32_t *directory, uint64_t dlength, int R,
-                            double lam, bool refine, float *A, uint32_t *flag);
+                            double lam, bool refine, bool light, float *A, uint32_t *flag);
 hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f, int nmodes, int R,
                            unsigned nblocks, double *part);
 // per-segment squared error: exactly one of c (mode-0 CISS order) / co (COO) is non-null

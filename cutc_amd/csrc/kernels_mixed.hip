@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "kernels_common.h"
 #include "kernels_mixed.h"
 
 namespace ctc {
@@ -39,13 +40,6 @@ __device__ inline uint16_t f2bf(float x) {
 }
 __device__ inline float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ inline float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
-
-// LDS ordering between lanes of one wave (no workgroup barrier)
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // v_mfma_f32_32x32x16_bf16 accumulator v of lane l: row 8(v/4) + 4(l/32) + v%4, column l%32
 __device__ inline int row32(int v, int l) { return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3); }
@@ -296,13 +290,6 @@ __device__ inline void reduce_m(T *mp, MixedSmem<R> &sm) {
 }
 
 // ---------------------------------------------------------- solve phase --
-// 1/x to full FP64 precision: hardware reciprocal estimate + two Newton steps
-__device__ inline double rcp_f64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-}
-
 // wave-level: factor tile (K, K) (Cholesky), replace it by the inverse of its
 // lower factor; with fold, also y_K = inverse * vy_K in place.
 // The factorisation eliminates in place in LDS without square roots (after

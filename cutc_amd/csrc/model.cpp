@@ -541,7 +541,8 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   {
     Prof p(m->ctx, "als_solve");
     CTC_HIP(launch_als_solve(st, pieces, d.slot_pb, d.ngroups ? gsum : nullptr, d.slot_gb, d.directory,
-                             d.dlength, (int)R, o->lambda, o->refine != 0, m->F[mode], m->dflag));
+                             d.dlength, (int)R, o->lambda, o->refine != 0, d.npieces <= 2 * d.dlength, m->F[mode],
+                             m->dflag));
   }
   m->resid_valid = false;
   return CTC_OK;
