@@ -344,9 +344,8 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 template <int R, int N>
-__global__ void __launch_bounds__(256) k_als_gram(CissArgs c, FactorArgs fa, uint32_t span_tiles,
-                                                  uint64_t nspans, const uint32_t *span_off,
-                                                  double *pieces) {
+__device__ __forceinline__ void als_gram_walk(const CissArgs &c, const FactorArgs &fa, uint32_t span_tiles,
+                                              uint64_t nspans, const uint32_t *span_off, double *pieces) {
   constexpr int CG = (R + 15) / 16;      // 16-column groups
   constexpr int NB = CG * (CG + 1) / 2;  // lower-triangle 16x16 blocks
   constexpr int NPS = 4;                 // nonzeros per MFMA step
@@ -450,6 +449,21 @@ __global__ void __launch_bounds__(256) k_als_gram(CissArgs c, FactorArgs fa, uin
     }
   }
   flush(slot + poff);
+}
+
+// Occupancy: at R <= 32 the walk fits 72 registers, which allows 7 waves per
+// SIMD instead of 6 (the walk waits on its row gathers, so more waves keep the
+// matrix cores busier); at R = 64 that bound would spill, so it keeps the
+// default.
+template <int R, int N>
+__global__ void __launch_bounds__(256, 7) k_als_gram(CissArgs c, FactorArgs fa, uint32_t span_tiles,
+                                                     uint64_t nspans, const uint32_t *span_off, double *pieces) {
+  als_gram_walk<R, N>(c, fa, span_tiles, nspans, span_off, pieces);
+}
+template <int R, int N>
+__global__ void __launch_bounds__(256) k_als_gram_wide(CissArgs c, FactorArgs fa, uint32_t span_tiles,
+                                                       uint64_t nspans, const uint32_t *span_off, double *pieces) {
+  als_gram_walk<R, N>(c, fa, span_tiles, nspans, span_off, pieces);
 }
 
 // ==========================================================================
@@ -958,7 +972,14 @@ hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &
     case 8: CTC_GRAM(8); break;
     case 16: CTC_GRAM(16); break;
     case 32: CTC_GRAM(32); break;
-    case 64: CTC_GRAM(64); break;
+    case 64:
+      if (nmodes == 3)
+        hipLaunchKernelGGL((k_als_gram_wide<64, 3>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans,
+                           span_off, pieces);
+      else
+        hipLaunchKernelGGL((k_als_gram_wide<64, 4>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans,
+                           span_off, pieces);
+      break;
     default: return hipErrorInvalidValue;
   }
 #undef CTC_GRAM
