@@ -31,6 +31,7 @@ struct MixedArgs {
   uint64_t nitems;
   const uint4 *thin;  // thin rows {slot, first, end, -}: n <= min(kThinRow, R)
   uint64_t nthin;
+  uint64_t nthin16;   // the first nthin16 of them have n <= 16
   void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])

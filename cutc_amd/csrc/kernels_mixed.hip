@@ -688,25 +688,35 @@ __device__ inline void store_row(const MixedArgs &a, MixedSmem<R> &sm, uint32_t 
 // cores, K + lambda I eliminated in place (L D L^T, no square roots), two
 // triangular solves, a = H^T z.  All in FP64 (the BF16 path stages the
 // BF16-rounded h), so no refinement step is needed.
-template <int R, int P>
+// NT = 16 serves rows of at most 16 nonzeros, NT = 32 the rest; the LDS of a
+// workgroup is sized for its NT, and the BF16 path keeps h as BF16 bits, so
+// the common short rows take ~6 KB of LDS at R = 128 (several waves per SIMD)
+// instead of the 25 KB a 32-row FP32 stage needs.
+template <int R, int P, int NT>
 struct ThinSmem {
-  typedef typename std::conditional<P == kPrecBF16, float, double>::type HT;
+  typedef typename std::conditional<P == kPrecBF16, uint16_t, double>::type HT;
   static constexpr int HS = R + 1;  // staged h stride (odd: spreads LDS banks)
-  static constexpr int KS = kThinRow + 1;
-  HT H[kThinRow * HS];
-  double K[kThinRow * KS];
-  double x[kThinRow];
+  static constexpr int KS = NT + 1;
+  HT H[NT * HS];
+  double K[NT * KS];
+  double x[NT];
   int bad;
+  __device__ static double hval(HT v) {
+    if constexpr (P == kPrecBF16)
+      return (double)__uint_as_float((uint32_t)v << 16);
+    else
+      return v;
+  }
 };
 
-template <int R, int N, int P>
-__global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
-  typedef ThinSmem<R, P> TS;
+template <int R, int N, int P, int NT>
+__global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
+  typedef ThinSmem<R, P, NT> TS;
   __shared__ TS sm;
-  const uint4 it = a.thin[blockIdx.x];
+  const uint4 it = a.thin[base + blockIdx.x];
   const int l = threadIdx.x;
   const int n = (int)(it.z - it.y);
-  const int nt = n <= 16 ? 16 : 32;  // system size handled (zero-padded)
+  constexpr int nt = NT;  // system size handled (zero-padded)
   // 1. stage h for the row's nonzeros (rows >= n zero) and x.  Lane p < n
   // loads nonzero p's ids once; the gathers of all nonzeros are then issued
   // back to back (independent loads, one memory latency for the row).
@@ -714,7 +724,7 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
   uint32_t myid[N - 1];
 #pragma unroll
   for (int s = 0; s < N - 1; ++s) myid[s] = l < n ? a.oidx[s][it.y + l] : 0u;
-  if (l < kThinRow) sm.x[l] = l < n ? (double)a.vals[it.y + l] : 0.0;
+  if (l < NT) sm.x[l] = l < n ? (double)a.vals[it.y + l] : 0.0;
 #pragma unroll 8
   for (int p = 0; p < nt; ++p) {
     uint32_t id[N - 1];
@@ -726,11 +736,11 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
       if (f >= R) continue;
       typename TS::HT h = 0;
       if (p < n) {
-        if (P == kPrecBF16) {
+        if constexpr (P == kPrecBF16) {
           float hf = 1.f;
 #pragma unroll
           for (int s = 0; s < N - 1; ++s) hf *= __uint_as_float((uint32_t)a.fb[s][(size_t)id[s] * R + f] << 16);
-          h = __uint_as_float((uint32_t)f2bf(hf) << 16);
+          h = f2bf(hf);
         } else {
           double hd = 1.0;
 #pragma unroll
@@ -747,10 +757,10 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
   f64x4 k00 = {0.0, 0.0, 0.0, 0.0}, k10 = k00, k11 = k00;
   for (int f0 = 0; f0 < R; f0 += 4) {
     const int f = f0 + (l >> 4);
-    const double h0 = (double)sm.H[(l & 15) * TS::HS + f];
+    const double h0 = TS::hval(sm.H[(l & 15) * TS::HS + f]);
     k00 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0, h0, k00, 0, 0, 0);
-    if (nt == 32) {
-      const double h1 = (double)sm.H[(16 + (l & 15)) * TS::HS + f];
+    if constexpr (NT == 32) {
+      const double h1 = TS::hval(sm.H[(16 + (l & 15)) * TS::HS + f]);
       k10 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1, h0, k10, 0, 0, 0);
       k11 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1, h1, k11, 0, 0, 0);
     }
@@ -765,7 +775,7 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
       sm.K[rr * TS::KS + cc] = pad ? (rr == cc ? 1.0 : 0.0) : val + (rr == cc ? lam : 0.0);
     };
     put(r, c, k00[v]);
-    if (nt == 32) {
+    if constexpr (NT == 32) {
       put(16 + r, c, k10[v]);
       put(16 + r, 16 + c, k11[v]);
     }
@@ -779,10 +789,10 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
     for (int k = 0; k < nt - 1; ++k) {
       const double dk = sm.K[k * TS::KS + k];
       ok = ok && dk > 0.0;
-      const double lck = sm.K[c * TS::KS + k] * rcp_f64(dk > 0.0 ? dk : 1.0);
+      const double lck = (c < NT ? sm.K[c * TS::KS + k] : 0.0) * rcp_f64(dk > 0.0 ? dk : 1.0);
       double aik[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) aik[e] = sm.K[(16 * half + e) * TS::KS + k];
+      for (int e = 0; e < 16; ++e) aik[e] = (16 * half + e < NT) ? sm.K[(16 * half + e) * TS::KS + k] : 0.0;
       wave_sync();
       if (c > k && c < nt) {
 #pragma unroll
@@ -811,14 +821,14 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a) {
     if (l < j) xi = __builtin_fma(-sm.K[j * TS::KS + l] * rdi, zj, xi);
   }
   wave_sync();
-  if (l < kThinRow) sm.x[l] = l < nt ? xi : 0.0;
+  if (l < NT) sm.x[l] = xi;
   wave_sync();
   // 5. a = H^T z
   float *dst = a.out + (size_t)a.directory[it.x] * R;
   const bool ok = sm.bad == 0;
   for (int f = l; f < R; f += 64) {
     double acc = 0.0;
-    for (int p = 0; p < n; ++p) acc = __builtin_fma((double)sm.H[p * TS::HS + f], sm.x[p], acc);
+    for (int p = 0; p < n; ++p) acc = __builtin_fma(TS::hval(sm.H[p * TS::HS + f]), sm.x[p], acc);
     dst[f] = ok ? (float)acc : 0.f;
   }
   if (!ok && l == 0) atomicOr(a.flag, 1u);
@@ -956,12 +966,17 @@ hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, in
   return hipGetLastError();
 }
 
-#define CTC_THIN_LAUNCH(RR, NN)                                                                         \
-  do {                                                                                                 \
-    if (a.prec == kPrecBF16)                                                                           \
-      hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecBF16>), dim3((unsigned)a.nthin), dim3(64), 0, st, a); \
-    else                                                                                               \
-      hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecF64>), dim3((unsigned)a.nthin), dim3(64), 0, st, a);  \
+// rows of at most 16 nonzeros first (NT = 16), then the rest (NT = 32)
+#define CTC_THIN_LAUNCH(RR, NN)                                                                                    \
+  do {                                                                                                            \
+    const uint64_t n16 = a.nthin16, n32 = a.nthin - a.nthin16;                                                    \
+    if (a.prec == kPrecBF16) {                                                                                    \
+      if (n16) hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecBF16, 16>), dim3((unsigned)n16), dim3(64), 0, st, a, 0ull); \
+      if (n32) hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecBF16, 32>), dim3((unsigned)n32), dim3(64), 0, st, a, n16);  \
+    } else {                                                                                                      \
+      if (n16) hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecF64, 16>), dim3((unsigned)n16), dim3(64), 0, st, a, 0ull);  \
+      if (n32) hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecF64, 32>), dim3((unsigned)n32), dim3(64), 0, st, a, n16);   \
+    }                                                                                                             \
   } while (0)
 
 hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R) {

@@ -29,13 +29,13 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   DevCiss &d = m->ciss[mode];
   if (d.mix_items && d.mix_chunk == chunk) return CTC_OK;
   const uint32_t thin = std::min<uint32_t>((uint32_t)kThinRow, m->rank);
-  std::vector<uint4> chunks, light, thins;
+  std::vector<uint4> chunks, light, thins, thins32;
   std::vector<uint32_t> hslot, hpb{0};
   uint32_t nparts = 0;
   for (uint64_t s = 0; s < d.dlength; ++s) {
     const uint32_t p0 = d.h_rowptr[s], p1 = d.h_rowptr[s + 1];
-    if (p1 - p0 <= thin) {
-      thins.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
+    if (p1 - p0 <= thin) {  // 16-row systems first, then 32-row ones
+      (p1 - p0 <= 16 ? thins : thins32).push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
       continue;
     }
     if (p1 - p0 <= chunk) {
@@ -48,6 +48,8 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   }
   // the long rows' chunks first, so the longest work starts earliest; the
   // thin rows (dual-form kernel) last
+  const size_t nthin16 = thins.size();
+  thins.insert(thins.end(), thins32.begin(), thins32.end());
   std::vector<uint4> items;
   items.reserve(chunks.size() + light.size() + thins.size());
   items.insert(items.end(), chunks.begin(), chunks.end());
@@ -71,6 +73,7 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   d.mix_chunk = chunk;
   d.mix_nitems = items.size() - thins.size();
   d.mix_nthin = thins.size();
+  d.mix_nthin16 = nthin16;
   d.mix_nheavy = hslot.size();
   d.mix_nparts = nparts;
   return CTC_OK;
@@ -120,6 +123,7 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.nitems = d.mix_nitems;
   a.thin = d.mix_items + d.mix_nitems;
   a.nthin = d.mix_nthin;
+  a.nthin16 = d.mix_nthin16;
   a.part = m->scratch;
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
