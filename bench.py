@@ -108,13 +108,15 @@ def mixed_leg(ct, ctx, a, log):
     nc, msc = ctx.kernel_stats("als_mixed_chunks")
     nh, msh = ctx.kernel_stats("als_mixed_heavy")
     nt, mst = ctx.kernel_stats("als_mixed_thin")
+    nd, msd = ctx.kernel_stats("als_mixed_dual")
     res.update({"s_per_iter": wall, "nnz_per_s": tr.nnz / wall,
                 "rows_kernel_ms_per_iter": msi / a.mixed_iters, "chunks_kernel_ms_per_iter": msc / a.mixed_iters,
                 "heavy_kernel_ms_per_iter": msh / a.mixed_iters, "thin_kernel_ms_per_iter": mst / a.mixed_iters,
+                "dual_kernel_ms_per_iter": msd / a.mixed_iters,
                 "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
     log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), long-row chunks {msc / a.mixed_iters:.0f} ms, "
         f"rows {msi / a.mixed_iters:.0f} ms, long-row solves {msh / a.mixed_iters:.0f} ms, thin rows "
-        f"{mst / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
+        f"{mst / a.mixed_iters:.0f} ms, 33-64-nonzero rows (dual) {msd / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
     # the plain FP32 path (FP64-accumulated Gram, same solves) from the same init
     fopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1)
     m.init_random(5)

@@ -14,6 +14,8 @@ namespace ctc {
 constexpr uint32_t kLightRow = 0xFFFFFFFFu;
 // rows with at most this many nonzeros are solved in the dual (n x n) form
 constexpr int kThinRow = 32;
+// rows with kThinRow < n <= kDualRow nonzeros at R >= 64 (dual form, 64 x 64)
+constexpr int kDualRow = 64;
 
 // Gram precision: BF16 operands with FP32 accumulation, or FP32 factors with
 // h formed and accumulated in FP64
@@ -32,6 +34,8 @@ struct MixedArgs {
   const uint4 *thin;  // thin rows {slot, first, end, -}: n <= min(kThinRow, R)
   uint64_t nthin;
   uint64_t nthin16;   // the first nthin16 of them have n <= 16
+  const uint4 *dual;  // rows {slot, first, end, -} with kThinRow < n <= kDualRow (R >= 64)
+  uint64_t ndual;
   void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])
@@ -45,6 +49,7 @@ struct MixedArgs {
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+hipError_t launch_rows_dual(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 size_t mixed_partial_bytes(int R, int prec);
 // FP32 -> BF16 (round to nearest even), n elements
 hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, uint64_t n);

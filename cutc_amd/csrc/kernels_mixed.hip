@@ -834,6 +834,158 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
   if (!ok && l == 0) atomicOr(a.flag, 1u);
 }
 
+// ------------------------------------------------------------- dual rows --
+// Rows with kThinRow < n <= kDualRow nonzeros at R >= 64, also in the dual
+// form a = H^T (H H^T + lambda I)^-1 x, as an n x n system padded to 64:
+// cheaper than the R x R primal solve (64^3 / 3 instead of R^3 / 3 FP64
+// flops, one wave instead of a barrier-separated blocked Cholesky).  One wave
+// per row: h staged in LDS (BF16 bits), K = H H^T tile by tile on
+// v_mfma_f64_16x16x4f64, each 16 x 16 tile passed through a small LDS buffer
+// into the registers of the lanes that own its columns (lane c holds column
+// c of K), then the root-free right-looking L D L^T with one LDS publish per
+// column (the forward solve folded in), the back solve from the owning lanes'
+// registers, a = H^T z.  LDS is the h stage plus 3 KB; the 64 FP64 column
+// entries per lane hold the kernel to one wave per SIMD.
+template <int R, int P>
+struct DualSmem {
+  typedef typename std::conditional<P == kPrecBF16, uint16_t, double>::type HT;
+  static constexpr int HS = R + 1;
+  HT H[kDualRow * HS];
+  double T[16 * 17];           // one K tile on its way to the column owners
+  double pub[kDualRow + 2];    // column k and z_k / x_k
+  double z[kDualRow];
+  __device__ static double hval(HT v) {
+    if constexpr (P == kPrecBF16)
+      return (double)__uint_as_float((uint32_t)v << 16);
+    else
+      return v;
+  }
+};
+
+template <int R, int N, int P>
+__global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
+  static_assert(R >= kDualRow, "the dual form pays only for n < R");
+  constexpr int NT = kDualRow;
+  typedef DualSmem<R, P> DS;
+  __shared__ DS sm;
+  const uint4 it = a.dual[blockIdx.x];
+  const int l = threadIdx.x;
+  const int n = (int)(it.z - it.y);
+  // 1. stage h (rows >= n zero)
+  constexpr int FPL = R / 64;
+  uint32_t myid[N - 1];
+#pragma unroll
+  for (int s = 0; s < N - 1; ++s) myid[s] = l < n ? a.oidx[s][it.y + l] : 0u;
+  const double xl = l < n ? (double)a.vals[it.y + l] : 0.0;
+#pragma unroll 8
+  for (int p = 0; p < NT; ++p) {
+    uint32_t id[N - 1];
+#pragma unroll
+    for (int s = 0; s < N - 1; ++s) id[s] = (uint32_t)__shfl((int)myid[s], p, 64);
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+      const int f = l + 64 * j;
+      typename DS::HT h = 0;
+      if (p < n) {
+        if constexpr (P == kPrecBF16) {
+          float hf = 1.f;
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) hf *= __uint_as_float((uint32_t)a.fb[s][(size_t)id[s] * R + f] << 16);
+          h = f2bf(hf);
+        } else {
+          double hd = 1.0;
+#pragma unroll
+          for (int s = 0; s < N - 1; ++s) hd *= (double)a.fr[s][(size_t)id[s] * R + f];
+          h = hd;
+        }
+      }
+      sm.H[p * DS::HS + f] = h;
+    }
+  }
+  wave_sync();
+  // 2. K = H H^T + lambda I (lower 16x16 tiles; padded rows/columns ->
+  // identity); lane c collects column c (rows >= c meaningful, the rest are
+  // don't-care)
+  const double lam = a.lam;
+  double g[NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i) g[i] = 0.0;
+#pragma unroll
+  for (int bi = 0; bi < NT / 16; ++bi) {
+#pragma unroll
+    for (int bj = 0; bj <= bi; ++bj) {
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+      for (int f0 = 0; f0 < R; f0 += 4) {
+        const int f = f0 + (l >> 4);
+        const double ha = DS::hval(sm.H[(16 * bi + (l & 15)) * DS::HS + f]);
+        const double hb = DS::hval(sm.H[(16 * bj + (l & 15)) * DS::HS + f]);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ha, hb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * bi + row16(v, l), c = 16 * bj + (l & 15);
+        const bool pad = r >= n || c >= n;
+        sm.T[row16(v, l) * 17 + (l & 15)] = pad ? (r == c ? 1.0 : 0.0) : acc[v] + (r == c ? lam : 0.0);
+      }
+      wave_sync();
+      if ((l >> 4) == bj) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) g[16 * bi + r] = sm.T[r * 17 + (l & 15)];
+      }
+      wave_sync();
+    }
+  }
+  double b = xl;
+  double *buf = sm.pub;  // column k (NT values), then z_k / x_k
+  bool bad = false;
+  double rd = 1.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    if (l == k) {
+#pragma unroll
+      for (int i = k; i < NT; ++i) buf[i] = g[i];
+      buf[NT] = b;
+    }
+    wave_sync();
+    double d = buf[k];
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
+    }
+    const double rk = rcp_f64(d);
+    const double zk = buf[NT];
+    const double lck = l > k ? buf[l] * rk : 0.0;
+#pragma unroll
+    for (int i = k + 1; i < NT; ++i) g[i] = __builtin_fma(-buf[i], lck, g[i]);
+    b = __builtin_fma(-lck, zk, b);
+    if (l == k) rd = rk;
+    wave_sync();
+  }
+  b *= rd;
+#pragma unroll
+  for (int k = NT - 1; k >= 0; --k) {
+    if (l == k) buf[NT] = b;
+    wave_sync();
+    const double xk = buf[NT];
+    const double lkc = l < k ? g[k] * rd : 0.0;
+    b = __builtin_fma(-lkc, xk, b);
+    wave_sync();
+  }
+  sm.z[l] = b;
+  wave_sync();
+  // 4. a = H^T z
+  float *dst = a.out + (size_t)a.directory[it.x] * R;
+#pragma unroll
+  for (int j = 0; j < FPL; ++j) {
+    const int f = l + 64 * j;
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc = __builtin_fma(DS::hval(sm.H[p * DS::HS + f]), sm.z[p], acc);
+    dst[f] = bad ? 0.f : (float)acc;
+  }
+  if (bad && l == 0) atomicOr(a.flag, 1u);
+}
+
 // ---------------------------------------------------------------- kernels --
 // Gram + MTTKRP of one item into the accumulators and sm.vm
 template <int R, int N>
@@ -978,6 +1130,28 @@ hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, in
       if (n32) hipLaunchKernelGGL((k_rows_thin<RR, NN, kPrecF64, 32>), dim3((unsigned)n32), dim3(64), 0, st, a, n16);   \
     }                                                                                                             \
   } while (0)
+
+#define CTC_DUAL_LAUNCH(RR, NN)                                                                        \
+  do {                                                                                                \
+    if (a.prec == kPrecBF16)                                                                          \
+      hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecBF16>), dim3((unsigned)a.ndual), dim3(64), 0, st, a); \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecF64>), dim3((unsigned)a.ndual), dim3(64), 0, st, a);  \
+  } while (0)
+
+hipError_t launch_rows_dual(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
+  if (a.ndual == 0) return hipSuccess;
+  if (R != 128) return hipErrorInvalidValue;  // dual rows are only formed at R = 128
+  if (nmodes == 3) {
+    CTC_DUAL_LAUNCH(128, 3);
+  } else if (nmodes == 4) {
+    CTC_DUAL_LAUNCH(128, 4);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+#undef CTC_DUAL_LAUNCH
 
 hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
   if (a.nthin == 0) return hipSuccess;

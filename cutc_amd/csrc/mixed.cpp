@@ -7,8 +7,9 @@
 // SG/als.cu:699 does), the items pass solves every row of at most
 // `chunk` nonzeros in one workgroup and writes partial systems for the chunks
 // of longer rows, the heavy pass sums each long row's partials and solves it,
-// and the thin pass solves the rows of at most min(32, R) nonzeros in their
-// dual n x n form, one wave each.
+// the thin pass solves the rows of at most min(32, R) nonzeros in their dual
+// n x n form, one wave each, and at R >= 64 the dual pass does the same for
+// rows of 33..64 nonzeros (64 x 64 systems).
 #include <algorithm>
 #include <vector>
 
@@ -29,13 +30,20 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   DevCiss &d = m->ciss[mode];
   if (d.mix_items && d.mix_chunk == chunk) return CTC_OK;
   const uint32_t thin = std::min<uint32_t>((uint32_t)kThinRow, m->rank);
-  std::vector<uint4> chunks, light, thins, thins32;
+  std::vector<uint4> chunks, light, thins, thins32, duals;
+  // dual 64 x 64 systems for rows of 33..64 nonzeros pay at R = 128 (at R = 64
+  // the primal 4 x 4-tile solve measured faster)
+  const uint32_t dual = m->rank >= 2 * (uint32_t)kDualRow ? (uint32_t)kDualRow : 0u;
   std::vector<uint32_t> hslot, hpb{0};
   uint32_t nparts = 0;
   for (uint64_t s = 0; s < d.dlength; ++s) {
     const uint32_t p0 = d.h_rowptr[s], p1 = d.h_rowptr[s + 1];
     if (p1 - p0 <= thin) {  // 16-row systems first, then 32-row ones
       (p1 - p0 <= 16 ? thins : thins32).push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
+      continue;
+    }
+    if (p1 - p0 <= dual) {
+      duals.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
       continue;
     }
     if (p1 - p0 <= chunk) {
@@ -55,6 +63,7 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   items.insert(items.end(), chunks.begin(), chunks.end());
   items.insert(items.end(), light.begin(), light.end());
   items.insert(items.end(), thins.begin(), thins.end());
+  items.insert(items.end(), duals.begin(), duals.end());
   for (void *p : {(void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb})
     if (p) CTC_HIP(hipFree(p));
   d.mix_items = nullptr;
@@ -71,9 +80,10 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   CTC_HIP(hipMemcpyAsync(d.mix_heavy_pb, hpb.data(), hpb.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   CTC_HIP(hipStreamSynchronize(st));
   d.mix_chunk = chunk;
-  d.mix_nitems = items.size() - thins.size();
+  d.mix_nitems = items.size() - thins.size() - duals.size();
   d.mix_nthin = thins.size();
   d.mix_nthin16 = nthin16;
+  d.mix_ndual = duals.size();
   d.mix_nheavy = hslot.size();
   d.mix_nparts = nparts;
   return CTC_OK;
@@ -124,6 +134,8 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.thin = d.mix_items + d.mix_nitems;
   a.nthin = d.mix_nthin;
   a.nthin16 = d.mix_nthin16;
+  a.dual = a.thin + a.nthin;
+  a.ndual = d.mix_ndual;
   a.part = m->scratch;
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
@@ -153,6 +165,10 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   if (a.nthin) {
     Prof p(m->ctx, bf16 ? "als_mixed_thin" : "als_rows_thin");
     CTC_HIP(launch_rows_thin(st, a, (int)N, (int)R));
+  }
+  if (a.ndual) {
+    Prof p(m->ctx, bf16 ? "als_mixed_dual" : "als_rows_dual");
+    CTC_HIP(launch_rows_dual(st, a, (int)N, (int)R));
   }
   m->resid_valid = false;
   return CTC_OK;
