@@ -86,7 +86,9 @@ extern "C" void ctc_batch_destroy(ctc_batch *b) {
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->t0) (void)hipEventDestroy(b->t0);
   if (b->t1) (void)hipEventDestroy(b->t1);
+  ctc_context *ctx = b->ctx;
   delete b;
+  context_release(ctx);
 }
 
 // concatenates the given problems' tensors with per-mode row offsets
@@ -127,6 +129,7 @@ extern "C" ctc_status ctc_batch_create(ctc_context *ctx, uint32_t nproblems, con
   CTC_HIP(hipSetDevice(ctx->device));
   ctc_batch *b = new ctc_batch();
   b->ctx = ctx;
+  context_retain(ctx);
   b->nmodes = N;
   b->nproblems = nproblems;
   b->place.resize(nproblems);

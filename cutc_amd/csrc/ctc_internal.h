@@ -76,9 +76,11 @@ struct DevCiss {
   uint32_t *slot_gb = nullptr;    // dlength + 1: groups of slot s (empty for light slots)
   // BF16-operand ALS work items (see kernels_mixed.h) for rows cut at mix_chunk nonzeros
   uint32_t mix_chunk = 0;
-  uint64_t mix_nitems = 0, mix_nheavy = 0, mix_nparts = 0, mix_nthin = 0, mix_nthin16 = 0, mix_ndual = 0;
+  uint64_t mix_nitems = 0, mix_nheavy = 0, mix_nparts = 0, mix_nthin = 0, mix_nthin16 = 0, mix_ndual = 0,
+           mix_ngroups = 0;
   uint4 *mix_items = nullptr;
-  uint32_t *mix_heavy_slot = nullptr, *mix_heavy_pb = nullptr;
+  uint32_t *mix_heavy_slot = nullptr, *mix_heavy_pb = nullptr, *mix_heavy_gb = nullptr;
+  uint2 *mix_grp = nullptr;  // long rows' partial groups {first partial, count}
   // host copies needed by the piece-table builder
   std::vector<uint32_t> h_tile_slot;
   std::vector<uint64_t> h_tile_mask;
@@ -106,6 +108,10 @@ struct KernelStat {
 };
 
 struct ctc_context {
+  // holders: the handle returned by ctc_context_create plus every model and
+  // batch created on it; the context is freed when the last one lets go, so
+  // handles may be destroyed in any order
+  int refs = 1;
   int device = 0;
   hipStream_t stream = nullptr;
   bool profiling = false;
@@ -119,6 +125,9 @@ struct ctc_context {
   std::vector<hipEvent_t> event_pool;
   hipDeviceProp_t props{};
 };
+
+void context_retain(ctc_context *c);
+void context_release(ctc_context *c);
 
 struct ctc_model {
   ctc_context *ctx = nullptr;

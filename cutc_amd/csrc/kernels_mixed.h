@@ -16,6 +16,7 @@ constexpr uint32_t kLightRow = 0xFFFFFFFFu;
 constexpr int kThinRow = 32;
 // rows with kThinRow < n <= kDualRow nonzeros at R >= 64 (dual form, 64 x 64)
 constexpr int kDualRow = 64;
+constexpr uint32_t kGroup = 32, kGroupMin = 64;
 
 // Gram precision: BF16 operands with FP32 accumulation, or FP32 factors with
 // h formed and accumulated in FP64
@@ -40,6 +41,13 @@ struct MixedArgs {
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])
   uint64_t nheavy;
+  // heavy rows with more than kGroupMin partials are first reduced in groups
+  // of kGroup consecutive partials: group g sums {grp[g].x, grp[g].x + grp[g].y)
+  // into gpart[g]; heavy row h then sums groups [heavy_gb[h], heavy_gb[h + 1])
+  void *gpart;
+  const uint2 *grp;
+  uint64_t ngroups;
+  const uint32_t *heavy_gb;
   double lam;
   int refine;  // iterative-refinement steps after the FP64 Cholesky solve
   uint32_t *flag;
@@ -48,6 +56,7 @@ struct MixedArgs {
 // items pass (light rows solved, heavy chunks to partials), then heavy rows
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+hipError_t launch_mixed_groups(hipStream_t st, const MixedArgs &a, int R);
 hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_rows_dual(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 size_t mixed_partial_bytes(int R, int prec);

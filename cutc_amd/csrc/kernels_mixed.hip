@@ -1060,8 +1060,15 @@ __global__ void __launch_bounds__(2 * R, 2) k_mixed_heavy(MixedArgs a) {
 #pragma unroll
     for (int v = 0; v < S::NV; ++v) acc[j][v] = 0;
   double mv = 0.0;
-  for (uint32_t pi = a.heavy_pb[h]; pi < a.heavy_pb[h + 1]; ++pi) {
-    const typename S::E *Pp = (const typename S::E *)a.part + (size_t)pi * S::PSZ;
+  uint32_t p0 = a.heavy_pb[h], p1 = a.heavy_pb[h + 1];
+  const typename S::E *src = (const typename S::E *)a.part;
+  if (a.heavy_gb && a.heavy_gb[h + 1] > a.heavy_gb[h]) {  // long row: its group sums
+    p0 = a.heavy_gb[h];
+    p1 = a.heavy_gb[h + 1];
+    src = (const typename S::E *)a.gpart;
+  }
+  for (uint32_t pi = p0; pi < p1; ++pi) {
+    const typename S::E *Pp = src + (size_t)pi * S::PSZ;
 #pragma unroll
     for (int j = 0; j < S::NSLOT; ++j) {
       if (!S::used(w, j)) continue;
@@ -1074,6 +1081,34 @@ __global__ void __launch_bounds__(2 * R, 2) k_mixed_heavy(MixedArgs a) {
   __syncthreads();
   const bool ok = solve_row<R>(sm, acc, a.lam, a.refine);
   store_row<R>(a, sm, a.heavy_slot[h], ok);
+}
+
+// group sums of consecutive partials of one long row (fixed order, FP64
+// accumulation, stored in the partials' own type), one workgroup per group
+template <typename E>
+__global__ void __launch_bounds__(256) k_mixed_group(const E *part, const uint2 *grp, uint64_t ngroups, size_t psz,
+                                                     E *gpart) {
+  const uint64_t g = blockIdx.x;
+  if (g >= ngroups) return;
+  const uint2 gr = grp[g];
+  const E *P = part + (size_t)gr.x * psz;
+  for (size_t e = threadIdx.x; e < psz; e += blockDim.x) {
+    double acc = 0.0;
+    for (uint32_t q = 0; q < gr.y; ++q) acc += (double)P[(size_t)q * psz + e];
+    gpart[g * psz + e] = (E)acc;
+  }
+}
+
+hipError_t launch_mixed_groups(hipStream_t st, const MixedArgs &a, int R) {
+  if (a.ngroups == 0) return hipSuccess;
+  const size_t bytes = mixed_partial_bytes(R, a.prec);
+  if (a.prec == kPrecBF16)
+    hipLaunchKernelGGL(k_mixed_group<float>, dim3((unsigned)a.ngroups), dim3(256), 0, st, (const float *)a.part,
+                       a.grp, a.ngroups, bytes / sizeof(float), (float *)a.gpart);
+  else
+    hipLaunchKernelGGL(k_mixed_group<double>, dim3((unsigned)a.ngroups), dim3(256), 0, st, (const double *)a.part,
+                       a.grp, a.ngroups, bytes / sizeof(double), (double *)a.gpart);
+  return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(256) k_to_bf16(const float *x, uint16_t *y, uint64_t n) {

@@ -64,10 +64,27 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   items.insert(items.end(), light.begin(), light.end());
   items.insert(items.end(), thins.begin(), thins.end());
   items.insert(items.end(), duals.begin(), duals.end());
-  for (void *p : {(void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb})
+  // long rows' partials in groups of kGroup (rows with more than kGroupMin)
+  std::vector<uint2> grp;
+  std::vector<uint32_t> hgb{0};
+  for (size_t h = 0; h + 1 < hpb.size(); ++h) {
+    if (hpb[h + 1] - hpb[h] > kGroupMin)
+      for (uint32_t q = hpb[h]; q < hpb[h + 1]; q += kGroup)
+        grp.push_back(make_uint2(q, std::min(kGroup, hpb[h + 1] - q)));
+    hgb.push_back((uint32_t)grp.size());
+  }
+  for (void *p : {(void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb, (void *)d.mix_grp,
+                  (void *)d.mix_heavy_gb})
     if (p) CTC_HIP(hipFree(p));
   d.mix_items = nullptr;
-  d.mix_heavy_slot = d.mix_heavy_pb = nullptr;
+  d.mix_heavy_slot = d.mix_heavy_pb = d.mix_heavy_gb = nullptr;
+  d.mix_grp = nullptr;
+  CTC_HIP(hipMalloc(&d.mix_grp, std::max<size_t>(1, grp.size()) * sizeof(uint2)));
+  CTC_HIP(hipMalloc(&d.mix_heavy_gb, hgb.size() * sizeof(uint32_t)));
+  if (!grp.empty())
+    CTC_HIP(hipMemcpyAsync(d.mix_grp, grp.data(), grp.size() * sizeof(uint2), hipMemcpyHostToDevice, m->ctx->stream));
+  CTC_HIP(hipMemcpyAsync(d.mix_heavy_gb, hgb.data(), hgb.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         m->ctx->stream));
   hipStream_t st = m->ctx->stream;
   CTC_HIP(hipMalloc(&d.mix_items, std::max<size_t>(1, items.size()) * sizeof(uint4)));
   CTC_HIP(hipMalloc(&d.mix_heavy_slot, std::max<size_t>(1, hslot.size()) * sizeof(uint32_t)));
@@ -86,6 +103,7 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   d.mix_ndual = duals.size();
   d.mix_nheavy = hslot.size();
   d.mix_nparts = nparts;
+  d.mix_ngroups = grp.size();
   return CTC_OK;
 }
 
@@ -95,7 +113,7 @@ ctc_status als_rows_prepare(ctc_model *m, const ctc_options *o, bool bf16) {
   for (uint32_t md = 0; md < m->nmodes; ++md) {
     CTC_TRY(ensure_ciss(m, md));
     CTC_TRY(ensure_mix_items(m, md, mix_chunk_of(o)));
-    need = std::max(need, std::max<size_t>(1, m->ciss[md].mix_nparts) *
+    need = std::max(need, std::max<size_t>(1, m->ciss[md].mix_nparts + m->ciss[md].mix_ngroups) *
                               mixed_partial_bytes((int)R, bf16 ? kPrecBF16 : kPrecF64));
     if (bf16 && !m->Fbf[md]) CTC_HIP(hipMalloc(&m->Fbf[md], m->dims[md] * R * sizeof(uint16_t)));
   }
@@ -112,7 +130,8 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   CTC_TRY(ensure_ciss(m, mode));
   CTC_TRY(ensure_mix_items(m, mode, chunk));
   DevCiss &d = m->ciss[mode];
-  CTC_TRY(ensure_scratch(m, std::max<size_t>(1, d.mix_nparts) * mixed_partial_bytes((int)R, prec)));
+  const size_t pbytes = mixed_partial_bytes((int)R, prec);
+  CTC_TRY(ensure_scratch(m, std::max<size_t>(1, d.mix_nparts + d.mix_ngroups) * pbytes));
   hipStream_t st = m->ctx->stream;
   MixedArgs a{};
   a.prec = prec;
@@ -140,6 +159,10 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
   a.nheavy = d.mix_nheavy;
+  a.gpart = (char *)m->scratch + (size_t)d.mix_nparts * pbytes;
+  a.grp = d.mix_grp;
+  a.ngroups = d.mix_ngroups;
+  a.heavy_gb = d.mix_heavy_gb;
   a.lam = o->lambda;
   a.refine = o->refine > 0 ? o->refine : 1;
   a.flag = m->dflag;
@@ -157,6 +180,10 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
     a.items = d.mix_items + nchunk;
     a.nitems = nall - nchunk;
     CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
+  }
+  if (a.ngroups) {
+    Prof p(m->ctx, bf16 ? "als_mixed_groups" : "als_rows_groups");
+    CTC_HIP(launch_mixed_groups(st, a, (int)R));
   }
   if (a.nheavy) {
     Prof p(m->ctx, bf16 ? "als_mixed_heavy" : "als_rows_heavy");

@@ -214,8 +214,10 @@ extern "C" ctc_status ctc_context_create(int device, ctc_context **out) {
   return CTC_OK;
 }
 
-extern "C" void ctc_context_destroy(ctc_context *c) {
-  if (!c) return;
+void context_retain(ctc_context *c) { ++c->refs; }
+
+void context_release(ctc_context *c) {
+  if (!c || --c->refs > 0) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto &p : c->pending) {
@@ -226,6 +228,8 @@ extern "C" void ctc_context_destroy(ctc_context *c) {
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+extern "C" void ctc_context_destroy(ctc_context *c) { context_release(c); }
 
 extern "C" ctc_status ctc_context_stream(ctc_context *c, void **stream) {
   if (!c || !stream) return fail(CTC_ERR_INVALID_ARG, "ctc_context_stream");
@@ -384,6 +388,8 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(d.mix_items);
     dev_free(d.mix_heavy_slot);
     dev_free(d.mix_heavy_pb);
+    dev_free(d.mix_grp);
+    dev_free(d.mix_heavy_gb);
     dev_free(m->Fbf[md]);
     dev_free(m->F[md]);
     dev_free(m->best[md]);
@@ -400,7 +406,9 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
   dev_free(m->dstat);
   dev_free(m->dflag);
   if (m->hstat) (void)hipHostFree(m->hstat);
+  ctc_context *ctx = m->ctx;
   delete m;
+  context_release(ctx);
 }
 
 extern "C" ctc_status ctc_model_create(ctc_context *ctx, const ctc_coo *train, const ctc_coo *validation,
@@ -419,6 +427,7 @@ extern "C" ctc_status ctc_model_create(ctc_context *ctx, const ctc_coo *train, c
   CTC_HIP(hipSetDevice(ctx->device));
   ctc_model *m = new ctc_model();
   m->ctx = ctx;
+  context_retain(ctx);
   m->nmodes = train->nmodes;
   m->rank = rank;
   m->train_nnz = train->nnz;
@@ -427,6 +436,7 @@ extern "C" ctc_status ctc_model_create(ctc_context *ctx, const ctc_coo *train, c
   ctc_status s = copy_coo(train, &tc);
   if (s != CTC_OK) {
     delete m;
+    context_release(ctx);
     return s;
   }
   m->train.reset(tc);

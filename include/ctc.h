@@ -159,6 +159,9 @@ void ctc_ciss_free(ctc_ciss *c);
 typedef struct ctc_context ctc_context;
 
 ctc_status ctc_context_create(int device, ctc_context **out);
+/* Releases the caller's handle.  Models and batches created on the context
+ * hold it too; it is freed when the last of them is destroyed, so the
+ * handles may be destroyed in any order. */
 void ctc_context_destroy(ctc_context *ctx);
 /* The context's hipStream_t. */
 ctc_status ctc_context_stream(ctc_context *ctx, void **stream);

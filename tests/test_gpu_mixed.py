@@ -65,6 +65,35 @@ def test_mixed_mode_update_matches_bf16_oracle(ctx, R, tile_len):
         m.set_factors(Fo)
 
 
+@pytest.mark.parametrize("R,prec", [(32, "mixed"), (128, "mixed"), (32, "fp32"), (128, "fp32")])
+def test_long_rows_reduced_in_groups(ctx, R, prec):
+    """Rows cut into more than 64 chunks have their chunk partials summed in
+    groups of 32 before the solve; mode 2 here has 4 rows of ~9000 nonzeros,
+    ~280 chunks of 32 each."""
+    tr, te = problem((2000, 1500, 4), 40000, seed=70 + R)
+    say(f"[long rows] R={R} {prec}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    rng = np.random.default_rng(R + 1)
+    F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in tr.dims]
+    m.set_factors(F)
+    ind, vals = tr.arrays()
+    oc = O.Coo(ind, vals, tr.dims)
+    if prec == "mixed":
+        opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, precision=ct.PREC_MIXED, tile_len=32)
+        Fo = [f.copy() for f in F]
+        assert O.als_mode_bf16(oc, 2, Fo, 0.05) == 0
+        tol = 1e-4
+    else:
+        opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, als_kernel=ct.ALS_ROWS, tile_len=32)
+        Fo = [f.astype(np.float64) for f in F]
+        assert O.als_mode(oc, 2, Fo, 0.05) == 0
+        tol = 1e-5
+    m.als_mode_update(2, opt)
+    e = relerr(m.factor(2), Fo[2].astype(np.float64))
+    say(f"  mode 2 rel err {e:.2e}")
+    assert e < tol, e
+
+
 def test_mixed_four_modes(ctx):
     tr, te = problem((120, 90, 30, 12), 30000, seed=61)
     say("[mixed 4-way] R=32")
