@@ -156,6 +156,22 @@ def test_als_config1_ten_iterations(ctx):
     assert worst < 1e-5, worst
 
 
+def test_context_handle_released_before_its_model():
+    """The context handle may be destroyed first: the model keeps the context
+    alive (reference count in the library) and still runs and frees cleanly."""
+    from cutc_amd import _lib
+    tr, te = small_problem(3, nnz=5000, seed=9)
+    say("[ctx_lifetime]")
+    c2 = ct.Context(0)
+    m = ct.Model(c2, tr, None, te, rank=16)
+    m.set_factors(init_factors(tr.dims, 16, 3))
+    _lib.lib().ctc_context_destroy(c2._h)  # the caller's handle goes first
+    c2._h = None
+    m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.05))
+    assert np.isfinite(m.evaluate(0.05)["rmse_te"])
+    del m
+
+
 def test_als_not_spd_is_reported(ctx):
     # lambda = 0 and rows with fewer nonzeros than R: singular normal systems
     tr, _ = ct.synth((500, 40, 30), 1500, test_fraction=0.0, seed=3)
