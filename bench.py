@@ -249,8 +249,8 @@ def main():
     ap.add_argument("--adjoint-lr", type=float, default=0.01)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              f"(the multi-GPU sharding is a design note, not an implementation).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solvers run on a single "
+              f"GPU and the multi-GPU sharding is described, not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
