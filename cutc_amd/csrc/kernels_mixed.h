@@ -55,6 +55,8 @@ struct MixedArgs {
 
 // items pass (light rows solved, heavy chunks to partials), then heavy rows
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R);
+// chunk items only (it.w != kLightRow): Gram into partials, no solve
+hipError_t launch_mixed_chunks(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, int R);
 hipError_t launch_mixed_groups(hipStream_t st, const MixedArgs &a, int R);
 hipError_t launch_rows_thin(hipStream_t st, const MixedArgs &a, int nmodes, int R);

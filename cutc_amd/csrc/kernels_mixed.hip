@@ -268,8 +268,8 @@ __device__ inline void gram_f64(const MixedArgs &a, uint32_t p0, uint32_t p1, f6
 
 // sums the per-thread m partials of the workgroup into sm.vm (FP64); ends
 // with a barrier
-template <int R, class T>
-__device__ inline void reduce_m(T *mp, MixedSmem<R> &sm) {
+template <int R, class T, class SM>
+__device__ inline void reduce_m(T *mp, SM &sm) {
   constexpr int NW = R / 32, FG8 = R / 8;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, fg = tid % FG8;
 #pragma unroll
@@ -987,19 +987,29 @@ __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
 }
 
 // ---------------------------------------------------------------- kernels --
-// Gram + MTTKRP of one item into the accumulators and sm.vm
-template <int R, int N>
-__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f32x16 *acc, MixedSmem<R> &sm) {
+// Gram + MTTKRP of one item into the accumulators and sm.vm (sm: MixedSmem,
+// or GramSmem for the chunk kernel; .tiles stages the h rows)
+template <int R, int N, class SM>
+__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f32x16 *acc, SM &sm) {
   float mp[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   gram_chunk<R, N>(a, it.y, it.z, acc, mp, (uint32_t *)sm.tiles);
   reduce_m<R>(mp, sm);
 }
-template <int R, int N>
-__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f64x4 *acc, MixedSmem<R> &sm) {
+template <int R, int N, class SM>
+__device__ inline void gram_item(const MixedArgs &a, const uint4 &it, f64x4 *acc, SM &sm) {
   double mp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   gram_f64<R, N>(a, it.y, it.z, acc, mp, sm.tiles);
   reduce_m<R>(mp, sm);
 }
+
+// the Gram phase's shared memory alone (chunks of long rows: no solve)
+template <int R, int P>
+struct GramSmem {
+  static constexpr size_t STG = P == kPrecBF16 ? (size_t)R * kHTS * 4 : (size_t)32 * MixedSmem<R>::HS * 8;
+  double tiles[STG / 8];
+  double vm[R];
+  double contrib[R / 32][R];
+};
 
 // accumulator slots of one wave and their place in a partial
 template <int R, int P>
@@ -1046,6 +1056,32 @@ __global__ void __launch_bounds__(2 * R, 2) k_mixed_items(MixedArgs a) {
   }
   const bool ok = solve_row<R>(sm, acc, a.lam, a.refine);
   store_row<R>(a, sm, it.x, ok);
+}
+
+// chunks of long rows only: Gram + MTTKRP into the chunk's partial.  Without
+// the solve's 79 KB of LDS tiles and its registers, several more workgroups
+// fit on a CU, and the chunk Gram is bound by how many row gathers are in
+// flight.
+template <int R, int N, int P>
+__global__ void __launch_bounds__(2 * R) k_mixed_chunks(MixedArgs a) {
+  typedef AccShape<R, P> S;
+  __shared__ GramSmem<R, P> sm;
+  const uint4 it = a.items[blockIdx.x];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  typename S::T acc[S::NSLOT];
+#pragma unroll
+  for (int j = 0; j < S::NSLOT; ++j)
+#pragma unroll
+    for (int v = 0; v < S::NV; ++v) acc[j][v] = 0;
+  gram_item<R, N>(a, it, acc, sm);
+  typename S::E *Pp = (typename S::E *)a.part + (size_t)it.w * S::PSZ;
+#pragma unroll
+  for (int j = 0; j < S::NSLOT; ++j) {
+    if (!S::used(w, j)) continue;
+#pragma unroll
+    for (int v = 0; v < S::NV; ++v) Pp[((w * S::NSLOT + j) * S::NV + v) * 64 + l] = acc[j][v];
+  }
+  if (tid < R) Pp[S::PSZ - R + tid] = (typename S::E)sm.vm[tid];
 }
 
 template <int R, int N, int P>
@@ -1144,6 +1180,12 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *x, uint16_t *y, ui
 hipError_t launch_mixed_items(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
   if (a.nitems == 0) return hipSuccess;
   CTC_ROW_SWITCH(k_mixed_items, a.nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_chunks(hipStream_t st, const MixedArgs &a, int nmodes, int R) {
+  if (a.nitems == 0) return hipSuccess;
+  CTC_ROW_SWITCH(k_mixed_chunks, a.nitems);
   return hipGetLastError();
 }
 

@@ -173,7 +173,7 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   if (nchunk) {
     Prof p(m->ctx, bf16 ? "als_mixed_chunks" : "als_rows_chunks");
     a.nitems = nchunk;
-    CTC_HIP(launch_mixed_items(st, a, (int)N, (int)R));
+    CTC_HIP(launch_mixed_chunks(st, a, (int)N, (int)R));
   }
   if (nall > nchunk) {
     Prof p(m->ctx, bf16 ? "als_mixed_items" : "als_rows_items");
