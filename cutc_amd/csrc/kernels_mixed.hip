@@ -783,10 +783,13 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
   wave_sync();
   // 3. K = L D L^T in place: after step k, K[i][j] (i > j) = l_ij d_j and
   // K[j][j] = d_j.  Lane l updates column c = l % 32 for rows of its half.
+  // The padding (rows and columns >= n) is the identity with no coupling, so
+  // steps k >= n - 1 change nothing: the elimination and both solves stop at
+  // the row's own size.
   {
     const int c = l & 31, half = l >> 5;
     bool ok = true;
-    for (int k = 0; k < nt - 1; ++k) {
+    for (int k = 0; k < n - 1; ++k) {
       const double dk = sm.K[k * TS::KS + k];
       ok = ok && dk > 0.0;
       const double lck = (c < NT ? sm.K[c * TS::KS + k] : 0.0) * rcp_f64(dk > 0.0 ? dk : 1.0);
@@ -803,7 +806,7 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
       }
       wave_sync();
     }
-    ok = ok && sm.K[(nt - 1) * TS::KS + nt - 1] > 0.0;
+    ok = ok && sm.K[(n - 1) * TS::KS + n - 1] > 0.0;
     if (!ok && l == 0) sm.bad = 1;
   }
   wave_sync();
@@ -811,12 +814,12 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
   double xi = l < nt ? sm.x[l] : 0.0;
   const double di = l < nt ? sm.K[l * TS::KS + l] : 1.0;
   const double rdi = rcp_f64(di > 0.0 ? di : 1.0);
-  for (int j = 0; j < nt - 1; ++j) {  // forward, unit lower L: l_ij = K[i][j] / d_j
+  for (int j = 0; j < n - 1; ++j) {  // forward, unit lower L: l_ij = K[i][j] / d_j
     const double yj = __shfl(xi * rdi, j, 64);  // y_j / d_j
     if (l > j && l < nt) xi = __builtin_fma(-sm.K[l * TS::KS + j], yj, xi);
   }
   xi *= rdi;
-  for (int j = nt - 1; j > 0; --j) {  // backward, L^T: l_ji = K[j][i] / d_i
+  for (int j = n - 1; j > 0; --j) {  // backward, L^T: l_ji = K[j][i] / d_i
     const double zj = __shfl(xi, j, 64);
     if (l < j) xi = __builtin_fma(-sm.K[j * TS::KS + l] * rdi, zj, xi);
   }
