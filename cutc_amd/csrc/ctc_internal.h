@@ -76,7 +76,7 @@ struct DevCiss {
   uint32_t *slot_gb = nullptr;    // dlength + 1: groups of slot s (empty for light slots)
   // BF16-operand ALS work items (see kernels_mixed.h) for rows cut at mix_chunk nonzeros
   uint32_t mix_chunk = 0;
-  uint64_t mix_nitems = 0, mix_nheavy = 0, mix_nparts = 0, mix_nthin = 0, mix_nthin16 = 0, mix_ndual = 0,
+  uint64_t mix_nitems = 0, mix_nheavy = 0, mix_nparts = 0, mix_nthin = 0, mix_nthin16 = 0, mix_ndual = 0, mix_ndual48 = 0,
            mix_ngroups = 0;
   uint4 *mix_items = nullptr;
   uint32_t *mix_heavy_slot = nullptr, *mix_heavy_pb = nullptr, *mix_heavy_gb = nullptr;

@@ -37,6 +37,7 @@ struct MixedArgs {
   uint64_t nthin16;   // the first nthin16 of them have n <= 16
   const uint4 *dual;  // rows {slot, first, end, -} with kThinRow < n <= kDualRow (R >= 64)
   uint64_t ndual;
+  uint64_t ndual48;   // the first ndual48 of them have n <= 48
   void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each
   const uint32_t *heavy_slot;  // heavy rows: directory slot
   const uint32_t *heavy_pb;    // heavy rows: partials [heavy_pb[h], heavy_pb[h + 1])

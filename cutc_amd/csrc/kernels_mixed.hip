@@ -849,14 +849,14 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
 // column (the forward solve folded in), the back solve from the owning lanes'
 // registers, a = H^T z.  LDS is the h stage plus 3 KB; the 64 FP64 column
 // entries per lane hold the kernel to one wave per SIMD.
-template <int R, int P>
+template <int R, int P, int NT>
 struct DualSmem {
   typedef typename std::conditional<P == kPrecBF16, uint16_t, double>::type HT;
   static constexpr int HS = R + 1;
-  HT H[kDualRow * HS];
-  double T[16 * 17];           // one K tile on its way to the column owners
-  double pub[kDualRow + 2];    // column k and z_k / x_k
-  double z[kDualRow];
+  HT H[NT * HS];
+  double T[16 * 17];     // one K tile on its way to the column owners
+  double pub[NT + 2];    // column k and z_k / x_k
+  double z[NT];
   __device__ static double hval(HT v) {
     if constexpr (P == kPrecBF16)
       return (double)__uint_as_float((uint32_t)v << 16);
@@ -865,13 +865,13 @@ struct DualSmem {
   }
 };
 
-template <int R, int N, int P>
-__global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
-  static_assert(R >= kDualRow, "the dual form pays only for n < R");
-  constexpr int NT = kDualRow;
-  typedef DualSmem<R, P> DS;
+// NT = 48 serves rows of at most 48 nonzeros, NT = 64 the rest
+template <int R, int N, int P, int NT>
+__global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a, uint64_t base) {
+  static_assert(R >= kDualRow && NT % 16 == 0 && NT <= 64, "dual systems of 48 or 64 rows");
+  typedef DualSmem<R, P, NT> DS;
   __shared__ DS sm;
-  const uint4 it = a.dual[blockIdx.x];
+  const uint4 it = a.dual[base + blockIdx.x];
   const int l = threadIdx.x;
   const int n = (int)(it.z - it.y);
   // 1. stage h (rows >= n zero)
@@ -880,6 +880,7 @@ __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
 #pragma unroll
   for (int s = 0; s < N - 1; ++s) myid[s] = l < n ? a.oidx[s][it.y + l] : 0u;
   const double xl = l < n ? (double)a.vals[it.y + l] : 0.0;
+  const int lc = l < NT ? l : NT - 1;  // lanes past NT own no column
 #pragma unroll 8
   for (int p = 0; p < NT; ++p) {
     uint32_t id[N - 1];
@@ -932,7 +933,7 @@ __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
         sm.T[row16(v, l) * 17 + (l & 15)] = pad ? (r == c ? 1.0 : 0.0) : acc[v] + (r == c ? lam : 0.0);
       }
       wave_sync();
-      if ((l >> 4) == bj) {
+      if ((l >> 4) == bj) {  // (lanes >= NT never match: bj < NT / 16)
 #pragma unroll
         for (int r = 0; r < 16; ++r) g[16 * bi + r] = sm.T[r * 17 + (l & 15)];
       }
@@ -958,7 +959,7 @@ __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
     }
     const double rk = rcp_f64(d);
     const double zk = buf[NT];
-    const double lck = l > k ? buf[l] * rk : 0.0;
+    const double lck = (l > k && l < NT) ? buf[lc] * rk : 0.0;
 #pragma unroll
     for (int i = k + 1; i < NT; ++i) g[i] = __builtin_fma(-buf[i], lck, g[i]);
     b = __builtin_fma(-lck, zk, b);
@@ -975,7 +976,7 @@ __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a) {
     b = __builtin_fma(-lkc, xk, b);
     wave_sync();
   }
-  sm.z[l] = b;
+  if (l < NT) sm.z[l] = b;
   wave_sync();
   // 4. a = H^T z
   float *dst = a.out + (size_t)a.directory[it.x] * R;
@@ -1211,12 +1212,17 @@ hipError_t launch_mixed_heavy(hipStream_t st, const MixedArgs &a, int nmodes, in
     }                                                                                                             \
   } while (0)
 
-#define CTC_DUAL_LAUNCH(RR, NN)                                                                        \
-  do {                                                                                                \
-    if (a.prec == kPrecBF16)                                                                          \
-      hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecBF16>), dim3((unsigned)a.ndual), dim3(64), 0, st, a); \
-    else                                                                                              \
-      hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecF64>), dim3((unsigned)a.ndual), dim3(64), 0, st, a);  \
+// rows of at most 48 nonzeros first (NT = 48), then the rest (NT = 64)
+#define CTC_DUAL_LAUNCH(RR, NN)                                                                                   \
+  do {                                                                                                           \
+    const uint64_t n48 = a.ndual48, n64 = a.ndual - a.ndual48;                                                    \
+    if (a.prec == kPrecBF16) {                                                                                   \
+      if (n48) hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecBF16, 48>), dim3((unsigned)n48), dim3(64), 0, st, a, 0ull); \
+      if (n64) hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecBF16, 64>), dim3((unsigned)n64), dim3(64), 0, st, a, n48);  \
+    } else {                                                                                                     \
+      if (n48) hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecF64, 48>), dim3((unsigned)n48), dim3(64), 0, st, a, 0ull);  \
+      if (n64) hipLaunchKernelGGL((k_rows_dual<RR, NN, kPrecF64, 64>), dim3((unsigned)n64), dim3(64), 0, st, a, n48);   \
+    }                                                                                                            \
   } while (0)
 
 hipError_t launch_rows_dual(hipStream_t st, const MixedArgs &a, int nmodes, int R) {

@@ -30,7 +30,7 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   DevCiss &d = m->ciss[mode];
   if (d.mix_items && d.mix_chunk == chunk) return CTC_OK;
   const uint32_t thin = std::min<uint32_t>((uint32_t)kThinRow, m->rank);
-  std::vector<uint4> chunks, light, thins, thins32, duals;
+  std::vector<uint4> chunks, light, thins, thins32, duals, duals64;
   // dual 64 x 64 systems for rows of 33..64 nonzeros pay at R = 128 (at R = 64
   // the primal 4 x 4-tile solve measured faster)
   const uint32_t dual = m->rank >= 2 * (uint32_t)kDualRow ? (uint32_t)kDualRow : 0u;
@@ -42,8 +42,8 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
       (p1 - p0 <= 16 ? thins : thins32).push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
       continue;
     }
-    if (p1 - p0 <= dual) {
-      duals.push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
+    if (p1 - p0 <= dual) {  // 48-row systems first, then 64-row ones
+      (p1 - p0 <= 48 ? duals : duals64).push_back(make_uint4((uint32_t)s, p0, p1, kLightRow));
       continue;
     }
     if (p1 - p0 <= chunk) {
@@ -58,6 +58,8 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   // thin rows (dual-form kernel) last
   const size_t nthin16 = thins.size();
   thins.insert(thins.end(), thins32.begin(), thins32.end());
+  const size_t ndual48 = duals.size();
+  duals.insert(duals.end(), duals64.begin(), duals64.end());
   std::vector<uint4> items;
   items.reserve(chunks.size() + light.size() + thins.size());
   items.insert(items.end(), chunks.begin(), chunks.end());
@@ -101,6 +103,7 @@ static ctc_status ensure_mix_items(ctc_model *m, uint32_t mode, uint32_t chunk) 
   d.mix_nthin = thins.size();
   d.mix_nthin16 = nthin16;
   d.mix_ndual = duals.size();
+  d.mix_ndual48 = ndual48;
   d.mix_nheavy = hslot.size();
   d.mix_nparts = nparts;
   d.mix_ngroups = grp.size();
@@ -155,6 +158,7 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.nthin16 = d.mix_nthin16;
   a.dual = a.thin + a.nthin;
   a.ndual = d.mix_ndual;
+  a.ndual48 = d.mix_ndual48;
   a.part = m->scratch;
   a.heavy_slot = d.mix_heavy_slot;
   a.heavy_pb = d.mix_heavy_pb;
