@@ -14,7 +14,7 @@ namespace ctc {
 constexpr uint32_t kLightRow = 0xFFFFFFFFu;
 // rows with at most this many nonzeros are solved in the dual (n x n) form
 constexpr int kThinRow = 32;
-// rows with kThinRow < n <= kDualRow nonzeros at R >= 64 (dual form, 64 x 64)
+// rows with kThinRow < n <= kDualRow nonzeros at R = 128 (dual form, 48 or 64 rows)
 constexpr int kDualRow = 64;
 constexpr uint32_t kGroup = 32, kGroupMin = 64;
 
@@ -35,7 +35,7 @@ struct MixedArgs {
   const uint4 *thin;  // thin rows {slot, first, end, -}: n <= min(kThinRow, R)
   uint64_t nthin;
   uint64_t nthin16;   // the first nthin16 of them have n <= 16
-  const uint4 *dual;  // rows {slot, first, end, -} with kThinRow < n <= kDualRow (R >= 64)
+  const uint4 *dual;  // rows {slot, first, end, -} with kThinRow < n <= kDualRow (R = 128)
   uint64_t ndual;
   uint64_t ndual48;   // the first ndual48 of them have n <= 48
   void *part;                  // partials of chunks, mixed_partial_bytes(R, prec) each

@@ -838,7 +838,7 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
 }
 
 // ------------------------------------------------------------- dual rows --
-// Rows with kThinRow < n <= kDualRow nonzeros at R >= 64, also in the dual
+// Rows with kThinRow < n <= kDualRow nonzeros at R = 128, also in the dual
 // form a = H^T (H H^T + lambda I)^-1 x, as an n x n system padded to 64:
 // cheaper than the R x R primal solve (64^3 / 3 instead of R^3 / 3 FP64
 // flops, one wave instead of a barrier-separated blocked Cholesky).  One wave
@@ -868,6 +868,7 @@ struct DualSmem {
 // NT = 48 serves rows of at most 48 nonzeros, NT = 64 the rest
 template <int R, int N, int P, int NT>
 __global__ void __launch_bounds__(64) k_rows_dual(MixedArgs a, uint64_t base) {
+  // one lane owns one column: at most 64 columns per wave
   static_assert(R >= kDualRow && NT % 16 == 0 && NT <= 64, "dual systems of 48 or 64 rows");
   typedef DualSmem<R, P, NT> DS;
   __shared__ DS sm;

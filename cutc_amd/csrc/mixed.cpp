@@ -8,7 +8,7 @@
 // `chunk` nonzeros in one workgroup and writes partial systems for the chunks
 // of longer rows, the heavy pass sums each long row's partials and solves it,
 // the thin pass solves the rows of at most min(32, R) nonzeros in their dual
-// n x n form, one wave each, and at R >= 64 the dual pass does the same for
+// n x n form, one wave each, and at R = 128 the dual pass does the same for
 // rows of 33..64 nonzeros (64 x 64 systems).
 #include <algorithm>
 #include <vector>
