@@ -839,7 +839,7 @@ __global__ void __launch_bounds__(64) k_rows_thin(MixedArgs a, uint64_t base) {
 
 // ------------------------------------------------------------- dual rows --
 // Rows with kThinRow < n <= kDualRow nonzeros at R = 128, also in the dual
-// form a = H^T (H H^T + lambda I)^-1 x, as an n x n system padded to 64:
+// form a = H^T (H H^T + lambda I)^-1 x, as an n x n system padded to 48 or 64:
 // cheaper than the R x R primal solve (64^3 / 3 instead of R^3 / 3 FP64
 // flops, one wave instead of a barrier-separated blocked Cholesky).  One wave
 // per row: h staged in LDS (BF16 bits), K = H H^T tile by tile on

@@ -9,7 +9,7 @@
 // of longer rows, the heavy pass sums each long row's partials and solves it,
 // the thin pass solves the rows of at most min(32, R) nonzeros in their dual
 // n x n form, one wave each, and at R = 128 the dual pass does the same for
-// rows of 33..64 nonzeros (64 x 64 systems).
+// rows of 33..64 nonzeros (systems padded to 48 or 64).
 #include <algorithm>
 #include <vector>
 
