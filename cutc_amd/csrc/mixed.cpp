@@ -168,7 +168,10 @@ ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool
   a.ngroups = d.mix_ngroups;
   a.heavy_gb = d.mix_heavy_gb;
   a.lam = o->lambda;
-  a.refine = o->refine > 0 ? o->refine : 1;
+  // the BF16 path always refines once against its FP32 accumulators; the FP32
+  // path's Gram is accumulated in FP64, so it refines only when asked (as the
+  // span kernels do)
+  a.refine = bf16 ? (o->refine > 0 ? o->refine : 1) : (o->refine > 0 ? o->refine : 0);
   a.flag = m->dflag;
   CTC_HIP(launch_fill(st, m->F[mode], m->dims[mode] * R, 0.f));
   // chunks of long rows and whole rows are the same kernel, launched apart so
