@@ -635,18 +635,34 @@ __device__ inline bool solve_row(MixedSmem<R> &sm, const Acc *acc, double lam, i
   if (tid < R) sm.vy[tid] = sm.vm[tid];
   if (tid == 0) sm.bad = 0;
   __syncthreads();
-  // blocked right-looking Cholesky; vy becomes y = L^-1 m on the way
+  // blocked right-looking Cholesky with one step of lookahead; vy becomes
+  // y = L^-1 m on the way.  After the panels of step K, the wave that owns
+  // diagonal block K+1 updates that tile and factors it at once, while the
+  // other waves apply the rest of step K's trailing updates: the serial
+  // diagonal factorisations overlap the trailing work instead of following it.
+  if (w == 0) diag_step(sm, 0, true);
+  __syncthreads();
   for (int K = 0; K < RT; ++K) {
-    if (w == K % NW) diag_step(sm, K, true);
-    __syncthreads();
     for (int I = K + 1; I < RT; ++I)
       if (w == I % NW) panel_step(sm, I, K);
     __syncthreads();
     if (K + 1 < RT) {
+      const int dw = (K + 1) % NW;  // owner of diagonal block K+1
+      if (w == dw) {
+        trail_step(sm, K + 1, K + 1, K);
+        diag_step(sm, K + 1, true);
+      }
+      // the other tiles of step K, dealt round-robin to the other waves (all
+      // waves when there is only one)
+      const int nother = NW > 1 ? NW - 1 : 1;
+      const int me = NW > 1 ? (w - dw - 1 + NW) % NW : 0;  // rank among the other waves
       int u = 0;
       for (int I = K + 1; I < RT; ++I)
-        for (int J = K + 1; J <= I; ++J, ++u)
-          if (u % NW == w) trail_step(sm, I, J, K);
+        for (int J = K + 1; J <= I; ++J) {
+          if (I == K + 1 && J == K + 1) continue;
+          if ((NW == 1 || w != dw) && u % nother == me) trail_step(sm, I, J, K);
+          ++u;
+        }
       __syncthreads();
     }
   }
