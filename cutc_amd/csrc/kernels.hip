@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(64) k_als_solve(const double *pieces, const ui
     __syncthreads();
   }
 
-  auto solve = [&](double *y) {
+  auto solve = [&](double *y) __attribute__((always_inline)) {
     for (int k = 0; k < R; ++k) {  // L y = b
       const double yk = y[k] / Gs[k * LD + k];
       __syncthreads();

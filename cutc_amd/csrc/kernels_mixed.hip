@@ -294,16 +294,19 @@ __device__ inline void reduce_m(T *mp, SM &sm) {
 // lower factor; with fold, also y_K = inverse * vy_K in place.
 // The factorisation eliminates in place in LDS without square roots (after
 // step k, a_ij for i > j holds l_ij d_j and a_jj the pivot d_j of A = L D L^T),
-// so each of its 16 dependent steps is one LDS round trip and one reciprocal;
-// the Cholesky factor C = L D^1/2 is formed at the end.  The inverse of C is
-// then formed column by column, one lane per column, by forward substitution.
+// so each of its 15 dependent steps is one LDS round trip and one reciprocal.
+// The same elimination builds M = L^-1 in the tile's otherwise unused upper
+// triangle, transposed (entry (j, r), j < r, holds m_rj): step k subtracts
+// l_rk times row k of M (final by then) from every row r > k, as Gauss-Jordan
+// does on an identity.  The inverse of the Cholesky factor C = L D^1/2 is then
+// D^-1/2 M, one scaling pass.
 template <int R>
-__device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
+__device__ __attribute__((always_inline)) inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
   double *T = sm.tiles + tileno(K, K) * 256;
   const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  double a[4];
+  double a[4];  // entries (4q + e, c): lower part of A, upper part m_c,4q+e (starts at 0)
 #pragma unroll
-  for (int e = 0; e < 4; ++e) a[e] = T[tix(4 * q + e, c)];
+  for (int e = 0; e < 4; ++e) a[e] = 4 * q + e >= c ? T[tix(4 * q + e, c)] : 0.0;
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < 15; ++k) {
@@ -311,6 +314,7 @@ __device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
     ok = ok && (akk > 0.0);
     const double rk = rcp_f64(akk > 0.0 ? akk : 1.0);
     const double ack = T[tix(c, k)] * rk;  // l_ck
+    // column k: a_ik below the pivot, m_ki (row k of M, final) above it
     double aik[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) aik[e] = T[tix(4 * q + e, k)];
@@ -318,44 +322,27 @@ __device__ inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int i = 4 * q + e;
-      if (i >= c && c > k) {
-        a[e] = __builtin_fma(-aik[e], ack, a[e]);
+      if (c > k && (i >= c || i <= k)) {
+        // lower: a_ic -= a_ik l_ck; upper: m_ci -= l_ck m_ki (m_kk = 1)
+        a[e] = __builtin_fma(-(i == k ? 1.0 : aik[e]), ack, a[e]);
         T[tix(i, c)] = a[e];
       }
     }
     wave_sync();
   }
   ok = ok && (T[tix(15, 15)] > 0.0);
-  // C = L D^1/2: c_ic = a_ic / sqrt(d_c), c_cc = sqrt(d_c)
-  const double dc = T[tix(c, c)];
-  const double sdc = sqrt(dc > 0.0 ? dc : 1.0), rsd = rcp_f64(sdc);
-  wave_sync();
+  // X = C^-1 = D^-1/2 M: x_ic = m_ic / sqrt(d_i) below the diagonal, 1 / sqrt(d_i) on it
+  double x[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int i = 4 * q + e;
-    T[tix(i, c)] = i > c ? a[e] * rsd : (i == c ? sdc : 0.0);
+    const double di = T[tix(i, i)];
+    const double m = i > c ? T[tix(c, i)] : (i == c ? 1.0 : 0.0);
+    x[e] = m * rcp_f64(sqrt(di > 0.0 ? di : 1.0));
   }
   wave_sync();
-  // X = C^-1, lane c < 16 forms column c: x_i = (delta_ic - sum_{j<i} c_ij x_j) / c_ii
-  double x[16];
-  if (l < 16) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < i; ++j) {
-        if (j & 1)
-          s1 = __builtin_fma(-T[tix(i, j)], x[j], s1);
-        else
-          s0 = __builtin_fma(-T[tix(i, j)], x[j], s0);
-      }
-      x[i] = (s0 + s1) * rcp_f64(T[tix(i, i)]);
-    }
-  }
-  wave_sync();
-  if (l < 16)
-#pragma unroll
-    for (int i = 0; i < 16; ++i) T[tix(i, c)] = i >= c ? x[i] : 0.0;
+  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = x[e];
   if (!ok && l == 0) sm.bad = 1;
   wave_sync();
   if (fold) {
