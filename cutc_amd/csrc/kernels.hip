@@ -473,8 +473,8 @@ __device__ __forceinline__ void als_gram_walk(const CissArgs &c, const FactorArg
 }
 
 // Occupancy: at R <= 32 the walk (with its one-step gather lookahead) is held
-// to 6 waves per SIMD, about 80 registers; at R = 64 it keeps the compiler's
-// default.  Measured on the 100M-nonzero tensor at R = 32: 4.56 ms per mode
+// to 6 waves per SIMD, about 80 registers; at R = 64, and at R = 32 with four
+// modes (which would spill at 80), it keeps the compiler's default.  Measured on the 100M-nonzero tensor at R = 32: 4.56 ms per mode
 // without lookahead at the default occupancy, 4.27 ms without lookahead at 7
 // waves, 4.00 ms with lookahead at 6.
 template <int R, int N>
@@ -993,7 +993,13 @@ hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &
   switch (R) {
     case 8: CTC_GRAM(8); break;
     case 16: CTC_GRAM(16); break;
-    case 32: CTC_GRAM(32); break;
+    case 32:
+      if (nmodes == 3)
+        CTC_GRAM(32);
+      else  // four modes need more than the 80 registers of 6 waves per SIMD (it would spill)
+        hipLaunchKernelGGL((k_als_gram_wide<32, 4>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans,
+                           span_off, pieces);
+      break;
     case 64:
       if (nmodes == 3)
         hipLaunchKernelGGL((k_als_gram_wide<64, 3>), dim3(grid), dim3(block), 0, st, c, f, span_tiles, nspans,

@@ -110,6 +110,26 @@ def test_als_mode_update(ctx, R, kernel, tile_len):
         m.set_factors([f.astype(np.float32) for f in F64])
 
 
+@pytest.mark.parametrize("R,kernel", [(R, ct.ALS_SPANS) for R in (8, 16, 32, 64)] +
+                         [(R, ct.ALS_ROWS) for R in (32, 64)])
+def test_als_mode_update_four_modes(ctx, R, kernel):
+    tr, te = small_problem(4, nnz=30000, seed=40 + R)
+    say(f"[als_mode4] R={R} kernel={kernel}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 6)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    F64 = [f.astype(np.float64) for f in F]
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, tile_len=256, als_kernel=kernel)
+    for mode in range(4):
+        m.als_mode_update(mode, opt)
+        assert O.als_mode(oc, mode, F64, 0.05) == 0
+        e = relerr(m.factor(mode), F64[mode])
+        assert e < 1e-5, (mode, e)
+        F64 = [f.astype(np.float32).astype(np.float64) for f in F64]
+        m.set_factors([f.astype(np.float32) for f in F64])
+
+
 def test_als_deterministic(ctx):
     tr, te = small_problem(3, nnz=30000, seed=77)
     say("[als_deterministic]")
