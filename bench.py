@@ -446,6 +446,30 @@ def main():
         log(f"batch ALS: {a.batch_iters} iters of {len(probs)} problems in {ms_graph:.1f} ms (graph), "
             f"{ms_eager:.1f} ms (direct); {bn * a.batch_iters / (ms_graph * 1e-3) / 1e9:.3f} Gnnz/s; "
             f"median rmse_te {np.median(rte):.4f}")
+        # the same run in mixed precision: ranks 32 and 64 on the BF16-operand
+        # row path (tuned for it), ranks 8 and 16 on the FP32 kernels; the FP32
+        # run above (factors within 1e-5 of the FP64 update) is the reference
+        mopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam, precision=ct.PREC_MIXED)
+        load()
+        mtuned = batch.autotune(mopt)
+        load()
+        batch.als(mopt, iters=a.batch_iters, graph=True)
+        mreps = []
+        for _ in range(3):
+            load()
+            mreps.append(batch.als(mopt, iters=a.batch_iters, graph=True))
+        ms_mixed = float(np.median(mreps))
+        _, mrte = batch.rmse()
+        gap = np.abs(mrte - rte)
+        out_algs["batch_als"]["mixed"] = {
+            "graph_ms": ms_mixed, "nnz_per_s": bn * a.batch_iters / (ms_mixed * 1e-3),
+            "speedup_vs_fp32": ms_graph / ms_mixed,
+            "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
+                      for (r, _), t in zip(batch.classes(), mtuned)],
+            "rmse_te_gap_vs_fp32_max": float(gap.max()), "rmse_te_gap_vs_fp32_median": float(np.median(gap))}
+        log(f"batch ALS mixed (ranks 32/64 BF16-operand): {ms_mixed:.1f} ms (graph), "
+            f"{ms_graph / ms_mixed:.2f}x the FP32 run; test RMSE gap to the FP32 run median {np.median(gap):.1e}, "
+            f"max {gap.max():.1e}; tuned {[(t['tile_len'], t['block']) for t in mtuned]}")
         del batch, probs, Fb
 
     # ----------------------------------------- adjoint (4-way, autograd)

@@ -239,9 +239,16 @@ extern "C" ctc_status ctc_batch_get_factor(ctc_batch *b, uint32_t problem, uint3
   return factor_io(b, problem, mode, host, true);
 }
 
+// the precision a class runs in: the BF16-operand path exists for the
+// row-fused kernels' ranks (32, 64, 128); other ranks keep the FP32 kernels
+static int32_t class_precision(uint32_t rank, int32_t asked) {
+  return asked == CTC_PREC_MIXED && (rank == 32 || rank == 64 || rank == 128) ? CTC_PREC_MIXED : CTC_PREC_FP32;
+}
+
 static ctc_options class_options(const RankClass &c, const ctc_options *o) {
   ctc_options t = *o;
   t.algorithm = CTC_ALS;
+  t.precision = class_precision(c.rank, o->precision);
   if (c.tuned) {
     t.tile_len = c.tile_len;
     t.block = c.block;
@@ -253,7 +260,7 @@ static ctc_options class_options(const RankClass &c, const ctc_options *o) {
 // that the sweep itself only launches kernels
 static ctc_status prepare_class(RankClass &c, const ctc_options *t) {
   ctc_model *m = c.model;
-  if (als_uses_rows(t, c.rank)) return als_rows_prepare(m, t, false);
+  if (als_uses_rows(t, c.rank)) return als_rows_prepare(m, t, t->precision == CTC_PREC_MIXED);
   const uint32_t S = span_tiles_of(t);
   size_t need = 0;
   for (uint32_t md = 0; md < m->nmodes; ++md) {
@@ -274,6 +281,7 @@ extern "C" ctc_status ctc_batch_autotune(ctc_batch *b, const ctc_options *o, ctc
     RankClass &c = b->cls[ci];
     ctc_options t = *o;
     t.algorithm = CTC_ALS;
+    t.precision = class_precision(c.rank, o->precision);
     ctc_tune_result r{};
     CTC_TRY(ctc_autotune(c.model, &t, &r));
     c.tuned = true;
@@ -327,6 +335,7 @@ static std::vector<uintptr_t> graph_key_of(ctc_batch *b, const ctc_options *o, u
       k.push_back((uintptr_t)d.mix_items);
       k.push_back((uintptr_t)d.mix_chunk);
       k.push_back((uintptr_t)c.model->F[md]);
+      k.push_back((uintptr_t)c.model->Fbf[md]);
     }
   }
   return k;
@@ -335,7 +344,8 @@ static std::vector<uintptr_t> graph_key_of(ctc_batch *b, const ctc_options *o, u
 extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t iters, int32_t use_graph,
                                     double *ms) {
   if (!b || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_batch_als");
-  if (o->precision != CTC_PREC_FP32) return fail(CTC_ERR_UNSUPPORTED, "batched ALS runs in FP32");
+  if (o->precision != CTC_PREC_FP32 && o->precision != CTC_PREC_MIXED)
+    return fail(CTC_ERR_INVALID_ARG, "ctc_batch_als: unknown precision %d", (int)o->precision);
   CTC_HIP(hipSetDevice(b->ctx->device));
   for (auto &c : b->cls) {
     const ctc_options t = class_options(c, o);

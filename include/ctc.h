@@ -395,12 +395,16 @@ ctc_status ctc_batch_class_rank(const ctc_batch *b, uint32_t cls, uint32_t *rank
  * max(train dims[mode], test dims[mode]) rows */
 ctc_status ctc_batch_set_factor(ctc_batch *b, uint32_t problem, uint32_t mode, const float *host);
 ctc_status ctc_batch_get_factor(ctc_batch *b, uint32_t problem, uint32_t mode, float *host);
-/* Measured (tile_len, block) per rank class for an ALS sweep (ctc_autotune on
- * each class); results has room for 4 entries.  Later ctc_batch_als calls use
+/* Measured (tile_len, block) per rank class for an ALS sweep in the precision
+ * each class runs under opt (ctc_autotune on each class); results has room
+ * for 4 entries.  Later ctc_batch_als calls use
  * each class's tuned pair in place of opt's.  Factors are left unchanged. */
 ctc_status ctc_batch_autotune(ctc_batch *b, const ctc_options *opt, ctc_tune_result *results,
                               uint32_t *nclasses);
-/* `iters` ALS iterations (modes 0..N-1 in turn) on every problem, FP32.
+/* `iters` ALS iterations (modes 0..N-1 in turn) on every problem.  With
+ * opt->precision = CTC_PREC_MIXED the classes of rank 32, 64 and 128 run the
+ * BF16-operand row-fused path (FP64 solve, one refinement) and the other
+ * ranks the FP32 kernels; CTC_PREC_FP32 runs every class in FP32.
  * use_graph != 0: the run is captured once into a hipGraph (one branch per
  * rank class, forked from and joined to the context's stream) and replayed
  * by later calls with the same iters and options; 0: launched directly.

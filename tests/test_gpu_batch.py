@@ -4,7 +4,8 @@ on each problem alone.
 
 Tolerances: every factor of every problem within 1e-5 relative (Frobenius)
 of the oracle after each run; per-problem RMSE within 1e-6 relative of the
-oracle's; graph replays bit-identical to the direct launches.
+oracle's; graph replays bit-identical to the direct launches.  The mixed-precision
+batch has its own bars (its test's docstring).
 """
 import sys
 
@@ -126,6 +127,57 @@ def test_batch_autotune_keeps_results(ctx):
             worst = max(worst, relerr(b.factor(p, m), F64[m]))
     say(f"  tuned {[(r['tile_len'], r['block']) for r in res]} worst {worst:.2e}")
     assert worst < 1e-5
+
+
+def test_batch_mixed_precision(ctx):
+    """CTC_PREC_MIXED on a batch: the rank-32 and rank-64 classes run the
+    BF16-operand row path, ranks 8 and 16 keep the FP32 kernels.  After one
+    sweep the FP32 classes' factors are bit-identical to an FP32 batch run and
+    each mode update of the mixed classes matches the BF16-operand oracle run
+    from that update's own inputs (the factors the sweep had produced so far)
+    to 1e-4 relative, the bar of the single-model test.  After four sweeps
+    every problem's test RMSE is within 1e-2 relative of the FP64 oracle's: a
+    sanity bound, not the precision target.  These are small problems at rank
+    32 and 64 with raw values and lambda 0.05 (test RMSE 4 to 10, far from
+    converged), where the BF16 rounding of the operands moves the test RMSE by
+    up to 2e-3 relative (measured worst, rank 64)."""
+    probs = make_problems(8, seed=6)
+    say(f"[batch_mixed] ranks {[p[2] for p in probs]}")
+    b = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    fp = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM)
+    mx = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM, precision=ct.PREC_MIXED)
+    load(b, probs)
+    b.als(fp, iters=1, graph=False)
+    ref = [[b.factor(p, m) for m in range(3)] for p in range(len(probs))]
+    load(b, probs)
+    b.als(mx, iters=1, graph=True)
+    worst = 0.0
+    for p, (tr, te, R, F) in enumerate(probs):
+        if R < 32:
+            for m in range(3):
+                assert np.array_equal(b.factor(p, m), ref[p][m]), (p, m)
+            continue
+        ind, vals = tr.arrays()
+        oc = O.Coo(ind, vals, tr.dims)
+        Fo = [f.copy() for f in F]
+        for m in range(3):
+            assert O.als_mode_bf16(oc, m, Fo, LAM) == 0
+            got = b.factor(p, m)
+            worst = max(worst, relerr(got, Fo[m].astype(np.float64)))
+            Fo[m] = got  # the next mode update's input, as on the GPU
+    say(f"  one sweep: worst rel err vs BF16-operand oracle {worst:.2e}")
+    assert worst < 1e-4
+    load(b, probs)
+    b.als(mx, iters=4, graph=True)
+    _, rte = b.rmse()
+    gaps = []
+    for p, (tr, te, R, F) in enumerate(probs):
+        F64 = oracle_run(tr, F, 4)
+        ind, vals = te.arrays()
+        o = O.rmse(O.Coo(ind, vals, te.dims), F64)
+        gaps.append(abs(rte[p] - o) / o)
+    say(f"  four sweeps: relative test RMSE gap to the FP64 oracle per problem {[f'{g:.1e}' for g in gaps]}")
+    assert max(gaps) < 1e-2
 
 
 def test_batch_rejects_bad_input(ctx):
