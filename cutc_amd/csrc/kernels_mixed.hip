@@ -308,7 +308,9 @@ __device__ __attribute__((always_inline)) inline void diag_step(MixedSmem<R> &sm
 #pragma unroll
   for (int e = 0; e < 4; ++e) a[e] = 4 * q + e >= c ? T[tix(4 * q + e, c)] : 0.0;
   bool ok = true;
-#pragma unroll
+  // kept rolled: unrolled, each step's constant addresses stay live and the
+  // enclosing kernels lose a wave per SIMD to the extra registers
+#pragma unroll 1
   for (int k = 0; k < 15; ++k) {
     const double akk = T[tix(k, k)];
     ok = ok && (akk > 0.0);
