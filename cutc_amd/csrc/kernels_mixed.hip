@@ -292,14 +292,18 @@ __device__ inline void reduce_m(T *mp, SM &sm) {
 // ---------------------------------------------------------- solve phase --
 // wave-level: factor tile (K, K) (Cholesky), replace it by the inverse of its
 // lower factor; with fold, also y_K = inverse * vy_K in place.
-// The factorisation eliminates in place in LDS without square roots (after
-// step k, a_ij for i > j holds l_ij d_j and a_jj the pivot d_j of A = L D L^T),
-// so each of its 15 dependent steps is one LDS round trip and one reciprocal.
-// The same elimination builds M = L^-1 in the tile's otherwise unused upper
-// triangle, transposed (entry (j, r), j < r, holds m_rj): step k subtracts
-// l_rk times row k of M (final by then) from every row r > k, as Gauss-Jordan
-// does on an identity.  The inverse of the Cholesky factor C = L D^1/2 is then
-// D^-1/2 M, one scaling pass.
+// The factorisation runs without square roots (after step k, a_ij for i > j
+// holds l_ij d_j and a_jj the pivot d_j of A = L D L^T) on the tile held in
+// registers, lane (q, c) owning entries (4q .. 4q+3, c).  At step k the four
+// lanes owning column k publish it to a 16-double buffer in the tile's own LDS
+// slot (free while the tile is in registers; two buffers alternate, so one
+// wave barrier per step suffices) and every lane updates its entries from it:
+// each of the 15 dependent steps is one LDS round trip and one reciprocal.
+// The same elimination builds M = L^-1 in the otherwise unused upper triangle,
+// transposed (entry (j, r), j < r, holds m_rj): step k subtracts l_rk times
+// row k of M (final by then, and published with column k) from every row
+// r > k, as Gauss-Jordan does on an identity.  The inverse of the Cholesky
+// factor C = L D^1/2 is then D^-1/2 M, one scaling pass.
 template <int R>
 __device__ __attribute__((always_inline)) inline void diag_step(MixedSmem<R> &sm, int K, bool fold) {
   double *T = sm.tiles + tileno(K, K) * 256;
@@ -307,31 +311,34 @@ __device__ __attribute__((always_inline)) inline void diag_step(MixedSmem<R> &sm
   double a[4];  // entries (4q + e, c): lower part of A, upper part m_c,4q+e (starts at 0)
 #pragma unroll
   for (int e = 0; e < 4; ++e) a[e] = 4 * q + e >= c ? T[tix(4 * q + e, c)] : 0.0;
+  wave_sync();  // the column buffers overwrite the tile's first 32 entries
   bool ok = true;
   // kept rolled: unrolled, each step's constant addresses stay live and the
   // enclosing kernels lose a wave per SIMD to the extra registers
 #pragma unroll 1
   for (int k = 0; k < 15; ++k) {
-    const double akk = T[tix(k, k)];
+    double *B = T + 16 * (k & 1);
+    if (c == k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) B[4 * q + e] = a[e];
+    wave_sync();
+    const double akk = B[k];
     ok = ok && (akk > 0.0);
     const double rk = rcp_f64(akk > 0.0 ? akk : 1.0);
-    const double ack = T[tix(c, k)] * rk;  // l_ck
-    // column k: a_ik below the pivot, m_ki (row k of M, final) above it
-    double aik[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) aik[e] = T[tix(4 * q + e, k)];
-    wave_sync();  // every lane has read step k's column before any write
+    const double ack = B[c] * rk;  // l_ck (used for c > k)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int i = 4 * q + e;
-      if (c > k && (i >= c || i <= k)) {
-        // lower: a_ic -= a_ik l_ck; upper: m_ci -= l_ck m_ki (m_kk = 1)
-        a[e] = __builtin_fma(-(i == k ? 1.0 : aik[e]), ack, a[e]);
-        T[tix(i, c)] = a[e];
-      }
+      // column k: a_ik below the pivot, m_ki (row k of M, final) above it
+      // lower: a_ic -= a_ik l_ck; upper: m_ci -= l_ck m_ki (m_kk = 1)
+      const double coef = i == k ? 1.0 : B[i];
+      if (c > k && (i >= c || i <= k)) a[e] = __builtin_fma(-coef, ack, a[e]);
     }
-    wave_sync();
   }
+  wave_sync();  // the last step's reads of the buffers are done
+#pragma unroll
+  for (int e = 0; e < 4; ++e) T[tix(4 * q + e, c)] = a[e];
+  wave_sync();
   ok = ok && (T[tix(15, 15)] > 0.0);
   // X = C^-1 = D^-1/2 M: x_ic = m_ic / sqrt(d_i) below the diagonal, 1 / sqrt(d_i) on it
   double x[4];
