@@ -240,9 +240,10 @@ extern "C" ctc_status ctc_batch_get_factor(ctc_batch *b, uint32_t problem, uint3
 }
 
 // the precision a class runs in: the BF16-operand path exists for the
-// row-fused kernels' ranks (32, 64, 128); other ranks keep the FP32 kernels
+// row-fused kernels' ranks, of which a batch has 32 and 64; ranks 8 and 16
+// keep the FP32 kernels
 static int32_t class_precision(uint32_t rank, int32_t asked) {
-  return asked == CTC_PREC_MIXED && (rank == 32 || rank == 64 || rank == 128) ? CTC_PREC_MIXED : CTC_PREC_FP32;
+  return asked == CTC_PREC_MIXED && (rank == 32 || rank == 64) ? CTC_PREC_MIXED : CTC_PREC_FP32;
 }
 
 static ctc_options class_options(const RankClass &c, const ctc_options *o) {

@@ -402,9 +402,9 @@ ctc_status ctc_batch_get_factor(ctc_batch *b, uint32_t problem, uint32_t mode, f
 ctc_status ctc_batch_autotune(ctc_batch *b, const ctc_options *opt, ctc_tune_result *results,
                               uint32_t *nclasses);
 /* `iters` ALS iterations (modes 0..N-1 in turn) on every problem.  With
- * opt->precision = CTC_PREC_MIXED the classes of rank 32, 64 and 128 run the
- * BF16-operand row-fused path (FP64 solve, one refinement) and the other
- * ranks the FP32 kernels; CTC_PREC_FP32 runs every class in FP32.
+ * opt->precision = CTC_PREC_MIXED the classes of rank 32 and 64 run the
+ * BF16-operand row-fused path (FP64 solve, one refinement) and those of rank
+ * 8 and 16 the FP32 kernels; CTC_PREC_FP32 runs every class in FP32.
  * use_graph != 0: the run is captured once into a hipGraph (one branch per
  * rank class, forked from and joined to the context's stream) and replayed
  * by later calls with the same iters and options; 0: launched directly.
