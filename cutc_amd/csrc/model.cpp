@@ -315,8 +315,10 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->max_bad_epochs = 3;                                               // :32
   // work-unit shapes measured on the 100M-nonzero benchmark tensor (the
   // reference's GA searched the same two knobs, GA/fitness.c:67-79)
-  // measured on the 100M-nonzero tensor at R = 32: SGD 2048, ALS and CCD++ 1024
-  o->tile_len = algorithm == CTC_SGD ? 2048 : 1024;
+  // measured on the 100M-nonzero tensor at R = 32: SGD 2048, CCD++ 1024; ALS
+  // leaves it 0, so that each of its kernel families takes its own measured
+  // default (span and FP32 row-fused kernels 1024, BF16 row-fused 4096 or 8192)
+  o->tile_len = algorithm == CTC_SGD ? 2048 : algorithm == CTC_ALS ? 0 : 1024;
   o->block = algorithm == CTC_SGD ? 64 : 256;
   o->seed = 0;
   o->mode_order = CTC_ORDER_FIXED;

@@ -31,7 +31,7 @@ inline FactorArgs rel_factors(float *const *F, uint32_t N, uint32_t mode) {
 
 // tiles per span from the options' tile_len (nonzeros per wave work unit)
 inline uint32_t span_tiles_of(const ctc_options *o) {
-  uint32_t tl = o && o->tile_len ? o->tile_len : 256;
+  uint32_t tl = o && o->tile_len ? o->tile_len : 1024;  // 0: the measured default
   tl = std::max<uint32_t>(64, (tl + 63) / 64 * 64);
   return tl / 64;
 }

@@ -229,7 +229,10 @@ typedef struct ctc_options {
   double tolerance;        /* TOLERANCE, SG/base.h:36 */
   uint32_t max_bad_epochs; /* MAX_BADEPOCHS, SG/base.h:32 */
   uint32_t tile_len;       /* nonzeros per wave work unit, multiple of 64
-                              (the unit DEFAULT_T_TILE_LENGTH sized, SG/base.h:62) */
+                              (the unit DEFAULT_T_TILE_LENGTH sized, SG/base.h:62);
+                              0 = the kernel family's measured default (the ALS
+                              default: spans and FP32 rows 1024, BF16 rows 4096,
+                              8192 at rank 128) */
   uint32_t block;          /* threads per workgroup (DEFAULT_BLOCKSIZE, SG/base.h:61) */
   uint64_t seed;           /* replaces srand(time(NULL)), SG/completion.c:426 */
   int32_t mode_order;      /* ctc_mode_order; SG/als.cu:684-685 rotates at random */
