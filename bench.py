@@ -311,7 +311,7 @@ def main():
                        "oracle_fp64_rmse_te_per_epoch": [round(x, 5) for x in ref] if ref else None,
                        "max_abs_gap_vs_oracle": gap, "epochs": a.warmup + a.steps,
                        "tile_len": opt.to_c().tile_len, "policy": a.sgd_update,
-                       "hot_row_threshold": opt.to_c().sgd_hot_rows or 262144, "alg_bytes_per_nnz": alg_bytes_per_nnz}
+                       "hot_row_threshold": opt.to_c().sgd_hot_rows or 65536, "alg_bytes_per_nnz": alg_bytes_per_nnz}
     log(f"SGD {wall / a.steps * 1e3:.2f} ms/epoch, kernel {kern_ms:.2f} ms, {value / 1e9:.2f} Gnnz/s, "
         f"rmse_te {rm0:.4f} -> {traj[-1]:.4f}; max gap vs FP64 sequential oracle {gap}")
 

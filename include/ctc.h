@@ -206,8 +206,9 @@ typedef enum ctc_als_kernel { CTC_ALS_AUTO = 0, CTC_ALS_SPANS = 1, CTC_ALS_ROWS 
 
 typedef enum ctc_sgd_update {
   CTC_SGD_AUTO = 0,       /* walked-mode rows racy; a row of another mode is racy when
-                             it has more than sgd_hot_rows nonzeros (its atomics would
-                             serialise on one L2 channel); otherwise a walker applies its
+                             it has more than sgd_hot_rows nonzeros and more than 4x its
+                             mode's mean (its atomics would serialise on one L2
+                             channel); otherwise a walker applies its
                              updates to its register copy of the row and adds their sum
                              with atomics when its run of nonzeros on that row ends */
   CTC_SGD_ATOMIC = 1,     /* walked-mode rows racy, every other row atomic at every nonzero */
@@ -244,7 +245,8 @@ typedef struct ctc_options {
   int32_t sgd_update;      /* ctc_sgd_update */
   int32_t spread;          /* 1: walkers visit spans in a strided order that
                               spreads a hot row's spans over the sweep */
-  uint32_t sgd_hot_rows;   /* CTC_SGD_AUTO threshold (nonzeros per row; 0 = 262144) */
+  uint32_t sgd_hot_rows;   /* CTC_SGD_AUTO threshold (nonzeros per row; 0 = 65536); a row
+                              is hot only if it also has more than 4x its mode's mean */
   int32_t als_kernel;      /* ctc_als_kernel (FP32 ALS) */
 } ctc_options;
 
