@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALS, ALS_AUTO, ALS_ROWS, ALS_SPANS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
-                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_HOT_ATOMIC, SGD_NEXT_RACY, SGD_RACY,
+                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, SGD, SGD_ATOMIC, SGD_AUTO, SGD_HOT_ATOMIC, SGD_NEXT_RACY, SGD_RACY, SGD_SYNC,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
 __all__ = ["Coo", "Ciss", "Context", "Model", "Batch", "Options", "synth", "tt_read", "ciss_alloc",
@@ -253,10 +253,11 @@ class Options:
     refine: int = 0
     eval_every_epoch: int = 1
     verbose: int = 0
-    sgd_update: int = SGD_AUTO
+    sgd_update: int = None  # None: the library default (SGD_SYNC)
     spread: int = 0
-    sgd_hot_rows: int = 0
+    sgd_hot_rows: int = None  # None: the library default (65536)
     als_kernel: int = 0
+    sgd_substep: int = None
 
     def to_c(self):
         o = _lib.Options()

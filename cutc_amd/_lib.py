@@ -16,7 +16,7 @@ ALS, SGD, CCD = 0, 1, 2
 LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
 ORDER_FIXED, ORDER_ROTATE = 0, 1
-SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY, SGD_HOT_ATOMIC = 0, 1, 2, 3, 4
+SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY, SGD_HOT_ATOMIC, SGD_SYNC = 0, 1, 2, 3, 4, 5
 ALS_AUTO, ALS_SPANS, ALS_ROWS = 0, 1, 2
 VALUES_RAW, VALUES_AFFINE_1_5, VALUES_INT_1_5 = 0, 1, 2
 
@@ -101,6 +101,7 @@ class Options(C.Structure):
         ("spread", C.c_int32),
         ("sgd_hot_rows", C.c_uint32),
         ("als_kernel", C.c_int32),
+        ("sgd_substep", C.c_uint32),
     ]
 
 

@@ -160,6 +160,27 @@ struct ctc_model {
   // BF16 mirrors of the factors (mixed-precision ALS)
   uint16_t *Fbf[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t hot_threshold = 0;
+  // bulk-synchronous SGD: per mode, the cold rows' fixed-point delta sums
+  // (zero between sub-steps), and the mode-0 rows long enough to meet two
+  // walkers in one sub-step (for the span / sub-step they were built for)
+  long long *sgd_D[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  // hot rows: index map (dims, 0xFFFFFFFF cold), row list, replicated sums
+  uint32_t *sgd_hidx[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_hrow[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  long long *sgd_Dh[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_ch[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t sgd_nhot[CTC_MAX_MODES] = {0, 0, 0, 0};
+  // staged modes (2..N-1): staging slots (walkers x sub x R) and, per
+  // sub-step, the rows that took deltas with their slots in walk order
+  float *sgd_stage[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_seg_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_seg_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_slots[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint64_t> sgd_seg_off[CTC_MAX_MODES];  // per sub-step: first segment (host)
+  uint32_t *sgd_long = nullptr;
+  uint64_t sgd_nlong = 0;
+  uint32_t sgd_long_span = 0, sgd_long_sub = 0;
+  uint64_t sgd_long_thr = 0;  // hot threshold the staged groups were built for
 
 };
 

@@ -212,6 +212,279 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
 }
 
 // ==========================================================================
+// Bulk-synchronous SGD (the default schedule).  One launch of k_sgd_sync is
+// one sub-step: walker w runs nonzeros [w*span + s*sub, +sub) of its span
+// from the factors as they stood when the sub-step began, and one launch of
+// k_sgd_merge then folds every row's handed-in deltas into the factors.
+//   * Along a run of consecutive nonzeros with the same index in a mode the
+//     walker keeps its copy of that row in registers (read where the run
+//     starts) and applies the reference step to it (SG/sgd.cu:86-99).
+//   * When a run ends its delta is handed in: as 64-bit fixed-point integers
+//     (2^-32 units) added atomically into the row's delta sum D, and a count
+//     of runs.  Integer addition is associative, so the sums do not depend on
+//     the order the walkers finish in and the sweep is bitwise reproducible.
+//   * A mode-0 row of at most long_min - 1 nonzeros cannot meet a second
+//     walker within one sub-step (two walkers' nonzeros of one sub-step are at
+//     least span - sub + 1 positions apart), so its walker stores its copy in
+//     place; longer mode-0 rows go through D like every other mode's rows.
+//   * k_sgd_merge: a cold row adds its delta sum (the reference's atomicAdd,
+//     SG/sgd.cu:96-98), a hot row (bit set in the mode's hot-row bitmap) adds
+//     the mean of its runs' deltas (replica averaging, NC/sgd.cu:229-262 with
+//     the sum of :523-525); both then clear D and the count.
+// ==========================================================================
+constexpr double kFixScale = 4294967296.0;        // 2^32
+constexpr double kFixInv = 1.0 / 4294967296.0;    // 2^-32
+
+template <int R, int N>
+__global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, SyncArgs sy, uint32_t span,
+                                                  uint32_t sub, uint32_t s, uint64_t nspans, uint32_t long_min,
+                                                  float step, float lam) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int WPW = 64 / W;
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  const int lane = lane_id();
+  const int sub_w = lane / W;
+  const int wl = lane % W;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t walker = wave * WPW + sub_w;
+  if (wave * WPW >= nspans) return;  // whole wave idle (uniform)
+  uint64_t e0 = 0, e1 = 0;
+  if (walker < nspans) {
+    const uint64_t send = (walker + 1) * span < c.nnz ? (walker + 1) * span : c.nnz;
+    e0 = walker * span + (uint64_t)s * sub;
+    e1 = e0 + sub < send ? e0 + sub : send;
+    if (e0 > e1) e0 = e1;
+  }
+  float *Fm[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) Fm[q] = fa.f[q];
+  float v[N][CPL], del[N][CPL];
+  uint32_t cur[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    cur[q] = NONE;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) v[q][k] = 0.f, del[q][k] = 0.f;
+  }
+  uint32_t slot = 0;
+  bool own0 = false;  // the current mode-0 row is stored in place (no other walker meets it)
+  bool staged[N];     // the current row of mode k writes its deltas to the staging slots
+#pragma unroll
+  for (int k = 0; k < N; ++k) staged[k] = false;
+  auto hand_in = [&](int q) {
+    if (staged[q]) return;  // every delta of the run is in its staging slots
+    if (q == 0 && own0) {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) Fm[0][(size_t)cur[0] * R + wl + k * W] = v[0][k];
+      return;
+    }
+    const uint32_t h = sy.hidx[q][cur[q]];
+    unsigned long long *d;
+    if (h == 0xFFFFFFFFu) {
+      d = (unsigned long long *)sy.D[q] + (size_t)cur[q] * R;
+    } else {
+      // a hot row's mean needs its count of runs; a cold row adds its delta
+      // sum, so the merge needs no count (an untouched row's sum is 0)
+      const size_t r = (size_t)h * kSyncReplicas + walker % kSyncReplicas;
+      d = (unsigned long long *)sy.Dh[q] + r * R;
+      if (wl == 0) atomicAdd(sy.ch[q] + r, 1u);
+    }
+#pragma unroll
+    for (int k = 0; k < CPL; ++k)
+      atomicAdd(d + wl + k * W, (unsigned long long)__double2ll_rn((double)del[q][k] * kFixScale));
+  };
+
+  for (uint32_t ch = 0; ch < sub; ch += W) {
+    const uint64_t el = e0 + ch + wl;
+    const bool vl = el < e1;
+    uint32_t il[N - 1];
+    float xl = 0.f;
+    bool hl = false;
+    if (vl) {
+#pragma unroll
+      for (int q = 0; q < N - 1; ++q) il[q] = c.oidx[q][el];
+      xl = c.vals[el];
+      hl = (c.tile_mask[el >> 6] >> (el & 63)) & 1ull;
+    } else {
+#pragma unroll
+      for (int q = 0; q < N - 1; ++q) il[q] = 0;
+    }
+    const uint64_t hm = __ballot(vl && hl);
+    for (int q = 0; q < W; ++q) {
+      const int src = sub_w * W + q;
+      uint32_t idx[N];
+#pragma unroll
+      for (int k = 0; k < N - 1; ++k) idx[1 + k] = (uint32_t)__shfl((int)il[k], src, 64);
+      const float x = __shfl(xl, src, 64);
+      const uint64_t e = e0 + ch + q;
+      if (e < e1) {
+        const bool head = (hm >> src) & 1ull;
+        bool new0 = false;
+        if (cur[0] == NONE) {
+          slot = tile_lane_slot(c.tile_slot[e >> 6], c.tile_mask[e >> 6], (int)(e & 63));
+          new0 = true;
+        } else if (head) {
+          slot = slot + 1;
+          new0 = true;
+        }
+        if (new0) {
+          idx[0] = c.directory[slot];
+        } else {
+          idx[0] = cur[0];
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if (idx[k] != cur[k]) {
+            if (cur[k] != NONE) hand_in(k);
+            if (k == 0) own0 = c.rowptr[slot + 1] - c.rowptr[slot] < long_min;
+            if (k >= 2) staged[k] = sy.stage[k] != nullptr && sy.hidx[k][idx[k]] == 0xFFFFFFFFu;
+            cur[k] = idx[k];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+              v[k][j] = Fm[k][(size_t)idx[k] * R + wl + j * W];
+              del[k][j] = 0.f;
+            }
+          }
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          float pv = v[0][j];
+#pragma unroll
+          for (int k = 1; k < N; ++k) pv *= v[k][j];
+          part += pv;
+        }
+        const float err = x - group_sum<W>(part);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          float d[N];
+#pragma unroll
+          for (int k = 0; k < N; ++k) {
+            float o = 1.f;
+#pragma unroll
+            for (int k2 = 0; k2 < N; ++k2)
+              if (k2 != k) o *= v[k2][j];
+            d[k] = step * (err * o - lam * v[k][j]);
+          }
+#pragma unroll
+          for (int k = 0; k < N; ++k) {
+            v[k][j] += d[k];
+            if (k >= 2 && staged[k])
+              sy.stage[k][(walker * sub + (e - e0)) * R + wl + j * W] = d[k];
+            else
+              del[k][j] += d[k];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (cur[k] != NONE) hand_in(k);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_sgd_merge(MergeArgs a) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int WPW = 64 / W;
+  const int lane = lane_id();
+  const int wl = lane % W;
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t i = wave * WPW + lane / W;
+  if (i >= a.off[a.nseg]) return;
+  int g = 0;
+  while (g + 1 < a.nseg && i >= a.off[g + 1]) ++g;
+  const int m = a.mode[g];
+  const uint64_t k = i - a.off[g];
+  if (a.is_hot[g]) {
+    // hot row k of mode m: sum its replicas, add the mean of its runs' deltas
+    uint32_t cnt = 0;
+    for (int r = 0; r < kSyncReplicas; ++r) cnt += a.ch[m][k * kSyncReplicas + r];
+    if (cnt == 0) return;
+    const uint32_t row = a.hrow[m][k];
+    float *gr = a.F[m] + (size_t)row * R;
+    long long *d = a.Dh[m] + k * kSyncReplicas * R;
+    const double sc = 1.0 / (double)cnt;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int col = wl + j * W;
+      long long sum = 0;
+      for (int r = 0; r < kSyncReplicas; ++r) {
+        sum += d[(size_t)r * R + col];
+        d[(size_t)r * R + col] = 0;
+      }
+      gr[col] = (float)((double)gr[col] + (double)sum * kFixInv * sc);
+    }
+    if (wl < kSyncReplicas) a.ch[m][k * kSyncReplicas + wl] = 0u;
+    if (W < kSyncReplicas)
+      for (int r = wl + W; r < kSyncReplicas; r += W) a.ch[m][k * kSyncReplicas + r] = 0u;
+    return;
+  }
+  // cold row: add its delta sum where it is nonzero
+  const uint32_t row = a.rows[g] ? a.rows[g][k] : (uint32_t)k;
+  float *gr = a.F[m] + (size_t)row * R;
+  long long *d = a.D[m] + (size_t)row * R;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int col = wl + j * W;
+    const long long dv = d[col];
+    if (dv != 0) {
+      gr[col] = (float)((double)gr[col] + (double)dv * kFixInv);
+      d[col] = 0;
+    }
+  }
+}
+
+// One workgroup per staged row: its walkers take the row's staging slots in
+// turn (FP64 partial sums), the partials are added in walker order, and the
+// row adds the total.  The order is fixed, so the result is reproducible.
+template <int R>
+__global__ void __launch_bounds__(256) k_sgd_stage_merge(StageArgs a) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int NW = 256 / W;  // walkers per workgroup
+  __shared__ double part[NW][R];
+  const uint64_t g = blockIdx.x;
+  if (g >= a.nseg) return;
+  const int wid = threadIdx.x / W;
+  const int wl = threadIdx.x % W;
+  const uint32_t b = a.seg_start[g], e = a.seg_start[g + 1];
+  double acc[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) acc[j] = 0.0;
+  uint32_t p = b + wid;
+  for (; p + 3 * NW < e; p += 4 * NW) {  // four independent slots in flight
+    float x[4][CPL];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float *src = a.stage + (size_t)a.slots[p + u * NW] * R;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) x[u][j] = src[wl + j * W];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) acc[j] += (double)x[u][j];
+  }
+  for (; p < e; p += NW) {
+    const float *src = a.stage + (size_t)a.slots[p] * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] += (double)src[wl + j * W];
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) part[wid][wl + j * W] = acc[j];
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += part[w][threadIdx.x];
+    float *gr = a.F + (size_t)a.seg_row[g] * R;
+    gr[threadIdx.x] = (float)((double)gr[threadIdx.x] + t);
+  }
+}
+
+// ==========================================================================
 // MTTKRP / adjoint walk: per row, acc = sum over its nonzeros of w * h with
 // h = hadamard of the other modes' rows and
 //   w = x                       (MTTKRP, SG/als.cu:295-312 intent), or
@@ -940,6 +1213,40 @@ hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, co
   if (nmodes == 3) return launch_sgd_n<3>(st, c, f, hot, R, span, nspans, stride, code, step, lam, grid, block);
   if (nmodes == 4) return launch_sgd_n<4>(st, c, f, hot, R, span, nspans, stride, code, step, lam, grid, block);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SyncArgs &sy, int nmodes,
+                           int R, uint32_t span, uint32_t sub, uint32_t s, uint32_t long_min, float step, float lam,
+                           unsigned block) {
+  if (c.nnz == 0) return hipSuccess;
+  const uint64_t nspans = (c.nnz + span - 1) / span;
+  const int W = R < 64 ? R : 64;
+  const uint64_t waves = (nspans + (64 / W) - 1) / (64 / W);
+  const unsigned grid = grid_for(waves, block);
+  if (nmodes == 3) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, 3>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam));
+  } else if (nmodes == 4) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, 4>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam));
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sgd_merge(hipStream_t st, const MergeArgs &a, int R) {
+  const uint64_t items = a.off[a.nseg];
+  if (items == 0) return hipSuccess;
+  const int W = R < 64 ? R : 64;
+  const uint64_t waves = (items + (64 / W) - 1) / (64 / W);
+  const unsigned grid = grid_for(waves, 256);
+  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_merge<RR>), dim3(grid), dim3(256), 0, st, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_sgd_stage_merge(hipStream_t st, const StageArgs &a, int R) {
+  if (a.nseg == 0) return hipSuccess;
+  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_stage_merge<RR>), dim3((unsigned)a.nseg), dim3(256), 0, st, a));
+  return hipGetLastError();
 }
 
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,

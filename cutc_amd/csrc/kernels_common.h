@@ -43,6 +43,54 @@ struct HotRows {
   const uint32_t *bits[4];
 };
 
+// Bulk-synchronous SGD (k_sgd_sync): per CISS-relative mode, the cold rows'
+// delta sums in 2^-32 fixed point (dims x R).  A hot row's delta sum and its
+// count of runs are spread over kSyncReplicas slots at
+// different addresses (walker w adds into slot w mod kSyncReplicas), so that
+// the walkers sharing it do not all queue on one memory channel; hidx[row] is
+// the row's index among its mode's hot rows (0xFFFFFFFF: cold).
+constexpr int kSyncReplicas = 32;
+// Modes 2.. (CISS-relative) change row at almost every nonzero; for their
+// cold rows a walker writes each nonzero's delta to stage[k] at slot
+// walker * sub + (position within the sub-step) instead of adding run sums
+// atomically, and k_sgd_stage_merge sums every row's slots in a fixed order.
+struct SyncArgs {
+  long long *D[4];
+  const uint32_t *hidx[4];
+  long long *Dh[4];   // hot rows x kSyncReplicas x R
+  uint32_t *ch[4];    // hot rows x kSyncReplicas
+  float *stage[4];    // null: the mode hands in run sums through D
+};
+
+// k_sgd_stage_merge: the rows of one staged mode that took deltas in one
+// sub-step; segment g is row seg_row[g], whose deltas are in the staging
+// slots slots[seg_start[g] .. seg_start[g + 1]).
+struct StageArgs {
+  float *F;
+  const float *stage;
+  const uint32_t *seg_row;
+  const uint32_t *seg_start;
+  const uint32_t *slots;
+  uint64_t nseg;
+};
+
+// k_sgd_merge: segment g covers merge items [off[g], off[g + 1]) of mode
+// mode[g].  A cold segment's item k is row rows[g][k] (rows[g] null: row k),
+// whose sum is in D; a hot segment's item k is hot row k of the mode, whose
+// sums are spread over the replicas of Dh / ch.
+struct MergeArgs {
+  float *F[4];
+  long long *D[4];
+  long long *Dh[4];
+  uint32_t *ch[4];
+  const uint32_t *hrow[4];  // hot row k of each mode
+  const uint32_t *rows[8];
+  int mode[8];
+  int is_hot[8];
+  uint64_t off[9];
+  int nseg;
+};
+
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
 
 // 1/x to full FP64 precision: hardware reciprocal estimate + two Newton steps

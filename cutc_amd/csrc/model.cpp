@@ -25,6 +25,9 @@ constexpr unsigned kEvalBlocks = 512;  // fixed grid for deterministic reduction
 // rules out whatever the threshold.
 constexpr uint64_t kSgdHotRows = 65536;
 constexpr double kSgdHotRel = 4.0;
+// bulk-synchronous SGD work units: walker span and sub-step (nonzeros)
+constexpr uint32_t kSgdSpan = 2048;
+constexpr uint32_t kSgdSubstep = 32;
 
 template <class T>
 ctc_status dev_alloc(T **p, size_t n) {
@@ -327,7 +330,7 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   // measured on the 100M-nonzero tensor at R = 32: SGD 2048, CCD++ 1024; ALS
   // leaves it 0, so that each of its kernel families takes its own measured
   // default (span and FP32 row-fused kernels 1024, BF16 row-fused 4096 or 8192)
-  o->tile_len = algorithm == CTC_SGD ? 2048 : algorithm == CTC_ALS ? 0 : 1024;
+  o->tile_len = algorithm == CTC_SGD ? kSgdSpan : algorithm == CTC_ALS ? 0 : 1024;
   o->block = algorithm == CTC_SGD ? 64 : 256;
   o->seed = 0;
   o->mode_order = CTC_ORDER_FIXED;
@@ -337,9 +340,10 @@ extern "C" ctc_status ctc_options_default(int32_t algorithm, ctc_options *o) {
   o->als_kernel = CTC_ALS_AUTO;
   o->eval_every_epoch = 1;
   o->verbose = 0;
-  o->sgd_update = CTC_SGD_AUTO;
+  o->sgd_update = CTC_SGD_SYNC;
   o->spread = 0;
   o->sgd_hot_rows = kSgdHotRows;
+  o->sgd_substep = kSgdSubstep;
   return CTC_OK;
 }
 
@@ -409,7 +413,17 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->FT[md]);
     dev_free(m->ccdx[md]);
     dev_free(m->hotbits[md]);
+    dev_free(m->sgd_D[md]);
+    dev_free(m->sgd_stage[md]);
+    dev_free(m->sgd_seg_row[md]);
+    dev_free(m->sgd_seg_start[md]);
+    dev_free(m->sgd_slots[md]);
+    dev_free(m->sgd_hidx[md]);
+    dev_free(m->sgd_hrow[md]);
+    dev_free(m->sgd_Dh[md]);
+    dev_free(m->sgd_ch[md]);
   }
+  dev_free(m->sgd_long);
   for (DevCoo *c : {&m->valid, &m->test}) {
     for (int s = 0; s < 4; ++s) dev_free(c->ind[s]);
     dev_free(c->vals);
@@ -603,8 +617,216 @@ static uint64_t spread_stride(uint64_t n, bool on) {
   return s % n ? s % n : 1;
 }
 
+// Hot-row bitmaps of every mode: rows with more training nonzeros than the
+// threshold and more than kSgdHotRel times the mode's mean.  The relative
+// bound keeps a mode whose rows are all alike (every row shared by many
+// walkers, none far hotter than the rest) out of the hot set.
+// The bulk-synchronous schedule also keeps, per mode, each row's index among
+// the hot rows, the hot rows' list and their replicated (zeroed) delta sums.
+static ctc_status ensure_hotbits(ctc_model *m, uint64_t thr) {
+  if (thr == m->hot_threshold && m->hotbits[0]) return CTC_OK;
+  const ctc_coo *t = m->train.get();
+  hipStream_t st = m->ctx->stream;
+  CTC_HIP(hipStreamSynchronize(st));
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    std::vector<uint32_t> cnt(m->dims[md], 0), bits((m->dims[md] + 31) / 32, 0);
+    std::vector<uint32_t> hidx(m->dims[md], 0xFFFFFFFFu), hrow;
+    for (uint64_t p = 0; p < t->nnz; ++p) cnt[t->ind[md][p]]++;
+    const double rel = kSgdHotRel * (double)t->nnz / (double)std::max<uint64_t>(1, m->dims[md]);
+    for (uint64_t i = 0; i < m->dims[md]; ++i)
+      if (cnt[i] > thr && (double)cnt[i] > rel) {
+        bits[i >> 5] |= 1u << (i & 31);
+        hidx[i] = (uint32_t)hrow.size();
+        hrow.push_back((uint32_t)i);
+      }
+    if (!m->hotbits[md]) CTC_HIP(hipMalloc(&m->hotbits[md], std::max<size_t>(1, bits.size()) * sizeof(uint32_t)));
+    if (!bits.empty())
+      CTC_HIP(hipMemcpy(m->hotbits[md], bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (void *p : {(void *)m->sgd_hidx[md], (void *)m->sgd_hrow[md], (void *)m->sgd_Dh[md], (void *)m->sgd_ch[md]})
+      dev_free(p);
+    m->sgd_hidx[md] = m->sgd_hrow[md] = m->sgd_ch[md] = nullptr;
+    m->sgd_Dh[md] = nullptr;
+    CTC_TRY(upload(&m->sgd_hidx[md], hidx.data(), hidx.size(), st));
+    CTC_TRY(upload(&m->sgd_hrow[md], hrow.data(), hrow.size(), st));
+    const size_t nrep = hrow.size() * kSyncReplicas;
+    CTC_TRY(dev_alloc(&m->sgd_Dh[md], nrep * m->rank));
+    CTC_TRY(dev_alloc(&m->sgd_ch[md], nrep));
+    if (nrep) {
+      CTC_HIP(hipMemsetAsync(m->sgd_Dh[md], 0, nrep * m->rank * sizeof(long long), st));
+      CTC_HIP(hipMemsetAsync(m->sgd_ch[md], 0, nrep * sizeof(uint32_t), st));
+    }
+    m->sgd_nhot[md] = hrow.size();
+    CTC_HIP(hipStreamSynchronize(st));
+  }
+  m->hot_threshold = thr;
+  return CTC_OK;
+}
+
+// nonzeros per walker and per sub-step of the bulk-synchronous schedule
+static ctc_status sync_shape(const ctc_options *o, uint32_t *span, uint32_t *sub) {
+  *span = o->tile_len ? o->tile_len : kSgdSpan;
+  *sub = o->sgd_substep ? o->sgd_substep : kSgdSubstep;
+  if (*sub > *span) return fail(CTC_ERR_INVALID_ARG, "sgd_substep %u exceeds tile_len %u", *sub, *span);
+  return CTC_OK;
+}
+
+// Buffers of the bulk-synchronous schedule: zeroed delta sums per mode (the
+// merge leaves them zero), and the list of mode-0 rows that
+// two walkers can meet in one sub-step (length >= span - sub + 2).
+static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
+  hipStream_t st = m->ctx->stream;
+  for (uint32_t md = 0; md < m->nmodes; ++md) {
+    if (m->sgd_D[md]) continue;
+    CTC_TRY(dev_alloc(&m->sgd_D[md], (size_t)m->dims[md] * m->rank));
+    CTC_HIP(hipMemsetAsync(m->sgd_D[md], 0, (size_t)m->dims[md] * m->rank * sizeof(long long), st));
+  }
+  if (m->sgd_long && m->sgd_long_span == span && m->sgd_long_sub == sub && m->sgd_long_thr == m->hot_threshold)
+    return CTC_OK;
+  const DevCiss &d = m->ciss[0];
+  // staged modes: slot of position p is walker * sub + (p - walker * span) % sub
+  // in sub-step (p - walker * span) / sub; per sub-step the cold rows' slots
+  // are grouped by row, each group in walk order
+  const uint64_t nnz = d.nnz, nspans = (nnz + span - 1) / span, nsub = (span + sub - 1) / sub;
+  for (uint32_t k = 2; k < m->nmodes; ++k) {
+    for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_seg_row[k], (void *)m->sgd_seg_start[k],
+                    (void *)m->sgd_slots[k]})
+      dev_free(p);
+    m->sgd_stage[k] = nullptr;
+    m->sgd_seg_row[k] = m->sgd_seg_start[k] = m->sgd_slots[k] = nullptr;
+    std::vector<uint32_t> idx(nnz), hidx(m->dims[k]);
+    CTC_HIP(hipMemcpy(idx.data(), d.oidx[k - 1], nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    CTC_HIP(hipMemcpy(hidx.data(), m->sgd_hidx[k], hidx.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    // bucket the cold positions by sub-step (stable), then sort each bucket by row
+    std::vector<uint64_t> boff(nsub + 1, 0);
+    for (uint64_t p = 0; p < nnz; ++p)
+      if (hidx[idx[p]] == 0xFFFFFFFFu) boff[(p % span) / sub + 1]++;
+    for (uint64_t s2 = 0; s2 < nsub; ++s2) boff[s2 + 1] += boff[s2];
+    std::vector<uint64_t> key(boff[nsub]);
+    {
+      std::vector<uint64_t> fill(boff.begin(), boff.end() - 1);
+      for (uint64_t p = 0; p < nnz; ++p) {
+        if (hidx[idx[p]] != 0xFFFFFFFFu) continue;
+        const uint64_t w = p / span, in = p - w * span, slotp = w * sub + in % sub;
+        key[fill[in / sub]++] = ((uint64_t)idx[p] << 32) | slotp;
+      }
+    }
+    parallel_for(nsub, host_threads(), [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t s2 = b; s2 < e; ++s2) std::sort(key.begin() + boff[s2], key.begin() + boff[s2 + 1]);
+    });
+    std::vector<uint32_t> seg_row, seg_start, slots(key.size());
+    std::vector<uint64_t> &soff = m->sgd_seg_off[k];
+    soff.assign(nsub + 1, 0);
+    for (uint64_t s2 = 0; s2 < nsub; ++s2) {
+      soff[s2] = seg_row.size();
+      for (uint64_t q = boff[s2]; q < boff[s2 + 1]; ++q) {
+        const uint32_t row = (uint32_t)(key[q] >> 32);
+        if (q == boff[s2] || row != (uint32_t)(key[q - 1] >> 32)) {
+          seg_row.push_back(row);
+          seg_start.push_back((uint32_t)q);
+        }
+        slots[q] = (uint32_t)key[q];
+      }
+    }
+    soff[nsub] = seg_row.size();
+    seg_start.push_back((uint32_t)key.size());
+    CTC_TRY(upload(&m->sgd_seg_row[k], seg_row.data(), seg_row.size(), st));
+    CTC_TRY(upload(&m->sgd_seg_start[k], seg_start.data(), seg_start.size(), st));
+    CTC_TRY(upload(&m->sgd_slots[k], slots.data(), slots.size(), st));
+    CTC_TRY(dev_alloc(&m->sgd_stage[k], (size_t)nspans * sub * m->rank));
+    CTC_HIP(hipStreamSynchronize(st));
+  }
+  const uint64_t long_min = (uint64_t)span - sub + 2;
+  std::vector<uint32_t> dir(d.dlength), rows;
+  CTC_HIP(hipMemcpy(dir.data(), d.directory, d.dlength * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (uint64_t sl = 0; sl < d.dlength; ++sl)
+    if ((uint64_t)d.h_rowptr[sl + 1] - d.h_rowptr[sl] >= long_min) rows.push_back(dir[sl]);
+  CTC_HIP(hipStreamSynchronize(st));
+  dev_free(m->sgd_long);
+  m->sgd_long = nullptr;
+  CTC_TRY(upload(&m->sgd_long, rows.data(), rows.size(), st));
+  CTC_HIP(hipStreamSynchronize(st));
+  m->sgd_nlong = rows.size();
+  m->sgd_long_span = span;
+  m->sgd_long_sub = sub;
+  m->sgd_long_thr = m->hot_threshold;
+  return CTC_OK;
+}
+
+// One epoch of the bulk-synchronous schedule: per sub-step, k_sgd_sync over
+// every walker, then k_sgd_merge over mode 0's long rows and every row of the
+// other modes.  Enqueued only (no synchronisation).
+static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) {
+  uint32_t span = 0, sub = 0;
+  CTC_TRY(sync_shape(o, &span, &sub));
+  CTC_TRY(ensure_hotbits(m, o->sgd_hot_rows ? o->sgd_hot_rows : kSgdHotRows));
+  CTC_TRY(ensure_sync(m, span, sub));
+  const uint32_t N = m->nmodes, R = m->rank;
+  SyncArgs sy{};
+  MergeArgs mg{};
+  mg.nseg = 0;
+  mg.off[0] = 0;
+  for (uint32_t k = 0; k < N; ++k) {
+    sy.D[k] = m->sgd_D[k];  // CISS mode 0: CISS-relative order is the identity
+    sy.hidx[k] = m->sgd_hidx[k];
+    sy.Dh[k] = m->sgd_Dh[k];
+    sy.ch[k] = m->sgd_ch[k];
+    sy.stage[k] = m->sgd_stage[k];
+    mg.F[k] = m->F[k];
+    mg.D[k] = m->sgd_D[k];
+    mg.Dh[k] = m->sgd_Dh[k];
+    mg.ch[k] = m->sgd_ch[k];
+    mg.hrow[k] = m->sgd_hrow[k];
+    // cold segment: mode 0's long rows, every row of mode 1 (the staged
+    // modes' cold rows are summed by k_sgd_stage_merge)
+    if (k < 2) {
+      mg.mode[mg.nseg] = (int)k;
+      mg.is_hot[mg.nseg] = 0;
+      mg.rows[mg.nseg] = k == 0 ? m->sgd_long : nullptr;
+      mg.off[mg.nseg + 1] = mg.off[mg.nseg] + (k == 0 ? m->sgd_nlong : m->dims[k]);
+      ++mg.nseg;
+    }
+    if (m->sgd_nhot[k]) {  // hot segment: the mode's hot rows
+      mg.mode[mg.nseg] = (int)k;
+      mg.is_hot[mg.nseg] = 1;
+      mg.rows[mg.nseg] = nullptr;
+      mg.off[mg.nseg + 1] = mg.off[mg.nseg] + m->sgd_nhot[k];
+      ++mg.nseg;
+    }
+  }
+  const float step = (float)(lr * o->sgd_scale);
+  const CissArgs c = ciss_args(m->ciss[0]);
+  const FactorArgs f = rel_factors(m->F, N, 0);
+  const uint32_t nsub = (span + sub - 1) / sub;
+  hipStream_t st = m->ctx->stream;
+  Prof pe(m->ctx, "sgd");
+  for (uint32_t s = 0; s < nsub; ++s) {
+    {
+      Prof p(m->ctx, "sgd_walk");
+      CTC_HIP(launch_sgd_sync(st, c, f, sy, (int)N, (int)R, span, sub, s, span - sub + 2, step, (float)o->lambda,
+                              block_of(o)));
+    }
+    {
+      Prof p(m->ctx, "sgd_merge");
+      CTC_HIP(launch_sgd_merge(st, mg, (int)R));
+      for (uint32_t k = 2; k < N; ++k) {
+        StageArgs sa{};
+        sa.F = m->F[k];
+        sa.stage = m->sgd_stage[k];
+        sa.seg_row = m->sgd_seg_row[k] + m->sgd_seg_off[k][s];
+        sa.seg_start = m->sgd_seg_start[k] + m->sgd_seg_off[k][s];
+        sa.slots = m->sgd_slots[k];
+        sa.nseg = m->sgd_seg_off[k][s + 1] - m->sgd_seg_off[k][s];
+        CTC_HIP(launch_sgd_stage_merge(st, sa, (int)R));
+      }
+    }
+  }
+  m->resid_valid = false;
+  return CTC_OK;
+}
+
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   CTC_TRY(ensure_ciss(m, 0));
+  if (o->sgd_update == CTC_SGD_SYNC) return sgd_epoch_sync(m, o, lr);
   // policy codes of k_sgd: PA * 64 + PB * 8 + PC with 0 racy, 1 atomic, 3 hot
   // bitmap (racy / atomic), 4 hot bitmap (racy / run-accumulated atomic)
   static const int code_of[5] = {0 * 64 + 4 * 8 + 4, 0 * 64 + 1 * 8 + 1, 0, 0 * 64 + 0 * 8 + 1, 0 * 64 + 3 * 8 + 3};
@@ -612,24 +834,7 @@ static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   const int code = code_of[o->sgd_update];
   HotRows hot{};
   if (o->sgd_update == CTC_SGD_AUTO || o->sgd_update == CTC_SGD_HOT_ATOMIC) {
-    const uint64_t thr = o->sgd_hot_rows ? o->sgd_hot_rows : kSgdHotRows;
-    if (thr != m->hot_threshold || !m->hotbits[0]) {
-      const ctc_coo *t = m->train.get();
-      for (uint32_t md = 0; md < m->nmodes; ++md) {
-        std::vector<uint32_t> cnt(m->dims[md], 0), bits((m->dims[md] + 31) / 32, 0);
-        for (uint64_t p = 0; p < t->nnz; ++p) cnt[t->ind[md][p]]++;
-        // hot: above the threshold and well above the mode's mean, so a mode
-        // whose rows are all alike (every row shared by many walkers, none
-        // far hotter than the rest) keeps its atomics: made racy, such a
-        // mode loses most of its updates and the sweep stalls
-        const double rel = kSgdHotRel * (double)t->nnz / (double)std::max<uint64_t>(1, m->dims[md]);
-        for (uint64_t i = 0; i < m->dims[md]; ++i)
-          if (cnt[i] > thr && (double)cnt[i] > rel) bits[i >> 5] |= 1u << (i & 31);
-        if (!m->hotbits[md]) CTC_HIP(hipMalloc(&m->hotbits[md], bits.size() * sizeof(uint32_t)));
-        CTC_HIP(hipMemcpy(m->hotbits[md], bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      }
-      m->hot_threshold = thr;
-    }
+    CTC_TRY(ensure_hotbits(m, o->sgd_hot_rows ? o->sgd_hot_rows : kSgdHotRows));
     for (uint32_t s2 = 0; s2 < m->nmodes; ++s2) hot.bits[s2] = m->hotbits[s2];  // CISS mode 0: identity order
   }
   const float step = (float)(lr * o->sgd_scale);

@@ -214,8 +214,20 @@ typedef enum ctc_sgd_update {
   CTC_SGD_ATOMIC = 1,     /* walked-mode rows racy, every other row atomic at every nonzero */
   CTC_SGD_RACY = 2,       /* every row racy (fastest; loses updates on shared rows) */
   CTC_SGD_NEXT_RACY = 3,  /* walked mode and the next mode racy, other modes atomic */
-  CTC_SGD_HOT_ATOMIC = 4  /* as AUTO, but a row that is not hot is re-read and updated
+  CTC_SGD_HOT_ATOMIC = 4, /* as AUTO, but a row that is not hot is re-read and updated
                              with atomics at every nonzero */
+  CTC_SGD_SYNC = 5        /* the default: bulk-synchronous and deterministic.  Walker w
+                             owns nonzeros [w*tile_len, (w+1)*tile_len) of the mode-0
+                             order; sub-step s runs nonzeros [w*tile_len + s*sgd_substep,
+                             +sgd_substep) of every walker from the factors as they stood
+                             when the sub-step began.  A walker keeps its copy of a row
+                             along a run of equal indices and hands in the run's delta;
+                             at the end of the sub-step a row adds the sum of its runs'
+                             deltas (the reference's atomicAdd, SG/sgd.cu:96-98), or their
+                             mean if it is hot (more than sgd_hot_rows nonzeros and more
+                             than 4x its mode's mean; replica averaging,
+                             NC/sgd.cu:229-262,523-525).  Sums are exact (fixed point),
+                             so every run of a sweep gives the same bits. */
 } ctc_sgd_update;
 
 typedef struct ctc_options {
@@ -245,9 +257,12 @@ typedef struct ctc_options {
   int32_t sgd_update;      /* ctc_sgd_update */
   int32_t spread;          /* 1: walkers visit spans in a strided order that
                               spreads a hot row's spans over the sweep */
-  uint32_t sgd_hot_rows;   /* CTC_SGD_AUTO threshold (nonzeros per row; 0 = 65536); a row
-                              is hot only if it also has more than 4x its mode's mean */
+  uint32_t sgd_hot_rows;   /* CTC_SGD_SYNC / CTC_SGD_AUTO hot-row threshold (nonzeros per
+                              row; 0 = 65536); a row is hot only if it also has more than
+                              4x its mode's mean */
   int32_t als_kernel;      /* ctc_als_kernel (FP32 ALS) */
+  uint32_t sgd_substep;    /* CTC_SGD_SYNC: nonzeros per walker per sub-step (0 = 32);
+                              at most tile_len */
 } ctc_options;
 
 /* Reference defaults for an algorithm (SG/base.h, SG/completion.c:460-480). */

@@ -10,6 +10,8 @@ Reference files restated (SG = ``software/single GPU/sgpu``):
   SG/ciss.c:68-195           CISS layout           -> ciss_reference_layout
   SG/als.cu:186-317,467-491  ALS mode update       -> als_mode / als
   SG/sgd.cu:21-143,264-324   SGD epoch             -> sgd_epoch / sgd
+  SG/sgd.cu:21-143 + NC/sgd.cu:229-262,523-525 (walker schedule, replica
+                             averaging)               -> sgd_sync_epoch / sgd_sync
   SG/ccd.cu:532-554, NC/ccd.cu:28-180  CCD++       -> ccd_epoch / ccd
   SG/completion.c:123-214,296-362      loss/RMSE   -> loss_sq / rmse / converge
 
@@ -22,6 +24,9 @@ Pins (each function is checked against something it is not a retyping of):
   als_mode                 per-row ridge regression via numpy.linalg.lstsq on the
                            stacked [H; sqrt(lambda) I] system; exact rank-1 recovery
   sgd_epoch                single-nonzero closed form; two-nonzero sequential form
+  sgd_sync_epoch           equals sgd_epoch with one walker and 1-nonzero sub-steps,
+                           and with walkers on disjoint rows; hand-computed sums and
+                           means of snapshot steps on a shared row
   ccd_epoch                per-column 1-D ridge optimality (zero partial gradient)
                            and monotone decrease of the objective
   grad                     central finite differences in FP64
@@ -283,6 +288,20 @@ def sgd_epoch(t: Coo, F, lr, lam, scale=1.0, order=None):
                             C.c_double(lam), C.c_double(scale))
 
 
+def sgd_sync_epoch(t: Coo, F, lr, lam, span, sub, hot_rows, scale=1.0, order=None, nthreads=1):
+    """One epoch of the bulk-synchronous walker schedule (libctc's default SGD):
+    walkers of `span` positions of `order`, sub-steps of `sub` nonzeros from
+    the sub-step's factors, per-run deltas summed on cold rows and averaged on
+    hot rows (more than hot_rows and 4x the mode's mean nonzeros).  Restates
+    SG/sgd.cu:21-143 (walker per tile, atomicAdd merge) with the replica
+    averaging of NC/sgd.cu:229-262,523-525 for hot rows."""
+    R = F[0].shape[1]
+    op = _u32p(order) if order is not None else None
+    _fn("orc_sgd_sync_epoch", F)(t.c(), op, C.c_uint32(R), _ptrs(F, _prec(F)), C.c_double(lr),
+                                 C.c_double(lam), C.c_double(scale), C.c_uint64(span), C.c_uint64(sub),
+                                 C.c_uint64(hot_rows), C.c_int(nthreads))
+
+
 def residual(t: Coo, F):
     res = np.empty(t.nnz, F[0].dtype)
     p = _fp(res) if res.dtype == np.float32 else _dp(res)
@@ -382,6 +401,22 @@ def sgd(t: Coo, F, lr, lam, epochs, test: Coo | None = None, scale=1.0, decay=No
                 lr = lr * 1.05 if obj < prev_obj else lr * 0.5
             prev_obj = obj
         elif decay is not None:
+            lr *= decay
+    return rm, lrs
+
+
+def sgd_sync(t: Coo, F, lr, lam, epochs, span, sub, hot_rows, test: Coo | None = None, scale=1.0, decay=None,
+             order=None, nthreads=1):
+    """Epochs of sgd_sync_epoch in mode-0 order with an exponential lr decay."""
+    if order is None:
+        order, _ = mode_order(t, 0)
+    rm, lrs = [], []
+    for _ in range(epochs):
+        lrs.append(lr)
+        sgd_sync_epoch(t, F, lr, lam, span, sub, hot_rows, scale, order=order, nthreads=nthreads)
+        if test is not None:
+            rm.append(rmse(test, F))
+        if decay is not None:
             lr *= decay
     return rm, lrs
 

@@ -89,6 +89,14 @@ int orc_ciss_reference_layout(const orc_coo *t, uint32_t mode, uint32_t tile_len
    * updated from their pre-update values, step lr*scale) */                          \
   void orc_sgd_epoch##SFX(const orc_coo *t, const uint32_t *order, uint32_t R,         \
                           REAL **F, double lr, double lambda, double scale);          \
+  /* one SGD epoch of the bulk-synchronous walker schedule (libctc's default):      \
+   * walkers of `span` positions of `order`, sub-steps of `sub` nonzeros from the      \
+   * sub-step's factors, per-run deltas summed (cold rows) or averaged (rows with     \
+   * more than hot_rows and 4x the mean nonzeros); see oracle_impl.h */               \
+  void orc_sgd_sync_epoch##SFX(const orc_coo *t, const uint32_t *order, uint32_t R,    \
+                               REAL **F, double lr, double lambda, double scale,       \
+                               uint64_t span, uint64_t sub, uint64_t hot_rows,         \
+                               int nthreads);                                          \
   /* residual r = x - prediction (CCD++ set-up) */                                     \
   void orc_residual##SFX(const orc_coo *t, uint32_t R, REAL *const *F, REAL *res);     \
   /* one CCD++ epoch (canonical, SG/ccd.cu:532-554, NC/ccd.cu:28-180).               \

@@ -188,6 +188,154 @@ void SFX(orc_sgd_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R, REA
   }
 }
 
+/* One epoch of the bulk-synchronous SGD schedule (libctc's default SGD).
+ *
+ * The reference's kernel gives every thread one tile of nonzeros in mode-0
+ * order and lets all of them add their updates with atomicAdd while the
+ * others read (SG/sgd.cu:21-143, :96-98); its multi-GPU version gives each
+ * replica a tile range and averages the replicas after the sweep
+ * (NC/sgd.cu:229-262, launched at :512, divides every replica by the GPU
+ * count; :523-525 sum the replicas with ncclAllReduce).  This schedule fixes which values each update
+ * reads, so that one restatement describes every run:
+ *   - walker w owns positions [w*span, (w+1)*span) of the walk `order`;
+ *   - sub-step s runs positions [w*span + s*sub, +sub) of every walker's span,
+ *     all from the factors as they stood when the sub-step began;
+ *   - a walker keeps its own copy of a row along a run of consecutive
+ *     nonzeros with the same index in that mode (read from the sub-step's
+ *     factors where the run starts; a later run on the same row in the same
+ *     sub-step starts from them again), applies the reference step
+ *     (SG/sgd.cu:86-99) to its copies, and hands in the run's delta when the
+ *     run ends;
+ *   - when the sub-step ends, a cold row adds the deltas of all its runs (the
+ *     reference's atomicAdd) and a hot row adds their mean (replica
+ *     averaging: it becomes the mean of the runs' copies).  A row of mode m is
+ *     hot when it has more than hot_rows nonzeros in t and more than 4x
+ *     t->nnz / dims[m].
+ * With span >= nnz and sub = 1 this is the sequential rule of orc_sgd_epoch.
+ * nthreads > 1 runs the walkers of a sub-step in parallel (the deltas are
+ * then added in FP64 in thread order, a difference of FP64 rounding only). */
+void SFX(orc_sgd_sync_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R, REAL **F,
+                             double lr, double lambda, double scale, uint64_t span, uint64_t sub,
+                             uint64_t hot_rows, int nthreads) {
+  const uint32_t N = t->nmodes;
+  const uint64_t n = t->nnz;
+  if (n == 0 || span == 0 || sub == 0 || R > 512) return;
+  const REAL step = (REAL)(lr * scale);
+  const REAL lam = (REAL)lambda;
+  const uint64_t W = (n + span - 1) / span;
+  const uint64_t nsub = (span + sub - 1) / sub;
+  int nth = nthreads > 0 ? nthreads : 1;
+  double *D[ORC_MAX_MODES];
+  uint32_t *nrun[ORC_MAX_MODES];
+  unsigned char *hot[ORC_MAX_MODES];
+  for (uint32_t m = 0; m < N; ++m) {
+    D[m] = (double *)calloc((size_t)t->dims[m] * R, sizeof(double));
+    nrun[m] = (uint32_t *)calloc((size_t)t->dims[m], sizeof(uint32_t));
+    hot[m] = (unsigned char *)calloc((size_t)t->dims[m], 1);
+    uint64_t *cnt = (uint64_t *)calloc((size_t)t->dims[m], sizeof(uint64_t));
+    for (uint64_t p = 0; p < n; ++p) cnt[t->ind[m][p]]++;
+    const double rel = 4.0 * (double)n / (double)t->dims[m];
+    for (uint64_t i = 0; i < t->dims[m]; ++i) hot[m][i] = cnt[i] > hot_rows && (double)cnt[i] > rel;
+    free(cnt);
+  }
+  for (uint64_t s = 0; s < nsub; ++s) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nth)
+#endif
+    for (int64_t w = 0; w < (int64_t)W; ++w) {
+      const uint64_t s1 = ((uint64_t)w + 1) * span < n ? ((uint64_t)w + 1) * span : n;
+      const uint64_t e0 = (uint64_t)w * span + s * sub;
+      const uint64_t e1 = e0 + sub < s1 ? e0 + sub : s1;
+      REAL v[ORC_MAX_MODES][512], del[ORC_MAX_MODES][512];
+      uint32_t cur[ORC_MAX_MODES];
+      int live[ORC_MAX_MODES] = {0, 0, 0, 0};
+      for (uint64_t q = e0; q < e1; ++q) {
+        const uint64_t p = order ? order[q] : q;
+        for (uint32_t m = 0; m < N; ++m) {
+          const uint32_t ix = t->ind[m][p];
+          if (live[m] && ix == cur[m]) continue;
+          if (live[m]) { /* the run on cur[m] ends: hand in its delta */
+            double *d = D[m] + (size_t)cur[m] * R;
+            for (uint32_t f = 0; f < R; ++f) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+              d[f] += (double)del[m][f];
+            }
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+            nrun[m][cur[m]] += 1;
+          }
+          live[m] = 1;
+          cur[m] = ix;
+          for (uint32_t f = 0; f < R; ++f) {
+            v[m][f] = F[m][(size_t)ix * R + f];
+            del[m][f] = 0;
+          }
+        }
+        REAL pred = 0; /* SG/sgd.cu:86-89 */
+        for (uint32_t f = 0; f < R; ++f) {
+          REAL x = v[0][f];
+          for (uint32_t m = 1; m < N; ++m) x *= v[m][f];
+          pred += x;
+        }
+        const REAL err = (REAL)t->vals[p] - pred; /* :90 */
+        for (uint32_t f = 0; f < R; ++f) {          /* :92-99, pre-update values */
+          REAL d[ORC_MAX_MODES];
+          for (uint32_t m = 0; m < N; ++m) {
+            REAL others = 1;
+            for (uint32_t o = 0; o < N; ++o)
+              if (o != m) others *= v[o][f];
+            d[m] = step * (err * others - lam * v[m][f]);
+          }
+          for (uint32_t m = 0; m < N; ++m) {
+            v[m][f] += d[m];
+            del[m][f] += d[m];
+          }
+        }
+      }
+      for (uint32_t m = 0; m < N; ++m) {
+        if (!live[m]) continue;
+        double *d = D[m] + (size_t)cur[m] * R;
+        for (uint32_t f = 0; f < R; ++f) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          d[f] += (double)del[m][f];
+        }
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        nrun[m][cur[m]] += 1;
+      }
+    }
+    /* merge: every row that took a run this sub-step */
+    for (uint32_t m = 0; m < N; ++m) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nth)
+#endif
+      for (int64_t i = 0; i < (int64_t)t->dims[m]; ++i) {
+        const uint32_t c = nrun[m][i];
+        if (!c) continue;
+        const double sc = hot[m][i] ? 1.0 / (double)c : 1.0;
+        REAL *g = F[m] + (size_t)i * R;
+        double *d = D[m] + (size_t)i * R;
+        for (uint32_t f = 0; f < R; ++f) {
+          g[f] = (REAL)((double)g[f] + d[f] * sc);
+          d[f] = 0.0;
+        }
+        nrun[m][i] = 0;
+      }
+    }
+  }
+  for (uint32_t m = 0; m < N; ++m) {
+    free(D[m]);
+    free(nrun[m]);
+    free(hot[m]);
+  }
+}
+
 void SFX(orc_residual)(const orc_coo *t, uint32_t R, REAL *const *F, REAL *res) {
   for (uint64_t p = 0; p < t->nnz; ++p)
     res[p] = (REAL)t->vals[p] - SFX(orc_predict)(t, R, F, p);

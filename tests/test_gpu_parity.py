@@ -4,8 +4,12 @@ CPU oracle on the same seeded inputs.
 Tolerances (from the reference's update rules as the oracle restates them):
   * MTTKRP, Gram-based ALS updates, CCD++ and the gradient: relative Frobenius
     error vs the FP64 oracle, FP32 on the GPU -> <= 1e-5 per factor.
-  * SGD (Hogwild on the GPU vs sequential in the oracle): per-epoch test RMSE
-    within 1e-3.
+  * SGD, default bulk-synchronous schedule (oracle.sgd_sync_epoch restates it):
+    relative factor error <= 1e-5 per factor after every epoch checked, and
+    per-epoch test RMSE within 1e-3 (and within 5% of how far the oracle's own
+    test RMSE moves).
+  * SGD, opt-in Hogwild policies vs the sequential oracle: per-epoch test RMSE
+    within 1e-3 on a 150K-nonzero tensor.
   * Loss / RMSE: FP64 accumulation on both sides -> 1e-6 relative.
 """
 import sys
@@ -206,8 +210,106 @@ def test_als_not_spd_is_reported(ctx):
 
 
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("policy,tile_len", [(ct.SGD_AUTO, None), (ct.SGD_AUTO, 1024), (ct.SGD_ATOMIC, 256),
-                                             (ct.SGD_HOT_ATOMIC, None)])
+# SGD, default schedule: bulk-synchronous walkers (oracle.sgd_sync_epoch)
+SYNC_CASES = [  # nmodes, R, span, sub-step, nnz
+    (3, 8, 96, 7, 20000), (3, 16, 512, 64, 40000), (3, 32, 256, 32, 40000), (3, 64, 2048, 128, 60000),
+    (3, 128, 1024, 64, 20000), (4, 16, 512, 48, 30000), (4, 32, 256, 32, 20000), (4, 128, 640, 40, 20000),
+]
+
+
+def sync_problem(N, nnz, seed):
+    dims = (600, 300, 50) if N == 3 else (200, 100, 30, 12)
+    zipf = (1.0, 1.0, 0.0) if N == 3 else (1.0, 1.0, 0.0, 0.0)
+    return ct.synth(dims, nnz, zipf=zipf, true_rank=4, value_map=ct.VALUES_INT_1_5, noise_std=0.3,
+                    test_fraction=0.05, seed=seed)
+
+
+@pytest.mark.parametrize("N,R,span,sub,nnz", SYNC_CASES)
+def test_sgd_sync_matches_oracle(ctx, N, R, span, sub, nnz):
+    """Two epochs of the default schedule against its FP64 restatement, with
+    hot rows (threshold 400) next to cold shared rows, walkers whose spans and
+    sub-steps do not align with tiles or wave widths, 3- and 4-way tensors."""
+    tr, te = sync_problem(N, nnz, 11 + R + N)
+    say(f"[sgd_sync] N={N} R={R} span={span} sub={sub} nnz={tr.nnz}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 5)
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    otr, ote = ocoo(tr), ocoo(te)
+    order, _ = O.mode_order(otr, 0)
+    opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.05, sgd_scale=1.0, tile_len=span, sgd_substep=sub,
+                     sgd_hot_rows=400)
+    for ep in range(2):
+        m.sgd_epoch(opt, lr=5e-3)
+        O.sgd_sync_epoch(otr, F64, 5e-3, 0.05, span, sub, 400, order=order, nthreads=4)
+        errs = [relerr(m.factor(k), F64[k]) for k in range(N)]
+        say(f"  epoch {ep + 1}: rel err", ["%.2e" % e for e in errs])
+        assert max(errs) < 1e-5, errs
+        assert abs(m.evaluate()["rmse_te"] - O.rmse(ote, F64)) < 1e-5
+
+
+def test_sgd_sync_is_bitwise_reproducible(ctx):
+    """Deltas are summed as fixed-point integers: two sweeps from the same
+    factors give the same bits (2M nonzeros, thousands of walkers on shared
+    rows)."""
+    tr, _ = ct.synth((50000, 8000, 500), 2_000_000, zipf=(1.0, 1.0, 0.0), true_rank=8,
+                     value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.0, seed=3)
+    say(f"[sgd_sync_bits] nnz={tr.nnz}")
+    m = ct.Model(ctx, tr, None, None, rank=32)
+    F = init_factors(tr.dims, 32, 2)
+    opt = ct.Options(algorithm=ct.SGD, rank=32, lambda_=0.05, sgd_scale=1.0, tile_len=2048, sgd_substep=32,
+                     sgd_hot_rows=1000)
+    outs = []
+    for _ in range(2):
+        m.set_factors(F)
+        for _ in range(2):
+            m.sgd_epoch(opt, lr=1e-3)
+        outs.append([m.factor(k).copy() for k in range(3)])
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_sgd_sync_test_rmse_trajectory(ctx):
+    """Twelve epochs of the default solve loop (exponential decay) on a skewed
+    150K-nonzero tensor where the test RMSE falls by more than 0.3: per epoch
+    within 1e-3 of the FP64 restatement and within 5% of the oracle's own
+    movement, and factors after epoch 1 within 1e-5."""
+    tr, te = ct.synth((3000, 1500, 200), 150_000, zipf=(1.0, 1.0, 0.0), true_rank=10,
+                      value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.01, seed=0x5EED0002)
+    R, lr, span, sub = 32, 2e-2, 1024, 32
+    say(f"[sgd_sync_traj] train={tr.nnz}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = [0.5 * f for f in init_factors(tr.dims, R, 6)]
+    otr, ote = ocoo(tr), ocoo(te)
+    order, _ = O.mode_order(otr, 0)
+    # factors after one epoch
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.05, learning_rate=lr, lr_schedule=ct.LR_EXP_DECAY,
+                     lr_decay=0.95, sgd_scale=1.0, max_epochs=12, max_bad_epochs=0, tile_len=span, sgd_substep=sub,
+                     sgd_hot_rows=2000)
+    m.sgd_epoch(opt, lr=lr)
+    O.sgd_sync_epoch(otr, F64, lr, 0.05, span, sub, 2000, order=order, nthreads=4)
+    assert max(relerr(m.factor(k), F64[k]) for k in range(3)) < 1e-5
+    # the solve loop
+    m.set_factors(F)
+    F64 = [f.astype(np.float64) for f in F]
+    rm_orc, _ = O.sgd_sync(otr, F64, lr, 0.05, 12, span, sub, 2000, test=ote, decay=0.95, order=order, nthreads=4)
+    res = m.solve(opt)
+    rm_gpu = [h["rmse_te"] for h in res.history[1:]]
+    say("  oracle", np.round(rm_orc, 5))
+    say("  gpu   ", np.round(rm_gpu, 5))
+    assert len(rm_gpu) == 12
+    moved = abs(O.rmse(ote, [f.astype(np.float64) for f in F]) - min(rm_orc))
+    assert moved > 0.3, moved
+    gap = np.max(np.abs(np.array(rm_gpu) - np.array(rm_orc)))
+    assert gap < 1e-3 and gap < 0.05 * moved, gap
+
+
+# --------------------------------------------------------------------------
+# SGD, opt-in Hogwild policies against the sequential rule
+@pytest.mark.parametrize("policy,tile_len", [(ct.SGD_AUTO, 2048), (ct.SGD_AUTO, 1024), (ct.SGD_ATOMIC, 256),
+                                             (ct.SGD_HOT_ATOMIC, 2048)])
 def test_sgd_test_rmse_trajectory(ctx, policy, tile_len):
     """SGD (lr 1e-3, x0.95 per epoch, lambda 0.05, scale 1) test-RMSE per
     epoch within 1e-3 of the sequential FP64 oracle."""
@@ -255,7 +357,7 @@ def test_sgd_uniform_mode_never_hot(ctx):
         m.set_factors(F)
         opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.05, learning_rate=5e-3,
                          lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0, max_epochs=6,
-                         max_bad_epochs=0, sgd_update=policy, sgd_hot_rows=thr)
+                         max_bad_epochs=0, sgd_update=policy, sgd_hot_rows=thr, tile_len=2048)
         runs.append(np.array([h["rmse_te"] for h in m.solve(opt).history]))
         say(f"  policy {policy} threshold {thr or 'default'}:", np.round(runs[-1], 5))
     assert np.max(np.abs(runs[0] - runs[1])) < 2e-3
@@ -263,10 +365,11 @@ def test_sgd_uniform_mode_never_hot(ctx):
 
 
 @pytest.mark.parametrize("R", [8, 16, 64, 128])
-@pytest.mark.parametrize("policy", [ct.SGD_AUTO, ct.SGD_ATOMIC, ct.SGD_RACY, ct.SGD_HOT_ATOMIC])
+@pytest.mark.parametrize("policy", [ct.SGD_AUTO, ct.SGD_ATOMIC, ct.SGD_RACY, ct.SGD_HOT_ATOMIC, ct.SGD_SYNC])
 def test_sgd_single_walker_matches_sequential(ctx, R, policy):
     """With one tile the GPU walk is one sequential walker: factors match the
-    sequential oracle to FP32 rounding."""
+    sequential oracle to FP32 rounding (the bulk-synchronous schedule with
+    1-nonzero sub-steps, so that every nonzero starts from the merged rows)."""
     tr, _ = ct.synth((10, 8, 6), 64, test_fraction=0.0, seed=R)
     say(f"[sgd_single] R={R}")
     m = ct.Model(ctx, tr, None, None, rank=R)
@@ -274,8 +377,8 @@ def test_sgd_single_walker_matches_sequential(ctx, R, policy):
     m.set_factors(F)
     F64 = [f.astype(np.float64) for f in F]
     O.sgd_epoch(ocoo(tr), F64, 1e-2, 0.1, 1.0)
-    m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.1, sgd_scale=1.0, tile_len=64, sgd_update=policy),
-                lr=1e-2)
+    m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=R, lambda_=0.1, sgd_scale=1.0, tile_len=64, sgd_update=policy,
+                           sgd_substep=1), lr=1e-2)
     for md in range(3):
         assert relerr(m.factor(md), F64[md]) < 1e-5
 

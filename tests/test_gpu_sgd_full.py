@@ -1,0 +1,62 @@
+"""Full-size SGD parity: the default bulk-synchronous schedule on the
+100M-nonzero benchmark tensor (480189 x 17770 x 2182, Zipf(1.0) modes 0-1,
+integer values, 1% test; R = 32, lambda 0.05, lr 1e-3 x 0.95 per epoch,
+scale 1, factors U(-1,1) from numpy.random.default_rng(1)) against the FP64
+oracle's run of the same schedule, stored by
+tests/golden/make_sgd_cfg2_sync.py in sgd_cfg2_sync_oracle.npz.
+
+Bars: per-epoch test RMSE within 1e-3 for all 20 epochs; 64 sampled rows of
+every factor within 1e-5 relative (Frobenius) after epochs 1, 2, 5 and 20
+(FP32 rounding on the GPU against FP64 in the oracle, compounded over the
+sweeps: measured 1e-6 after epoch 20)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cutc_amd as ct
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "sgd_cfg2_sync_oracle.npz")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.timeout(600)
+def test_sgd_default_schedule_full_size_matches_oracle():
+    gold = np.load(GOLD, allow_pickle=False)
+    dims = (480189, 17770, 2182)
+    say("[sgd_full] generating the 100M-nonzero tensor")
+    tr, te = ct.synth(dims, 100_000_000, zipf=(1.0, 1.0, 0.0), true_rank=10, value_map=ct.VALUES_INT_1_5,
+                      noise_std=0.3, test_fraction=0.01, seed=0x5EED0002)
+    ctx = ct.Context(0)
+    m = ct.Model(ctx, tr, None, te, rank=32)
+    del tr
+    rng = np.random.default_rng(1)
+    m.set_factors([rng.uniform(-1, 1, (d, 32)).astype(np.float32) for d in dims])
+    opt = ct.Options(algorithm=ct.SGD, rank=32, lambda_=0.05, sgd_scale=1.0)
+    c = opt.to_c()
+    assert (c.tile_len, c.sgd_substep, c.sgd_hot_rows) == (int(gold["span"]), int(gold["sub"]), int(gold["hot"]))
+    assert abs(m.evaluate()["rmse_te"] - float(gold["rmse_te_initial"])) < 1e-6
+    tol = {1: 1e-5, 2: 1e-5, 5: 1e-5, 20: 1e-5}
+    lr, rm = 1e-3, []
+    for e in range(1, 21):
+        m.sgd_epoch(opt, lr=lr)
+        lr *= 0.95
+        rm.append(m.evaluate()["rmse_te"])
+        say(f"  epoch {e}: gpu {rm[-1]:.6f} oracle {gold['rmse_te'][e - 1]:.6f}")
+        if e in tol:
+            for k in range(3):
+                rows = gold[f"rows{k}"]
+                got = m.factor(k)[rows].astype(np.float64)
+                ref = gold[f"f{k}_e{e}"]
+                err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+                say(f"    factor {k} sampled rows rel err {err:.2e}")
+                assert err < tol[e], (e, k, err)
+    gap = np.max(np.abs(np.array(rm) - gold["rmse_te"]))
+    say(f"  max test-RMSE gap {gap:.2e}")
+    assert gap < 1e-3

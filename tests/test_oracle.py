@@ -415,6 +415,129 @@ def test_sgd_four_way_step():
 
 
 # --------------------------------------------------------------------------
+# bulk-synchronous SGD schedule (libctc's default SGD)
+# --------------------------------------------------------------------------
+NO_HOT = 2**63
+
+
+def _sgd_step(x, rows, lr, lam):
+    """Reference step on one nonzero from the given rows (SG/sgd.cu:86-99):
+    returns the deltas of every row."""
+    prod = np.prod(rows, axis=0)
+    e = x - np.sum(prod)
+    out = []
+    for m, r in enumerate(rows):
+        others = np.prod([rows[o] for o in range(len(rows)) if o != m], axis=0)
+        out.append(lr * (e * others - lam * r))
+    return out
+
+
+@pytest.mark.parametrize("hot", [NO_HOT, 0])
+def test_sgd_sync_one_walker_unit_substeps_is_sequential(hot):
+    """One walker over the whole stream with 1-nonzero sub-steps is the
+    sequential rule, whatever the hot-row threshold (one run per sub-step: its
+    sum and mean agree)."""
+    rng = np.random.default_rng(41)
+    dims = (9, 7, 5)
+    t = rand_coo(rng, dims, 150)
+    order, _ = O.mode_order(t, 0)
+    F = rand_factors(rng, dims, 4)
+    G = [f.copy() for f in F]
+    O.sgd_epoch(t, F, 0.05, 0.1, 1.0, order=order)
+    O.sgd_sync_epoch(t, G, 0.05, 0.1, span=t.nnz, sub=1, hot_rows=hot, order=order)
+    for f, g in zip(F, G):
+        np.testing.assert_allclose(g, f, rtol=1e-13, atol=1e-15)
+
+
+def test_sgd_sync_walkers_on_disjoint_rows_are_sequential():
+    """Walkers whose spans share no row of any mode do not interact: the
+    schedule equals the sequential rule over the whole stream."""
+    rng = np.random.default_rng(42)
+    blk, nb, per = (4, 3, 5), 4, 20
+    ind = [[], [], []]
+    vals = []
+    for b in range(nb):  # block b: rows [b*d, (b+1)*d) of every mode, 20 nonzeros
+        lin = rng.choice(int(np.prod(blk)), size=per, replace=False)
+        i0, r = np.divmod(lin, blk[1] * blk[2])
+        i1, i2 = np.divmod(r, blk[2])
+        for m, ii in enumerate((i0, i1, i2)):
+            ind[m].extend((ii + b * blk[m]).tolist())
+        vals.extend((rng.normal(size=per) + 3).tolist())
+    dims = tuple(nb * d for d in blk)
+    t = O.Coo(ind, vals, dims)
+    order, _ = O.mode_order(t, 0)
+    F = rand_factors(rng, dims, 3)
+    G = [f.copy() for f in F]
+    O.sgd_epoch(t, F, 0.03, 0.05, 1.0, order=order)
+    O.sgd_sync_epoch(t, G, 0.03, 0.05, span=per, sub=1, hot_rows=NO_HOT, order=order)
+    for f, g in zip(F, G):
+        np.testing.assert_allclose(g, f, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("hot", [False, True])
+def test_sgd_sync_shared_row_sum_or_mean(hot):
+    """Two walkers, one nonzero each, sharing row 0 of mode 1: both steps read
+    the sub-step's factors; the row takes the sum of the two deltas (cold) or
+    their mean (hot: 2 nonzeros > hot_rows = 1 and > 4 * 2 / 10)."""
+    rng = np.random.default_rng(43)
+    dims, R = (2, 10, 2), 3
+    t = O.Coo([[0, 1], [0, 0], [0, 1]], [2.0, -1.0], dims)
+    F = rand_factors(rng, dims, R)
+    F0 = [f.copy() for f in F]
+    lr, lam = 0.1, 0.02
+    d0 = _sgd_step(2.0, [F0[0][0], F0[1][0], F0[2][0]], lr, lam)
+    d1 = _sgd_step(-1.0, [F0[0][1], F0[1][0], F0[2][1]], lr, lam)
+    O.sgd_sync_epoch(t, F, lr, lam, span=1, sub=1, hot_rows=1 if hot else NO_HOT)
+    np.testing.assert_allclose(F[0][0], F0[0][0] + d0[0], rtol=1e-14)
+    np.testing.assert_allclose(F[0][1], F0[0][1] + d1[0], rtol=1e-14)
+    np.testing.assert_allclose(F[2][1], F0[2][1] + d1[2], rtol=1e-14)
+    shared = (d0[1] + d1[1]) / 2 if hot else d0[1] + d1[1]
+    np.testing.assert_allclose(F[1][0], F0[1][0] + shared, rtol=1e-14)
+    np.testing.assert_array_equal(F[1][1:], F0[1][1:])
+
+
+def test_sgd_sync_runs_restart_from_the_substep_factors():
+    """One walker, one 3-nonzero sub-step: the mode-0 row is one run (the
+    walker's copy carries its updates), mode-2 row 0 is visited twice with
+    another row between, so its second run starts again from the sub-step's
+    factors and the row takes both runs' deltas."""
+    rng = np.random.default_rng(44)
+    dims, R = (1, 3, 2), 4
+    t = O.Coo([[0, 0, 0], [0, 1, 2], [0, 1, 0]], [1.5, 2.5, 0.5], dims)
+    F = rand_factors(rng, dims, R)
+    F0 = [f.copy() for f in F]
+    lr, lam = 0.07, 0.03
+    a = F0[0][0].copy()
+    da1 = _sgd_step(1.5, [a, F0[1][0], F0[2][0]], lr, lam)
+    a = a + da1[0]
+    da2 = _sgd_step(2.5, [a, F0[1][1], F0[2][1]], lr, lam)
+    a = a + da2[0]
+    da3 = _sgd_step(0.5, [a, F0[1][2], F0[2][0]], lr, lam)
+    a = a + da3[0]
+    O.sgd_sync_epoch(t, F, lr, lam, span=3, sub=3, hot_rows=NO_HOT)
+    np.testing.assert_allclose(F[0][0], a, rtol=1e-14)
+    np.testing.assert_allclose(F[2][0], F0[2][0] + da1[2] + da3[2], rtol=1e-14)
+    np.testing.assert_allclose(F[2][1], F0[2][1] + da2[2], rtol=1e-14)
+    for j, d in enumerate((da1, da2, da3)):
+        np.testing.assert_allclose(F[1][j], F0[1][j] + d[1], rtol=1e-14)
+
+
+def test_sgd_sync_threads_agree():
+    """Walkers of a sub-step run in parallel: the result differs from the
+    one-thread run by FP64 rounding only."""
+    rng = np.random.default_rng(45)
+    dims = (40, 30, 20)
+    t = rand_coo(rng, dims, 3000)
+    order, _ = O.mode_order(t, 0)
+    F = rand_factors(rng, dims, 8)
+    G = [f.copy() for f in F]
+    O.sgd_sync_epoch(t, F, 0.01, 0.05, span=128, sub=16, hot_rows=100, order=order, nthreads=1)
+    O.sgd_sync_epoch(t, G, 0.01, 0.05, span=128, sub=16, hot_rows=100, order=order, nthreads=4)
+    for f, g in zip(F, G):
+        np.testing.assert_allclose(g, f, rtol=1e-12, atol=1e-14)
+
+
+# --------------------------------------------------------------------------
 # CCD++
 # --------------------------------------------------------------------------
 
