@@ -1,53 +1,267 @@
 #!/usr/bin/env python3
 """Benchmark of the tensor-completion hot path on one MI355X.
 
-Workload (the benchmark definition's CCD++/SGD configuration): a synthetic
-3-way tensor 480189 x 17770 x 2182 with 100M distinct nonzeros, modes 0-1
-Zipf(1.0) and mode 2 uniform, integer values 1..5 from a rank-10 CP model
-mapped to [1,5] plus N(0,0.3) noise, rounded and clipped; 1% test split;
-rank R = 32, lambda = 0.05.
+Headline workload (the CCD++/SGD configuration): a synthetic 3-way tensor
+480189 x 17770 x 2182 with 100M distinct nonzeros, modes 0-1 Zipf(1.0) and
+mode 2 uniform, integer values 1..5 from a rank-10 CP model mapped to [1,5]
+plus N(0,0.3) noise, rounded and clipped; 1% test split; rank R = 32,
+lambda = 0.05.
 
-A step is one SGD epoch over the training nonzeros (lr 1e-3, x0.95 per
-epoch).  `value` is training nonzeros per second over the K timed epochs
-with the tensor and factors resident in HBM.  CCD++ and ALS sweeps on the
-same tensor are reported under "algorithms".
+A step is one SGD epoch over the training nonzeros (the default
+bulk-synchronous schedule, lr 1e-3, x0.95 per epoch).  `value` is training
+nonzeros per second over the K timed epochs with the tensor and factors
+resident in HBM.  The per-epoch test RMSE is compared with the FP64 oracle of
+the same schedule (tests/golden/sgd_cfg2_sync_oracle.npz): `parity_ok` is
+false, and the exit status 3, when any epoch is 1e-3 or more away.
 
-roofline: algorithmic bytes per SGD launch (12 + 24 R bytes per nonzero:
-index/value stream plus three factor rows read and written) divided by the
-kernel's average duration measured with HIP events on the launch stream,
-against MI355X's 8 TB/s.  cpu_baseline: the oracle's sequential FP32 SGD
-epoch (one core) timed on this host over a bounded sample of the same
-training stream.
+roofline: algorithmic bytes per launch of the walk kernel k_sgd_sync (12 + 24R
+bytes per nonzero: index/value stream plus three factor rows read and
+written, times the nonzeros of one sub-step) divided by its average duration
+measured with HIP events on the launch stream, against 8 TB/s.
+cpu_baseline: the oracle's FP32 twin of the same schedule on all host cores
+over the full training stream; cpu_baselines adds the one-core sequential
+rule and the ALS / CCD++ CPU figures.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The other configurations are legs under "algorithms" (each in a time budget,
+an error in one leg is recorded and the others run): the one-GPU shares of the
+sharded SGD (50M and 12.5M nonzeros of the same generator), MTTKRP, CCD++ and
+ALS (FP32 and mixed precision) on the headline tensor, ALS on the 1M-nonzero
+tensor on GPU and CPU, the batched ALS workload (512 problems), SGD through
+the loss adjoint and torch.optim.SGD on the 140M-nonzero 4-way tensor, and
+mixed-precision ALS at R = 128 on the 500M-nonzero tensor.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--algs ...]
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+T_START = time.time()
 
 
 def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+    print(f"[bench {time.time() - T_START:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
-def mixed_leg(ct, ctx, a, log):
+def elapsed():
+    return time.time() - T_START
+
+
+def cfg2_tensor(ct, nnz, seed=0x5EED0002):
+    return ct.synth((480189, 17770, 2182), nnz, zipf=(1.0, 1.0, 0.0), true_rank=10, value_map=ct.VALUES_INT_1_5,
+                    noise_std=0.3, test_fraction=0.01, seed=seed)
+
+
+def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0):
+    """SGD epochs with the default schedule; -> (ms list, trajectory, walk ms per launch, walk launches per epoch,
+    options)."""
+    import numpy as np  # noqa: F401
+    model.set_factors(F0)
+    opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3, lr_schedule=ct.LR_EXP_DECAY,
+                     lr_decay=0.95, sgd_scale=1.0)
+    t0 = time.time()
+    model.prepare(opt)
+    model.sgd_epoch(opt, lr=0.0)  # builds the schedule's tables (lr 0: factors unchanged)
+    log(f"SGD set-up {time.time() - t0:.1f}s")
+    lr, times, traj = 1e-3, [], []
+    for e in range(warmup + steps):
+        if e == warmup:
+            ctx.reset_stats()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        model.sgd_epoch(opt, lr=lr)  # returns after the stream is drained
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        if e >= warmup:
+            times.append(dt)
+        lr *= 0.95
+        traj.append(model.evaluate(lam)["rmse_te"])  # untimed
+    n_w, ms_w = ctx.kernel_stats("sgd_walk")
+    n_m, ms_m = ctx.kernel_stats("sgd_merge")
+    return times, traj, ms_w / max(n_w, 1), n_w / max(steps, 1), ms_m / max(steps, 1), opt
+
+
+# ---------------------------------------------------------------- config 1
+def als_cfg1_leg(ct, ctx, a):
+    """Plain ALS, FP32, 2000 x 1500 x 300 with 1M uniform distinct nonzeros
+    (rank-8 CP truth mapped to [1,5] + N(0,0.1), 90/10 split), R = 16,
+    lambda 0.05, 10 iterations on the GPU and on the host cores (the oracle's
+    FP32 twin, all cores and one core), and the FP64 oracle's trajectory for
+    the test-RMSE gap and factor error."""
+    import numpy as np
+    import oracle as O
+    R, lam, iters = 16, 0.05, 10
+    tr, te = ct.synth((2000, 1500, 300), 1_000_000, true_rank=8, value_map=ct.VALUES_AFFINE_1_5, noise_std=0.1,
+                      test_fraction=0.1, seed=0x5EED0001)
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    rng = np.random.default_rng(11)
+    F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in tr.dims]
+    m.set_factors(F0)
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam)
+    m.prepare(opt)
+    for md in range(3):
+        m.als_mode_update(md, opt)  # warm-up sweep
+    m.set_factors(F0)
+    ctx.synchronize()
+    times, rm_gpu = [], []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        for md in range(3):
+            m.als_mode_update(md, opt)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        rm_gpu.append(m.evaluate(lam)["rmse_te"])
+    Fg = [m.factor(md) for md in range(3)]
+    ind, vals = tr.arrays()
+    ot = O.Coo(ind, vals, tr.dims)
+    ind, vals = te.arrays()
+    ote = O.Coo(ind, vals, te.dims)
+    orders = [O.mode_order(ot, md) for md in range(3)]
+    nth = os.cpu_count() or 1
+    # FP64 oracle trajectory (the parity reference)
+    F64 = [f.astype(np.float64) for f in F0]
+    rm64 = []
+    for _ in range(iters):
+        for md in range(3):
+            O.als_mode(ot, md, F64, lam, order=orders[md], nthreads=nth)
+        rm64.append(O.rmse(ote, F64, nthreads=nth))
+    ferr = max(float(np.linalg.norm(Fg[md].astype(np.float64) - F64[md]) / np.linalg.norm(F64[md])) for md in range(3))
+    # FP32 twin timed on all cores (10 iterations) and on one core (2 iterations)
+    cpu = {}
+    for cores, its in ((nth, iters), (1, 2)):
+        F32 = [f.copy() for f in F0]
+        t0 = time.perf_counter()
+        for _ in range(its):
+            for md in range(3):
+                O.als_mode(ot, md, F32, lam, order=orders[md], nthreads=cores)
+        dt = (time.perf_counter() - t0) / its
+        cpu[f"cores_{cores}"] = {"s_per_iter": dt, "nnz_per_s": tr.nnz / dt, "iters": its}
+    gpu_s = float(np.median(times))
+    res = {"dims": list(tr.dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam, "iters": iters,
+           "gpu_s_per_iter": gpu_s, "gpu_nnz_per_s": tr.nnz / gpu_s, "rmse_te_gpu": [round(x, 6) for x in rm_gpu],
+           "rmse_te_fp64_oracle": [round(x, 6) for x in rm64],
+           "max_rmse_te_gap_vs_fp64": float(max(abs(x - y) for x, y in zip(rm_gpu, rm64))),
+           "factor_rel_err_vs_fp64_after_10": ferr, "cpu": cpu, "host_cores": nth}
+    log(f"config 1 ALS: GPU {gpu_s * 1e3:.2f} ms/iter, CPU {cpu[f'cores_{nth}']['s_per_iter'] * 1e3:.1f} ms/iter "
+        f"({nth} cores), {cpu['cores_1']['s_per_iter'] * 1e3:.1f} ms/iter (1 core); test-RMSE gap vs FP64 "
+        f"{res['max_rmse_te_gap_vs_fp64']:.1e}, factor rel err {ferr:.1e}")
+    return res
+
+
+# ------------------------------------------------------ config 2 (others)
+def mttkrp_leg(ct, ctx, model, nnz, R):
+    ctx.reset_stats()
+    for md in range(3):
+        model.mttkrp(md)  # warm-up (builds the CISS of each mode)
+    ctx.reset_stats()
+    per_mode = []
+    for md in range(3):
+        for _ in range(3):
+            model.mttkrp(md)
+        n_m, ms_m = ctx.kernel_stats("mttkrp")
+        per_mode.append(ms_m / max(n_m, 1))
+        ctx.reset_stats()
+    mb = 12 + 8 * R  # stream + two gathered rows per nonzero
+    log(f"MTTKRP kernel ms per mode {[round(t, 3) for t in per_mode]}")
+    return {"kernel_ms_per_mode": per_mode, "alg_bytes_per_nnz": mb,
+            "alg_gbs_per_mode": [mb * nnz / (t * 1e-3) / 1e9 for t in per_mode]}
+
+
+def ccd_leg(ct, ctx, model, nnz, R, lam, F0, epochs, tile_len):
+    copt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=lam, tile_len=tile_len or None)
+    model.set_factors(F0)
+    t0 = time.time()
+    model.prepare(copt)
+    log(f"CCD set-up {time.time() - t0:.1f}s")
+    model.ccd_epoch(copt)  # warm-up (builds the residual copies)
+    ctx.reset_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(epochs):
+        model.ccd_epoch(copt)
+    ctx.synchronize()
+    w = (time.perf_counter() - t0) / epochs
+    n_c, ms_c = ctx.kernel_stats("ccd_col")
+    ev = model.evaluate(lam)
+    col_ms = ms_c / max(n_c, 1)
+    N = 3
+    alg_epoch = R * (N * 20 + 56) * nnz  # per (column, mode) pass 20 B/nnz; per column 56 B/nnz add-back + subtract
+    log(f"CCD++ {w * 1e3:.1f} ms/epoch ({nnz / w / 1e9:.3f} Gnnz/s), column pass {col_ms:.3f} ms, "
+        f"rmse_te {ev['rmse_te']:.4f}")
+    return {"s_per_epoch": w, "nnz_per_s": nnz / w, "col_kernel_ms": col_ms,
+            "alg_bytes_per_nnz_epoch": R * (N * 20 + 56), "alg_gbs": alg_epoch / w / 1e9,
+            "alg_frac_of_hbm": alg_epoch / w / 1e9 / HBM_PEAK_GBS, "rmse_te": ev["rmse_te"], "epochs": 1 + epochs}
+
+
+def als_cfg2_leg(ct, ctx, model, nnz, R, lam, F0, epochs, tile_len):
+    """ALS on the headline tensor: both FP32 kernel families and the mixed
+    (BF16-operand) path, each from the same init; the mixed path's test-RMSE
+    gap is against the FP32 spans path (whose updates match the FP64 oracle
+    to 1e-5 in the parity tests)."""
+    res_als = {}
+    for kname, kern, prec in (("spans", ct.ALS_SPANS, ct.PREC_FP32), ("rows", ct.ALS_ROWS, ct.PREC_FP32),
+                              ("mixed", ct.ALS_ROWS, ct.PREC_MIXED)):
+        aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=tile_len or None, als_kernel=kern,
+                          precision=prec, refine=1 if prec == ct.PREC_MIXED else 0)
+        model.set_factors(F0)
+        t0 = time.time()
+        model.prepare(aopt)
+        log(f"ALS ({kname}) set-up {time.time() - t0:.1f}s")
+        for md in range(3):
+            model.als_mode_update(md, aopt)  # warm-up
+        model.set_factors(F0)
+        ctx.reset_stats()
+        ctx.synchronize()
+        wsum, rms = 0.0, []
+        for _ in range(epochs):
+            t1 = time.perf_counter()
+            for md in range(3):
+                model.als_mode_update(md, aopt)
+            ctx.synchronize()
+            wsum += time.perf_counter() - t1
+            rms.append(model.evaluate(lam)["rmse_te"])  # untimed
+        w = wsum / epochs
+        r = {"s_per_epoch": w, "nnz_per_s": nnz / w, "rmse_te_per_epoch": [round(x, 6) for x in rms],
+             "epochs": 1 + epochs}
+        if kern == ct.ALS_SPANS:
+            n_g, ms_g = ctx.kernel_stats("als_gram")
+            n_s, ms_s = ctx.kernel_stats("als_solve")
+            gram_ms = ms_g / max(n_g, 1)
+            r.update({"gram_kernel_ms_avg": gram_ms, "solve_kernel_ms_avg": ms_s / max(n_s, 1),
+                      "gram_tflops_fp64": R * (R + 1) * nnz / (gram_ms * 1e-3) / 1e12 if gram_ms else None})
+        res_als[kname] = r
+        log(f"ALS {kname}: {w * 1e3:.1f} ms/epoch ({nnz / w / 1e9:.3f} Gnnz/s), rmse_te {[round(x, 5) for x in rms]}")
+    base = res_als["spans"]
+    mx = res_als["mixed"]
+    mx["speedup_vs_fp32_spans"] = base["s_per_epoch"] / mx["s_per_epoch"]
+    mx["speedup_vs_fp32_rows"] = res_als["rows"]["s_per_epoch"] / mx["s_per_epoch"]
+    mx["rmse_te_gap_vs_fp32"] = float(max(abs(x - y) for x, y in zip(mx["rmse_te_per_epoch"],
+                                                                     base["rmse_te_per_epoch"])))
+    return res_als
+
+
+def mixed_leg(ct, ctx, a):
     """ALS at R=128 in the BF16-operand path on a 5M x 5M x 1000 tensor with
     500M nonzeros (modes 0-1 Zipf(1.1)), values from a rank-32 CP model mapped
-    to [1,5] plus N(0,0.2), 90/10 split, lambda 0.05.  A sample of rows of the
-    first mode update is checked against an FP64 numpy solve of the same
-    normal equations (Gram of the other modes' rows + lambda I) from the same
-    inputs."""
+    to [1,5] plus N(0,0.2), 90/10 split, lambda 0.05, against the plain FP32
+    path (FP64-accumulated Gram, same solves) from the same init.  In the
+    first sweep of each path, 1000 sampled rows of every mode's update are
+    compared with the FP64 ridge solution of the same normal equations from the
+    same input factors (the oracle's orc_als_rows_f64)."""
     import numpy as np
+    import oracle as O
     R, lam = 128, 0.05
     dims = (5_000_000, 5_000_000, 1000)
     t0 = time.time()
@@ -55,97 +269,83 @@ def mixed_leg(ct, ctx, a, log):
                       noise_std=0.2, test_fraction=0.1, seed=0x5EED0005)
     log(f"mixed: tensor train {tr.nnz} test {te.nnz} in {time.time() - t0:.1f}s")
     m = ct.Model(ctx, tr, None, te, rank=R)
-    m.init_random(5)
-    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, precision=ct.PREC_MIXED, refine=1)
-    t0 = time.time()
-    m.prepare(opt)
-    log(f"mixed: CISS build+upload {time.time() - t0:.1f}s")
     res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam}
-    # sampled FP64 check of the first mode-0 update
-    if not a.no_cpu:
+    nth = os.cpu_count() or 1
+    check = not a.no_cpu
+    if check:
         t0 = time.time()
-        F = [m.factor(md) for md in range(3)]
-        ind, vals = tr.arrays()  # the training tensor is in mode-0 order
+        ind, vals = tr.arrays()
+        ot = O.Coo(ind, vals, dims)
+        del ind, vals
+        orders = [O.mode_order(ot, md) for md in range(3)]
         rng = np.random.default_rng(0)
-        rows = np.sort(rng.choice(dims[0], a.mixed_sample, replace=False)).astype(np.uint32)
-        lo = np.searchsorted(ind[0], rows, "left")
-        hi = np.searchsorted(ind[0], rows, "right")
-        ref = np.zeros((len(rows), R))
-        for k in range(len(rows)):
-            if hi[k] == lo[k]:
-                continue  # a row without nonzeros stays 0
-            q = slice(lo[k], hi[k])
-            H = F[1][ind[1][q]].astype(np.float64) * F[2][ind[2][q]].astype(np.float64)
-            ref[k] = np.linalg.solve(H.T @ H + lam * np.eye(R), H.T @ vals[q])
-        log(f"mixed: FP64 solve of {len(rows)} sampled rows in {time.time() - t0:.1f}s")
-        del ind, vals, F
-    ctx.reset_stats()
-    m.als_mode_update(0, opt)
-    if not a.no_cpu:
-        got = m.factor(0)[rows].astype(np.float64)
-        nz = np.linalg.norm(ref, axis=1) > 0
-        rel = np.linalg.norm(got[nz] - ref[nz], axis=1) / np.linalg.norm(ref[nz], axis=1)
-        res["sampled_rows"] = int(nz.sum())
-        res["row_rel_err_vs_fp64_median"] = float(np.median(rel))
-        res["row_rel_err_vs_fp64_max"] = float(rel.max())
-        res["factor_rel_err_vs_fp64_sampled"] = float(np.linalg.norm(got[nz] - ref[nz]) / np.linalg.norm(ref[nz]))
-        log(f"mixed: sampled rows rel err vs FP64 median {np.median(rel):.2e} max {rel.max():.2e}")
-    for md in (1, 2):
-        m.als_mode_update(md, opt)
-    rm = [m.evaluate(lam)["rmse_te"]]
-    ctx.reset_stats()
-    wall = 0.0
-    for _ in range(a.mixed_iters):
-        ctx.synchronize()
-        t0 = time.perf_counter()
+        rows = []
         for md in range(3):
+            rp = orders[md][1]
+            cnt = np.diff(rp)
+            nonempty = np.nonzero(cnt)[0]
+            pick = rng.choice(nonempty, min(a.mixed_sample, len(nonempty)), replace=False)
+            # at most 2e7 nonzeros of FP64 Gram work per mode (mode 2's rows hold ~450K each)
+            keep = max(1, int(np.searchsorted(np.cumsum(cnt[pick]), 20_000_000)))
+            rows.append(np.sort(pick[:keep]).astype(np.uint32))
+        log(f"mixed: host copies and mode orders for the FP64 check {time.time() - t0:.1f}s")
+    for pname, opt in (("mixed", ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, precision=ct.PREC_MIXED, refine=1)),
+                       ("fp32", ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1))):
+        m.init_random(5)
+        t0 = time.time()
+        m.prepare(opt)
+        log(f"mixed: {pname} set-up {time.time() - t0:.1f}s")
+        errs = []
+        for md in range(3):  # first sweep, checked per mode
+            if check:
+                F = [m.factor(k) for k in range(3)]
+                t1 = time.time()
+                ref = O.als_rows(ot, md, F, lam, rows[md], order=orders[md], nthreads=nth)
+                del F
             m.als_mode_update(md, opt)
-        ctx.synchronize()
-        wall += time.perf_counter() - t0
-        rm.append(m.evaluate(lam)["rmse_te"])  # untimed
-    wall /= a.mixed_iters
-    ni, msi = ctx.kernel_stats("als_mixed_items")
-    nc, msc = ctx.kernel_stats("als_mixed_chunks")
-    nh, msh = ctx.kernel_stats("als_mixed_heavy")
-    nt, mst = ctx.kernel_stats("als_mixed_thin")
-    nd, msd = ctx.kernel_stats("als_mixed_dual")
-    res.update({"s_per_iter": wall, "nnz_per_s": tr.nnz / wall,
-                "rows_kernel_ms_per_iter": msi / a.mixed_iters, "chunks_kernel_ms_per_iter": msc / a.mixed_iters,
-                "heavy_kernel_ms_per_iter": msh / a.mixed_iters, "thin_kernel_ms_per_iter": mst / a.mixed_iters,
-                "dual_kernel_ms_per_iter": msd / a.mixed_iters,
-                "rmse_te_per_iter": [round(x, 5) for x in rm], "iters": 1 + a.mixed_iters})
-    log(f"mixed ALS R=128: {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), long-row chunks {msc / a.mixed_iters:.0f} ms, "
-        f"rows {msi / a.mixed_iters:.0f} ms, long-row solves {msh / a.mixed_iters:.0f} ms, thin rows "
-        f"{mst / a.mixed_iters:.0f} ms, 33-64-nonzero rows (dual) {msd / a.mixed_iters:.0f} ms per iteration; rmse_te {rm}")
-    # the plain FP32 path (FP64-accumulated Gram, same solves) from the same init
-    fopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, als_kernel=ct.ALS_ROWS, refine=1)
-    m.init_random(5)
-    m.prepare(fopt)
-    w32, r32 = 0.0, []
-    for _ in range(1 + a.mixed_iters):
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for md in range(3):
-            m.als_mode_update(md, fopt)
-        ctx.synchronize()
-        w32 += time.perf_counter() - t0
-        r32.append(m.evaluate(lam)["rmse_te"])
-    w32 /= 1 + a.mixed_iters
-    res.update({"fp32_s_per_iter": w32, "fp32_nnz_per_s": tr.nnz / w32, "speedup_vs_fp32": w32 / wall,
-                "fp32_rmse_te_per_iter": [round(x, 5) for x in r32],
-                "rmse_te_gap_vs_fp32_path": [round(abs(x - y), 6) for x, y in zip(rm, r32)]})
-    log(f"FP32 row path R=128: {w32:.3f} s/iter; mixed speed-up {w32 / wall:.2f}x; rmse_te {r32}")
+            if check:
+                got = m.factor(md)[rows[md]].astype(np.float64)
+                nz = np.linalg.norm(ref, axis=1) > 0
+                rel = np.linalg.norm(got[nz] - ref[nz], axis=1) / np.linalg.norm(ref[nz], axis=1)
+                errs.append({"mode": md, "rows": int(nz.sum()),
+                             "factor_rel_err": float(np.linalg.norm(got[nz] - ref[nz]) / np.linalg.norm(ref[nz])),
+                             "row_rel_err_median": float(np.median(rel)), "row_rel_err_max": float(rel.max())})
+                log(f"mixed: {pname} mode {md}: sampled rows vs FP64 factor rel err {errs[-1]['factor_rel_err']:.2e} "
+                    f"(FP64 solves {time.time() - t1:.1f}s)")
+        rm = [m.evaluate(lam)["rmse_te"]]
+        ctx.reset_stats()
+        wall = 0.0
+        for _ in range(a.mixed_iters):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for md in range(3):
+                m.als_mode_update(md, opt)
+            ctx.synchronize()
+            wall += time.perf_counter() - t0
+            rm.append(m.evaluate(lam)["rmse_te"])  # untimed
+        wall /= a.mixed_iters
+        r = {"s_per_iter": wall, "nnz_per_s": tr.nnz / wall, "rmse_te_per_iter": [round(x, 6) for x in rm],
+             "iters": 1 + a.mixed_iters, "first_sweep_vs_fp64_sampled_rows": errs}
+        if pname == "mixed":
+            for k, lab in (("als_mixed_items", "rows"), ("als_mixed_chunks", "chunks"), ("als_mixed_heavy", "heavy"),
+                           ("als_mixed_thin", "thin"), ("als_mixed_dual", "dual")):
+                r[f"{lab}_kernel_ms_per_iter"] = ctx.kernel_stats(k)[1] / a.mixed_iters
+        res[pname] = r
+        log(f"mixed: {pname} R=128 {wall:.3f} s/iter ({tr.nnz / wall / 1e9:.3f} Gnnz/s), rmse_te {rm}")
+    res["speedup_vs_fp32"] = res["fp32"]["s_per_iter"] / res["mixed"]["s_per_iter"]
+    res["rmse_te_gap_vs_fp32_path"] = float(max(abs(x - y) for x, y in zip(res["mixed"]["rmse_te_per_iter"],
+                                                                           res["fp32"]["rmse_te_per_iter"])))
     return res
 
 
-def adjoint_leg(ct, ctx, a, log):
-    """The loss adjoint behind torch.autograd on the 4-way tensor 1M x 500K x
-    1000 x 24 with 140M nonzeros (modes 0-1 Zipf(1.0)), values from a rank-16
-    CP model plus N(0, 0.2), R = 32: full-gradient steps with torch's Adam
-    through `completion_loss` (the adjoint kernel returns the loss and every
-    factor gradient in one walk per mode), and a central finite-difference
-    check (eps = 1e-3, the loss in FP64 numpy) of the gradient on a
-    1000-nonzero subsample."""
+def adjoint_leg(ct, ctx, a):
+    """SGD through the loss adjoint behind torch.autograd on the 4-way tensor
+    1M x 500K x 1000 x 24 with 140M nonzeros (modes 0-1 Zipf(1.0)), values from
+    a rank-16 CP model plus N(0, 0.2), R = 32: full-gradient steps of
+    torch.optim.SGD on the mean loss through `completion_loss` (the adjoint
+    kernel returns the loss and every factor gradient in one walk per mode),
+    and a central finite-difference check (eps = 1e-3, the loss in FP64 numpy)
+    of the gradient on a 1000-nonzero subsample."""
     import numpy as np
     import torch
     from cutc_amd.autograd import completion_loss
@@ -157,8 +357,11 @@ def adjoint_leg(ct, ctx, a, log):
     log(f"adjoint: tensor train {tr.nnz} test {te.nnz} in {time.time() - t0:.1f}s")
     m = ct.Model(ctx, tr, None, te, rank=R)
     g = torch.Generator(device="cuda").manual_seed(3)
-    F = [((torch.rand(d, R, device="cuda", generator=g) * 2 - 1) * 0.5).requires_grad_() for d in dims]
-    opt = torch.optim.Adam(F, lr=a.adjoint_lr)
+    # U(0, 0.6) entries: the truth's factors are U(0, 1), and a zero-mean init
+    # starts full-gradient descent at the cubic model's saddle at 0
+    F = [(torch.rand(d, R, device="cuda", generator=g) * 0.6).requires_grad_() for d in dims]
+    opt = torch.optim.SGD(F, lr=a.adjoint_lr)
+    scale = 1.0 / tr.nnz
     loss = completion_loss(m, F, lam)  # first call builds the per-mode CISS copies
     ctx.reset_stats()
     losses, times = [], []
@@ -166,7 +369,7 @@ def adjoint_leg(ct, ctx, a, log):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         opt.zero_grad()
-        loss = completion_loss(m, F, lam)
+        loss = completion_loss(m, F, lam) * scale
         loss.backward()
         opt.step()
         torch.cuda.synchronize()
@@ -174,15 +377,17 @@ def adjoint_leg(ct, ctx, a, log):
         losses.append(float(loss.detach()))
     n_a, ms_a = ctx.kernel_stats("adjoint")
     per_mode_ms = ms_a / max(n_a, 1)
-    alg = 16 + 12 * R  # per nonzero per mode: three indices, the value, three gathered rows
-    res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam, "optimizer": f"Adam lr {a.adjoint_lr}",
+    N = 4
+    alg = 4 + 4 * (N - 1) + 4 * R * (N - 1) + 4 * R  # stream, the other rows gathered, the own row read/written
+    res = {"dims": list(dims), "train_nnz": tr.nnz, "rank": R, "lambda": lam,
+           "optimizer": f"torch.optim.SGD lr {a.adjoint_lr} on the mean loss",
            "step_ms": 1e3 * float(np.median(times)), "adjoint_ms_per_mode": per_mode_ms,
-           "adjoint_alg_gbs": alg * tr.nnz / (per_mode_ms * 1e-3) / 1e9,
-           "nnz_per_s": tr.nnz / float(np.median(times)), "loss_per_step": [round(x, 2) for x in losses]}
+           "adjoint_alg_bytes_per_nnz_mode": alg,
+           "roofline": {"bound": "hbm", "achieved": alg * tr.nnz / (per_mode_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": alg * tr.nnz / (per_mode_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "nnz_per_s": tr.nnz / float(np.median(times)), "mean_loss_per_step": [round(x, 6) for x in losses]}
     log(f"adjoint: step {res['step_ms']:.1f} ms, adjoint {per_mode_ms:.2f} ms per mode "
-        f"({res['adjoint_alg_gbs']:.0f} GB/s algorithmic), loss {losses[0]:.4g} -> {losses[-1]:.4g}")
-    # gradient of the loss on a 1000-nonzero subsample against central finite
-    # differences of the same loss evaluated in FP64 with numpy
+        f"({res['roofline']['frac']:.2f} of HBM), mean loss {losses[0]:.4g} -> {losses[-1]:.4g}")
     ind, vals = tr.arrays()
     rng = np.random.default_rng(0)
     pick = np.sort(rng.choice(tr.nnz, 1000, replace=False))
@@ -192,8 +397,6 @@ def adjoint_leg(ct, ctx, a, log):
     ms = ct.Model(ctx, ct.Coo.from_arrays(sub_ind, sub_vals, dims), None, None, rank=R)
     Fd = [f.detach().clone().requires_grad_() for f in F]
     completion_loss(ms, Fd, lam).backward()
-    # the subsample's rows of each factor in FP64; the loss's data term only
-    # reads these
     H = [f.detach()[torch.from_numpy(ix.astype(np.int64)).cuda()].double().cpu().numpy() for f, ix in zip(Fd, sub_ind)]
 
     def data_loss(H):
@@ -216,9 +419,90 @@ def adjoint_leg(ct, ctx, a, log):
         errs.append(abs(gg - fd) / (abs(fdd) + abs(lam * old) + 1e-12))
     res["fd_check_entries"] = len(errs)
     res["fd_max_rel_err"] = float(max(errs))
-    log(f"adjoint: central differences (eps 1e-3, FP64) on {len(errs)} gradient entries of a 1000-nonzero "
-        f"subsample, max rel err {max(errs):.2e}")
+    log(f"adjoint: central differences on {len(errs)} gradient entries, max rel err {max(errs):.2e}")
     return res
+
+
+def batch_leg(ct, ctx, a, lam):
+    import numpy as np
+    from cutc_amd.workloads import batch_problems, init_factors
+    t0 = time.time()
+    probs = batch_problems(a.batch_problems)
+    log(f"batch: {len(probs)} problems, {sum(p[0].nnz for p in probs)} train nnz, generated in "
+        f"{time.time() - t0:.1f}s")
+    t0 = time.time()
+    batch = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    brng = np.random.default_rng(4)
+    Fb = [init_factors(batch.dims[i], p[2], brng) for i, p in enumerate(probs)]
+
+    def load():
+        for i in range(len(probs)):
+            for m in range(3):
+                batch.set_factor(i, m, Fb[i][m])
+
+    bopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam)
+    load()
+    tuned = batch.autotune(bopt)
+    log(f"batch setup+autotune {time.time() - t0:.1f}s: {tuned}")
+    ms_eager = batch.als(bopt, iters=a.batch_iters, graph=False)
+    load()
+    batch.als(bopt, iters=a.batch_iters, graph=True)  # capture + first replay
+    reps = []
+    for _ in range(3):
+        load()
+        reps.append(batch.als(bopt, iters=a.batch_iters, graph=True))
+    ms_graph = float(np.median(reps))
+    rtr, rte = batch.rmse()
+    bn = batch.train_nnz
+    out = {"problems": len(probs), "classes": batch.classes(), "train_nnz": bn, "iters": a.batch_iters,
+           "graph_ms": ms_graph, "direct_ms": ms_eager, "nnz_per_s": bn * a.batch_iters / (ms_graph * 1e-3),
+           "s_per_iter": ms_graph * 1e-3 / a.batch_iters,
+           "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
+                     for (r, _), t in zip(batch.classes(), tuned)],
+           "rmse_te_median": float(np.median(rte)), "rmse_tr_median": float(np.median(rtr))}
+    log(f"batch ALS: {a.batch_iters} iters of {len(probs)} problems in {ms_graph:.1f} ms (graph), "
+        f"{ms_eager:.1f} ms (direct); median rmse_te {np.median(rte):.4f}")
+    mopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam, precision=ct.PREC_MIXED)
+    load()
+    mtuned = batch.autotune(mopt)
+    load()
+    batch.als(mopt, iters=a.batch_iters, graph=True)
+    mreps = []
+    for _ in range(3):
+        load()
+        mreps.append(batch.als(mopt, iters=a.batch_iters, graph=True))
+    ms_mixed = float(np.median(mreps))
+    _, mrte = batch.rmse()
+    gap = np.abs(mrte - rte)
+    ranks = np.array([p[2] for p in probs])
+    out["mixed"] = {
+        "graph_ms": ms_mixed, "nnz_per_s": bn * a.batch_iters / (ms_mixed * 1e-3), "speedup_vs_fp32": ms_graph / ms_mixed,
+        "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
+                  for (r, _), t in zip(batch.classes(), mtuned)],
+        "rmse_te_gap_vs_fp32_max": float(gap.max()), "rmse_te_gap_vs_fp32_median": float(np.median(gap)),
+        "rmse_te_gap_vs_fp32_by_rank": {int(r): {"median": float(np.median(gap[ranks == r])),
+                                                 "max": float(gap[ranks == r].max())} for r in np.unique(ranks)}}
+    log(f"batch ALS mixed: {ms_mixed:.1f} ms (graph), {ms_graph / ms_mixed:.2f}x the FP32 run; test-RMSE gap median "
+        f"{np.median(gap):.1e}, max {gap.max():.1e}")
+    return out
+
+
+def share_leg(ct, ctx, a, nnz, R, lam):
+    """One GPU's share of the sharded SGD: the same generator at a share's
+    nonzero count, through the same kernels and default schedule."""
+    import numpy as np
+    tr, te = cfg2_tensor(ct, nnz)
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    rng = np.random.default_rng(1)
+    F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in tr.dims]
+    times, traj, walk_ms, walks, merge_ms, _ = sgd_run(ct, ctx, m, tr.nnz, R, lam, 5, 1, F0)
+    w = float(np.median(times))
+    alg = (12 + 24 * R) * tr.nnz / walks
+    r = {"nnz": nnz, "train_nnz": tr.nnz, "s_per_epoch": w, "nnz_per_s": tr.nnz / w,
+         "walk_kernel_ms": walk_ms, "walk_launches_per_epoch": walks, "merge_ms_per_epoch": merge_ms,
+         "roofline_frac": alg / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rmse_te_per_epoch": [round(x, 5) for x in traj]}
+    log(f"share {nnz}: {w * 1e3:.2f} ms/epoch ({tr.nnz / w / 1e9:.2f} Gnnz/s), walk roofline {r['roofline_frac']:.2f}")
+    return r
 
 
 def main():
@@ -230,23 +514,21 @@ def main():
     ap.add_argument("--rank", type=int, default=32)
     ap.add_argument("--ccd-epochs", type=int, default=3)
     ap.add_argument("--als-epochs", type=int, default=3)
-    ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default")
-    ap.add_argument("--sgd-update", default="auto", choices=["auto", "atomic", "racy", "next_racy", "hot_atomic"],
-                    help="how concurrent walkers combine updates of shared rows (see ctc_sgd_update)")
-    ap.add_argument("--sgd-hot-rows", type=int, default=0,
-                    help="auto policy: rows with more nonzeros than this are updated racily (0 = default)")
-    ap.add_argument("--cpu-sample", type=int, default=0, help="training nonzeros in the CPU sample (0 = all)")
-    ap.add_argument("--cpu-epochs", type=int, default=2)
+    ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default (CCD++ / ALS legs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--algs", default="sgd,ccd,als,batch")
+    ap.add_argument("--algs", default="share,cfg1,mttkrp,ccd,als,batch,adjoint,mixed",
+                    help="legs after the SGD headline")
+    ap.add_argument("--budget", type=float, default=560.0,
+                    help="seconds; a leg that would start later is skipped and recorded as such")
+    ap.add_argument("--shares", default="50000000,12500000")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--batch-iters", type=int, default=10)
     ap.add_argument("--mixed-nnz", type=int, default=500_000_000)
-    ap.add_argument("--mixed-iters", type=int, default=5)
+    ap.add_argument("--mixed-iters", type=int, default=3)
     ap.add_argument("--mixed-sample", type=int, default=1000)
     ap.add_argument("--adjoint-nnz", type=int, default=140_000_000)
     ap.add_argument("--adjoint-steps", type=int, default=10)
-    ap.add_argument("--adjoint-lr", type=float, default=0.01)
+    ap.add_argument("--adjoint-lr", type=float, default=50.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solvers run on a single "
@@ -257,262 +539,139 @@ def main():
     import cutc_amd as ct
 
     R, lam = a.rank, 0.05
-    dims = (480189, 17770, 2182)
     t0 = time.time()
-    train, test = ct.synth(dims, a.nnz, zipf=(1.0, 1.0, 0.0), true_rank=10,
-                           value_map=ct.VALUES_INT_1_5, noise_std=0.3, test_fraction=0.01,
-                           seed=0x5EED0002)
+    train, test = cfg2_tensor(ct, a.nnz)
+    dims = train.dims
+    train_nnz = train.nnz
     log(f"synthetic tensor: train {train.nnz} test {test.nnz} in {time.time() - t0:.1f}s")
     ctx = ct.Context(0)
     model = ct.Model(ctx, train, None, test, rank=R)
     ctx.set_profiling(True)
-    algs = set(a.algs.split(","))
+    algs = [x for x in a.algs.split(",") if x]
     out_algs = {}
 
-    # ------------------------------------------------------------------ SGD
+    # ------------------------------------------------------- SGD (headline)
     rng = np.random.default_rng(1)  # the oracle trajectory fixture starts from this init
     F0 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
     model.set_factors(F0)
-    policy = {"auto": ct.SGD_AUTO, "atomic": ct.SGD_ATOMIC, "racy": ct.SGD_RACY,
-              "next_racy": ct.SGD_NEXT_RACY, "hot_atomic": ct.SGD_HOT_ATOMIC}[a.sgd_update]
-    opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3,
-                     lr_schedule=ct.LR_EXP_DECAY, lr_decay=0.95, sgd_scale=1.0,
-                     tile_len=a.tile_len or None, sgd_update=policy, sgd_hot_rows=a.sgd_hot_rows)
-    t0 = time.time()
-    model.prepare(opt)
-    log(f"SGD CISS build+upload {time.time() - t0:.1f}s")
     rm0 = model.evaluate(lam)["rmse_te"]
-    lr, times, traj = 1e-3, [], []
-    for e in range(a.warmup + a.steps):
-        if e == a.warmup:
-            ctx.reset_stats()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        model.sgd_epoch(opt, lr=lr)  # returns after the stream is drained
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
-        if e >= a.warmup:
-            times.append(dt)
-        lr *= 0.95
-        traj.append(model.evaluate(lam)["rmse_te"])  # untimed
-    n_sgd, ms_sgd = ctx.kernel_stats("sgd")
+    times, traj, walk_ms, walks, merge_ms, opt = sgd_run(ct, ctx, model, train.nnz, R, lam, a.steps, a.warmup, F0)
     wall = sum(times)
     value = train.nnz * a.steps / wall
-    kern_ms = ms_sgd / max(n_sgd, 1)
+    c = opt.to_c()
     alg_bytes_per_nnz = 12 + 24 * R
-    achieved = alg_bytes_per_nnz * train.nnz / (kern_ms * 1e-3) / 1e9
-    ref = None
-    fx = os.path.join(ROOT, "tests", "golden", "sgd_cfg2_oracle.json")
+    alg_per_launch = alg_bytes_per_nnz * train.nnz / walks  # one walk launch runs one sub-step of every walker
+    achieved = alg_per_launch / (walk_ms * 1e-3) / 1e9
+    ref, gap = None, None
+    fx = os.path.join(ROOT, "tests", "golden", "sgd_cfg2_sync_oracle.npz")
     if a.nnz == 100_000_000 and R == 32 and os.path.exists(fx):
-        ref = json.load(open(fx))["rmse_te"]
-    gap = max(abs(x - y) for x, y in zip(traj, ref)) if ref else None
-    out_algs["sgd"] = {"s_per_epoch": wall / a.steps, "nnz_per_s": value, "kernel_ms": kern_ms,
-                       "rmse_te_initial": rm0, "rmse_te_per_epoch": [round(x, 5) for x in traj],
-                       "oracle_fp64_rmse_te_per_epoch": [round(x, 5) for x in ref] if ref else None,
-                       "max_abs_gap_vs_oracle": gap, "epochs": a.warmup + a.steps,
-                       "tile_len": opt.to_c().tile_len, "policy": a.sgd_update,
-                       "hot_row_threshold": opt.to_c().sgd_hot_rows or 65536, "alg_bytes_per_nnz": alg_bytes_per_nnz}
-    log(f"SGD {wall / a.steps * 1e3:.2f} ms/epoch, kernel {kern_ms:.2f} ms, {value / 1e9:.2f} Gnnz/s, "
-        f"rmse_te {rm0:.4f} -> {traj[-1]:.4f}; max gap vs FP64 sequential oracle {gap}")
+        g = np.load(fx, allow_pickle=False)
+        if (int(g["span"]), int(g["sub"]), int(g["hot"])) == (c.tile_len, c.sgd_substep, c.sgd_hot_rows):
+            ref = [float(x) for x in g["rmse_te"]]
+    if ref:
+        n = min(len(ref), len(traj))
+        gap = max(abs(x - y) for x, y in zip(traj[:n], ref[:n]))
+    parity_ok = gap is not None and gap < 1e-3
+    seq = None
+    fs = os.path.join(ROOT, "tests", "golden", "sgd_cfg2_oracle.json")
+    if a.nnz == 100_000_000 and R == 32 and os.path.exists(fs):
+        seq = json.load(open(fs))["rmse_te"]
+    out_algs["sgd"] = {
+        "schedule": "bulk-synchronous walkers (default)", "span": c.tile_len, "sub_step": c.sgd_substep,
+        "hot_row_threshold": c.sgd_hot_rows, "s_per_epoch": wall / a.steps, "nnz_per_s": value,
+        "walk_kernel_ms_avg": walk_ms, "walk_launches_per_epoch": walks, "merge_ms_per_epoch": merge_ms,
+        "rmse_te_initial": rm0, "rmse_te_per_epoch": [round(x, 6) for x in traj],
+        "oracle_fp64_same_schedule_rmse_te": [round(x, 6) for x in ref] if ref else None,
+        "max_abs_gap_vs_oracle": gap, "parity_ok": parity_ok,
+        "sequential_rule_fp64_rmse_te": [round(x, 5) for x in seq] if seq else None,
+        "gap_to_sequential_rule": ([round(x - y, 4) for x, y in zip(traj, seq)] if seq else None),
+        "epochs": a.warmup + a.steps, "alg_bytes_per_nnz": alg_bytes_per_nnz}
+    log(f"SGD {wall / a.steps * 1e3:.2f} ms/epoch, walk {walk_ms:.3f} ms x {walks:.0f}, merges {merge_ms:.2f} ms, "
+        f"{value / 1e9:.2f} Gnnz/s, rmse_te {rm0:.4f} -> {traj[-1]:.4f}; max gap vs FP64 oracle {gap}")
 
-    # -------------------------------------------------------------- MTTKRP
-    if "mttkrp" in algs or "als" in algs:
-        ctx.reset_stats()
-        for md in range(3):
-            model.mttkrp(md)  # warm-up (builds the CISS of each mode)
-        ctx.reset_stats()
-        per_mode = []
-        for md in range(3):
-            for _ in range(3):
-                model.mttkrp(md)
-            n_m, ms_m = ctx.kernel_stats("mttkrp")
-            per_mode.append(ms_m / max(n_m, 1))
-            ctx.reset_stats()
-        mb = 12 + 8 * R  # stream + two gathered rows per nonzero
-        out_algs["mttkrp"] = {"kernel_ms_per_mode": per_mode, "alg_bytes_per_nnz": mb,
-                              "alg_gbs_per_mode": [mb * train.nnz / (t * 1e-3) / 1e9 for t in per_mode]}
-        log(f"MTTKRP kernel ms per mode {[round(t, 3) for t in per_mode]}, algorithmic GB/s "
-            f"{[round(mb * train.nnz / (t * 1e-3) / 1e9) for t in per_mode]}")
-
-    # --------------------------------------------------------------- CCD++
-    if "ccd" in algs and a.ccd_epochs > 0:
-        copt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=lam, tile_len=a.tile_len or None)
-        model.set_factors(F0)
-        t0 = time.time()
-        model.prepare(copt)
-        log(f"CCD CISS build+upload {time.time() - t0:.1f}s")
-        model.ccd_epoch(copt)  # warm-up (builds the residual copies)
-        ctx.reset_stats()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.ccd_epochs):
-            model.ccd_epoch(copt)
-        ctx.synchronize()
-        w = (time.perf_counter() - t0) / a.ccd_epochs
-        n_c, ms_c = ctx.kernel_stats("ccd_col")
-        ev = model.evaluate(lam)
-        col_ms = ms_c / max(n_c, 1)
-        # bytes per nonzero per column pass: 2 index streams + residual read/write
-        # (16 B) + the gathered column entries (3 N - 1 = 8 x 4 B)
-        col_bytes = (8 + 8 + 32) * train.nnz
-        out_algs["ccd"] = {"s_per_epoch": w, "nnz_per_s": train.nnz / w, "col_kernel_ms": col_ms,
-                           "col_kernel_gbs": col_bytes / (col_ms * 1e-3) / 1e9,
-                           "rmse_te": ev["rmse_te"], "epochs": 1 + a.ccd_epochs}
-        log(f"CCD++ {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), column pass {col_ms:.3f} ms, "
-            f"rmse_te {ev['rmse_te']:.4f}")
-
-    # ----------------------------------------------------------------- ALS
-    if "als" in algs and a.als_epochs > 0:
-        res_als = {}
-        for kname, kern in (("rows", ct.ALS_ROWS), ("spans", ct.ALS_SPANS)):
-            aopt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=lam, tile_len=a.tile_len or None,
-                              als_kernel=kern)
-            model.set_factors(F0)
-            t0 = time.time()
-            model.prepare(aopt)
-            log(f"ALS ({kname}) set-up {time.time() - t0:.1f}s")
-            for md in range(3):
-                model.als_mode_update(md, aopt)  # warm-up
-            ctx.reset_stats()
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(a.als_epochs):
-                for md in range(3):
-                    model.als_mode_update(md, aopt)
-            ctx.synchronize()
-            w = (time.perf_counter() - t0) / a.als_epochs
-            ev = model.evaluate(lam)
-            r = {"s_per_epoch": w, "nnz_per_s": train.nnz / w, "rmse_te": ev["rmse_te"], "epochs": 1 + a.als_epochs}
-            if kern == ct.ALS_ROWS:
-                n_i, ms_i = ctx.kernel_stats("als_rows_items")
-                n_h, ms_h = ctx.kernel_stats("als_rows_heavy")
-                n_c, ms_c = ctx.kernel_stats("als_rows_chunks")
-                n_t, ms_t = ctx.kernel_stats("als_rows_thin")
-                nm = 3 * a.als_epochs
-                r.update({"rows_kernel_ms_per_mode": ms_i / nm, "chunks_kernel_ms_per_mode": ms_c / nm,
-                          "heavy_kernel_ms_per_mode": ms_h / nm, "thin_kernel_ms_per_mode": ms_t / nm})
-                log(f"ALS rows {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), per mode: chunks "
-                    f"{ms_c / nm:.2f} ms, rows {ms_i / nm:.2f} ms, long-row solves {ms_h / nm:.2f} ms, thin rows "
-                    f"{ms_t / nm:.2f} ms; rmse_te {ev['rmse_te']:.4f}")
-            else:
-                n_g, ms_g = ctx.kernel_stats("als_gram")
-                n_s, ms_s = ctx.kernel_stats("als_solve")
-                gram_ms = ms_g / max(n_g, 1)
-                r.update({"gram_kernel_ms_avg": gram_ms, "solve_kernel_ms_avg": ms_s / max(n_s, 1),
-                          "gram_tflops_fp64": R * (R + 1) * train.nnz / (gram_ms * 1e-3) / 1e12 if gram_ms else None})
-                log(f"ALS spans {w * 1e3:.1f} ms/epoch ({train.nnz / w / 1e9:.3f} Gnnz/s), gram {gram_ms:.2f} ms, "
-                    f"solve {ms_s / max(n_s, 1):.2f} ms, rmse_te {ev['rmse_te']:.4f}")
-            res_als[kname] = r
-        out_algs["als"] = res_als
-
-    # ------------------------------------------------------- batched ALS
-    if "batch" in algs and a.batch_problems > 0:
-        from cutc_amd.workloads import batch_problems, init_factors
-        t0 = time.time()
-        probs = batch_problems(a.batch_problems)
-        log(f"batch: {len(probs)} problems, {sum(p[0].nnz for p in probs)} train nnz, generated in "
-            f"{time.time() - t0:.1f}s")
-        t0 = time.time()
-        batch = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
-        brng = np.random.default_rng(4)
-        Fb = [init_factors(batch.dims[i], p[2], brng) for i, p in enumerate(probs)]
-
-        def load():
-            for i in range(len(probs)):
-                for m in range(3):
-                    batch.set_factor(i, m, Fb[i][m])
-
-        bopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam)
-        load()
-        tuned = batch.autotune(bopt)
-        log(f"batch setup+autotune {time.time() - t0:.1f}s: {tuned}")
-        ms_eager = batch.als(bopt, iters=a.batch_iters, graph=False)
-        load()
-        batch.als(bopt, iters=a.batch_iters, graph=True)  # capture + first replay
-        reps = []
-        for _ in range(3):
-            load()
-            reps.append(batch.als(bopt, iters=a.batch_iters, graph=True))
-        ms_graph = float(np.median(reps))
-        rtr, rte = batch.rmse()
-        bn = batch.train_nnz
-        out_algs["batch_als"] = {
-            "problems": len(probs), "classes": batch.classes(), "train_nnz": bn, "iters": a.batch_iters,
-            "graph_ms": ms_graph, "direct_ms": ms_eager, "nnz_per_s": bn * a.batch_iters / (ms_graph * 1e-3),
-            "s_per_iter": ms_graph * 1e-3 / a.batch_iters,
-            "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
-                      for (r, _), t in zip(batch.classes(), tuned)],
-            "rmse_te_median": float(np.median(rte)), "rmse_tr_median": float(np.median(rtr))}
-        log(f"batch ALS: {a.batch_iters} iters of {len(probs)} problems in {ms_graph:.1f} ms (graph), "
-            f"{ms_eager:.1f} ms (direct); {bn * a.batch_iters / (ms_graph * 1e-3) / 1e9:.3f} Gnnz/s; "
-            f"median rmse_te {np.median(rte):.4f}")
-        # the same run in mixed precision: ranks 32 and 64 on the BF16-operand
-        # row path (tuned for it), ranks 8 and 16 on the FP32 kernels; the FP32
-        # run above (factors within 1e-5 of the FP64 update) is the reference
-        mopt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=lam, precision=ct.PREC_MIXED)
-        load()
-        mtuned = batch.autotune(mopt)
-        load()
-        batch.als(mopt, iters=a.batch_iters, graph=True)
-        mreps = []
-        for _ in range(3):
-            load()
-            mreps.append(batch.als(mopt, iters=a.batch_iters, graph=True))
-        ms_mixed = float(np.median(mreps))
-        _, mrte = batch.rmse()
-        gap = np.abs(mrte - rte)
-        out_algs["batch_als"]["mixed"] = {
-            "graph_ms": ms_mixed, "nnz_per_s": bn * a.batch_iters / (ms_mixed * 1e-3),
-            "speedup_vs_fp32": ms_graph / ms_mixed,
-            "tuned": [{"rank": r, "tile_len": t["tile_len"], "block": t["block"]}
-                      for (r, _), t in zip(batch.classes(), mtuned)],
-            "rmse_te_gap_vs_fp32_max": float(gap.max()), "rmse_te_gap_vs_fp32_median": float(np.median(gap))}
-        log(f"batch ALS mixed (ranks 32/64 BF16-operand): {ms_mixed:.1f} ms (graph), "
-            f"{ms_graph / ms_mixed:.2f}x the FP32 run; test RMSE gap to the FP32 run median {np.median(gap):.1e}, "
-            f"max {gap.max():.1e}; tuned {[(t['tile_len'], t['block']) for t in mtuned]}")
-        del batch, probs, Fb
-
-    # ----------------------------------------- adjoint (4-way, autograd)
-    if "adjoint" in algs:
-        out_algs["adjoint"] = adjoint_leg(ct, ctx, a, log)
-
-    # ------------------------------------------ mixed-precision ALS, R = 128
-    if "mixed" in algs:
-        out_algs["als_mixed"] = mixed_leg(ct, ctx, a, log)
-
-    copy_gbs = ctx.copy_bandwidth(1 << 30)
-
-    # --------------------------------------------------------- CPU baseline
-    cpu = None
+    # --------------------------------------------------- CPU baselines
+    cpu, cpu_all = None, {}
     if not a.no_cpu:
         import oracle as O
-        n = min(a.cpu_sample or train.nnz, train.nnz)
+        nth = os.cpu_count() or 1
         ind, vals = train.arrays()
-        sub = O.Coo([x[:n] for x in ind], vals[:n], dims)
-        rng = np.random.default_rng(1)
-        F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
-        order = np.arange(n, dtype=np.uint32)  # the training stream is in mode-0 order
-        lr = 1e-3
+        ot = O.Coo(ind, vals, dims)
+        del ind, vals
+        order = np.arange(train.nnz, dtype=np.uint32)  # the training stream is in mode-0 order
+        F = [f.copy() for f in F0]
         t0 = time.perf_counter()
-        for _ in range(a.cpu_epochs):
-            O.sgd_epoch(sub, F, lr, lam, 1.0, order=order)
-            lr *= 0.95
+        O.sgd_sync_epoch(ot, F, 1e-3, lam, c.tile_len, c.sgd_substep, c.sgd_hot_rows, order=order, nthreads=nth)
         dt = time.perf_counter() - t0
-        cpu = {"value": n * a.cpu_epochs / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
-               "sample": f"{a.cpu_epochs} sequential FP32 SGD epochs (the oracle's FP32 twin, one core) over "
-                         f"{'all' if n == train.nnz else 'the first'} {n} training nonzeros in mode-0 order, "
-                         f"R={R}; {dt:.1f} s"}
-        log(f"CPU baseline {n * a.cpu_epochs / dt / 1e6:.2f} Mnnz/s over {a.cpu_epochs} x {n} nnz ({dt:.1f}s)")
+        cpu = {"value": train.nnz / dt, "unit": "nnz/s", "cores": nth, "kind": "port",
+               "sample": f"one epoch of the same bulk-synchronous SGD schedule in FP32 (the oracle's FP32 twin) over "
+                         f"all {train.nnz} training nonzeros, R={R}, {nth} threads; {dt:.1f} s"}
+        cpu_all["sgd_schedule_all_cores"] = cpu
+        F = [f.copy() for f in F0]
+        t0 = time.perf_counter()
+        O.sgd_epoch(ot, F, 1e-3, lam, 1.0, order=order)
+        dt1 = time.perf_counter() - t0
+        cpu_all["sgd_sequential_1_core"] = {"value": train.nnz / dt1, "unit": "nnz/s", "cores": 1, "kind": "port",
+                                            "sample": f"one sequential FP32 SGD epoch over all {train.nnz} nonzeros; "
+                                                      f"{dt1:.1f} s"}
+        log(f"CPU SGD: schedule {train.nnz / dt / 1e6:.1f} Mnnz/s on {nth} threads, sequential "
+            f"{train.nnz / dt1 / 1e6:.1f} Mnnz/s on one core")
+        # CCD++ on the host: the first 10M (all cores) / 2M (one core) nonzeros of the stream, one epoch
+        for cores, n in ((nth, 10_000_000), (1, 2_000_000)):
+            if elapsed() > a.budget:
+                break
+            n = min(n, train.nnz)
+            sub = O.Coo([x[:n] for x in ot.ind], ot.vals[:n], dims)
+            Fc = [f.copy() for f in F0]
+            res = O.residual(sub, Fc)
+            orders = [O.mode_order(sub, md) for md in range(3)]
+            t0 = time.perf_counter()
+            O.ccd_epoch(sub, Fc, lam, res, orders=orders, nthreads=cores)
+            dtc = time.perf_counter() - t0
+            cpu_all[f"ccd_{cores}_cores"] = {"value": n / dtc, "unit": "nnz/s", "cores": cores, "kind": "port",
+                                             "s_per_epoch": dtc,
+                                             "sample": f"one FP32 CCD++ epoch (R={R}, all {R} columns, 3 modes) over "
+                                                       f"the first {n} nonzeros of the training stream"}
+            log(f"CPU CCD++ {cores} cores: {n / dtc / 1e6:.2f} Mnnz/s on {n} nonzeros")
+        del ot
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sgd_traffic.json")
-    if os.path.exists(tpath):
+    def run_leg(name, fn):
+        if elapsed() > a.budget:
+            out_algs[name] = {"skipped": f"time budget: {elapsed():.0f}s elapsed before the leg started"}
+            log(f"{name}: skipped (budget)")
+            return
         try:
-            tj = json.load(open(tpath))
-            if tj.get("nnz") == train.nnz and tj.get("rank") == R:
-                traffic = tj.get("hbm_bytes_per_launch")
-        except Exception:  # noqa: BLE001
-            traffic = None
+            t0 = time.time()
+            out_algs[name] = fn()
+            if isinstance(out_algs[name], dict):
+                out_algs[name]["leg_wall_s"] = time.time() - t0
+        except Exception as e:  # noqa: BLE001 -- record the failure, keep the other legs
+            out_algs[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: failed\n{traceback.format_exc()}")
+        gc.collect()
+
+    if "mttkrp" in algs:
+        run_leg("mttkrp", lambda: mttkrp_leg(ct, ctx, model, train.nnz, R))
+    if "ccd" in algs:
+        run_leg("ccd", lambda: ccd_leg(ct, ctx, model, train.nnz, R, lam, F0, a.ccd_epochs, a.tile_len))
+    if "als" in algs:
+        run_leg("als", lambda: als_cfg2_leg(ct, ctx, model, train.nnz, R, lam, F0, a.als_epochs, a.tile_len))
+    copy_gbs = ctx.copy_bandwidth(1 << 30)
+    del model, train, test
+    gc.collect()
+    if "share" in algs:
+        shares = {}
+        for nz in [int(x) for x in a.shares.split(",") if x]:
+            run_leg(f"share_{nz}", lambda nz=nz: share_leg(ct, ctx, a, nz, R, lam))
+            shares[str(nz)] = out_algs.pop(f"share_{nz}")
+        out_algs["sgd_one_gpu_share"] = shares
+    if "cfg1" in algs:
+        run_leg("als_1M_cfg1", lambda: als_cfg1_leg(ct, ctx, a))
+    if "batch" in algs:
+        run_leg("batch_als", lambda: batch_leg(ct, ctx, a, lam))
+    if "adjoint" in algs:
+        run_leg("adjoint_sgd_4way", lambda: adjoint_leg(ct, ctx, a))
+    if "mixed" in algs:
+        run_leg("als_mixed_R128", lambda: mixed_leg(ct, ctx, a))
 
     line = {
         "metric": "nnz/s and s/epoch (ALS,CCD++,SGD) at 100M nnz R=32; test-RMSE gap vs FP64",
@@ -527,18 +686,33 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "parity_ok": parity_ok,
         "config": {"workload": "sgd_epoch_100M_nnz_R32", "dims": list(dims), "nnz": a.nnz,
-                   "train_nnz": train.nnz, "rank": R, "lambda": lam, "lr": 1e-3, "lr_decay": 0.95,
-                   "zipf": [1.0, 1.0, 0.0], "values": "int 1..5", "test_fraction": 0.01},
+                   "train_nnz": train_nnz, "rank": R, "lambda": lam,
+                   "lr": 1e-3, "lr_decay": 0.95, "zipf": [1.0, 1.0, 0.0], "values": "int 1..5",
+                   "test_fraction": 0.01, "sgd_schedule": {"span": c.tile_len, "sub_step": c.sgd_substep,
+                                                           "hot_rows": c.sgd_hot_rows}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sgd", "kernel_ms": kern_ms,
-                     "alg_bytes_per_launch": alg_bytes_per_nnz * train.nnz},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sgd_sync", "kernel_ms": walk_ms, "alg_bytes_per_launch": alg_per_launch},
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_all,
         "copy_gbs_measured": copy_gbs,
+        "wall_s": elapsed(),
         "algorithms": out_algs,
     }
+    tpath = os.path.join(ROOT, "profiles", "sgd_sync_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            tj = json.load(open(tpath))
+            if tj.get("nnz") == a.nnz and tj.get("rank") == R:
+                line["roofline"]["traffic"] = tj.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            pass
     print(json.dumps(line), flush=True)
+    if not parity_ok and ref is not None:
+        log(f"SGD test-RMSE trajectory is {gap:.2e} from the FP64 oracle of the same schedule (bar 1e-3)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -40,7 +40,7 @@ class Coo:
         self._p = ptr  # POINTER(_lib.Coo)
 
     def __del__(self):
-        if getattr(self, "_p", None) and _lib.alive():
+        if getattr(self, "_p", None) and _lib is not None and _lib.alive():
             lib().ctc_coo_free(self._p)
             self._p = None
 
@@ -159,7 +159,7 @@ class Ciss:
         self._v = v
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib.alive():
+        if getattr(self, "_h", None) and _lib is not None and _lib.alive():
             lib().ctc_ciss_free(self._h)
             self._h = None
 
@@ -278,7 +278,7 @@ class Context:
         check(lib().ctc_context_create(device, C.byref(self._h)))
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib.alive():
+        if getattr(self, "_h", None) and _lib is not None and _lib.alive():
             lib().ctc_context_destroy(self._h)
             self._h = None
 
@@ -336,7 +336,7 @@ class Model:
                                      test.c if test else None, self.rank, C.byref(self._h)))
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib.alive():
+        if getattr(self, "_h", None) and _lib is not None and _lib.alive():
             lib().ctc_model_destroy(self._h)
             self._h = None
 
@@ -458,7 +458,7 @@ class Batch:
             self._h = None
 
     def __del__(self):
-        if _lib.alive():
+        if _lib is not None and _lib.alive():
             self._release()
 
     def set_factor(self, p, mode, F):
