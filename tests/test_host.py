@@ -319,9 +319,10 @@ def test_options_defaults_follow_reference():
 
 
 def test_work_unit_defaults():
-    """Measured defaults: SGD work units of 2048 nonzeros, CCD++ 1024; ALS 0, so
-    that each ALS kernel family applies its own measured length; SGD hot rows
-    above 65536 nonzeros."""
+    """Measured defaults: SGD walkers of 2048 nonzeros in 32-nonzero sub-steps of
+    the bulk-synchronous schedule, CCD++ work units of 1024; ALS 0, so that each
+    ALS kernel family applies its own measured length; SGD hot rows above 65536
+    nonzeros."""
     import ctypes as C
     from cutc_amd import _lib
     for alg, tl in ((ct.ALS, 0), (ct.SGD, 2048), (ct.CCD, 1024)):
@@ -329,7 +330,7 @@ def test_work_unit_defaults():
         assert _lib.lib().ctc_options_default(alg, C.byref(o)) == 0
         assert o.tile_len == tl, (alg, o.tile_len)
         if alg == ct.SGD:
-            assert o.sgd_hot_rows == 65536
+            assert o.sgd_hot_rows == 65536 and o.sgd_substep == 32 and o.sgd_update == ct.SGD_SYNC
 
 
 def test_context_without_gpu_reports_error():
