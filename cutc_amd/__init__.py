@@ -414,6 +414,18 @@ class Model:
                              C.c_void_p(stream) if stream else None))
         return loss.value
 
+    def grad_device_noalloc(self, factor_ptrs, grad_ptrs, lam, loss_ptr=None, reg_per_occurrence=False,
+                            stream=None):
+        """ctc_grad_noalloc: the same gradient on device buffers without any
+        device allocation or synchronisation; the loss goes to the device
+        double at loss_ptr (None: not written).  Enqueued on `stream`."""
+        n = self.nmodes
+        fp = (C.c_void_p * n)(*factor_ptrs)
+        gp = (C.c_void_p * n)(*grad_ptrs)
+        check(lib().ctc_grad_noalloc(self._h, fp, gp, lam, 1 if reg_per_occurrence else 0,
+                                     C.c_void_p(loss_ptr) if loss_ptr else None,
+                                     C.c_void_p(stream) if stream else None))
+
     def autotune(self, opt: Options):
         o = opt.to_c()
         r = _lib.TuneResult()

@@ -195,6 +195,7 @@ SIGNATURES = {
     "ctc_evaluate": (C.c_int, [vp, C.c_double, P(C.c_double), P(C.c_double), P(C.c_double),
                                P(C.c_double), P(C.c_double)]),
     "ctc_grad": (C.c_int, [vp, P(vp), P(vp), C.c_double, C.c_int, P(C.c_double), vp]),
+    "ctc_grad_noalloc": (C.c_int, [vp, P(vp), P(vp), C.c_double, C.c_int, vp, vp]),
     "ctc_tc_main_ciss": (C.c_int, [P(Coo), P(Coo), P(Coo), P(Coo), P(Coo), C.c_int, P(Options)]),
     "ctc_tc_als": (C.c_int, [P(Coo), P(Coo), P(Coo), P(Coo), P(Coo), P(P(Matrix)), P(P(Matrix)),
                              C.c_int, C.c_double, P(C.c_double), P(C.c_double), P(C.c_uint64),

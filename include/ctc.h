@@ -342,6 +342,19 @@ ctc_status ctc_evaluate(ctc_model *m, double lambda, double *loss, double *frob,
  * kernel (SG/sgd.cu:92-99) steps along, summed over nonzeros. */
 ctc_status ctc_grad(ctc_model *m, const float *const *factors, float *const *grads,
                     double lambda, int reg_per_occurrence, double *loss, void *stream);
+/* Allocation-free ctc_grad.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving or training loops).  Same gradients as ctc_grad; the loss is
+ * written to the device double loss_device (may be NULL).  It calls no
+ * hipMalloc/hipFree, issues no memset and does not synchronise; its partial
+ * sums live in a fixed device array inside the library (1 MiB per GPU, held
+ * until the process ends), so calls that overlap on one GPU race.  Every
+ * mode's CISS must already be built (ctc_model_prepare with ALS or CCD++
+ * options, or one ctc_grad call), else CTC_ERR_INVALID_ARG; a tensor whose
+ * partial sums would not fit the array (more than 16384 blocks per walk even
+ * with the longest walker spans) is rejected with CTC_ERR_INVALID_ARG. */
+ctc_status ctc_grad_noalloc(ctc_model *m, const float *const *factors, float *const *grads,
+                            double lambda, int reg_per_occurrence, double *loss_device, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Reference-shaped entry points (drop-in for callers of the reference's C   */
