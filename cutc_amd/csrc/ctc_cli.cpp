@@ -15,6 +15,7 @@
 //   --autotune (replaces the GA of software/autotuning)      --json (summary line)
 //   --dump-factors PREFIX (writes PREFIX<mode>.txt)
 //   --precision fp32|mixed (BF16-operand ALS)  --refine K   --sgd-update P  --sgd-hot-rows N
+//   --sgd-substep N (nonzeros per walker per sub-step of the default SGD schedule)
 //
 //   ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]  (data preparation)
 #include <cstdio>
@@ -32,7 +33,8 @@ static void usage() {
           "           [--max-epochs E] [--tol T] [--max-bad-epochs B] [--tile-len N] [--block B]\n"
           "           [--seed S] [--mode-order fixed|rotate] [--lr-schedule bold|exp|const]\n"
           "           [--lr-decay G] [--inner I] [--precision fp32|mixed] [--refine K]\n"
-          "           [--sgd-update auto|atomic|racy|next-racy|hot-atomic] [--sgd-hot-rows N]\n"
+          "           [--sgd-update sync|auto|atomic|racy|next-racy|hot-atomic] [--sgd-hot-rows N]\n"
+          "           [--sgd-substep N]\n"
           "           [--autotune] [--json] [--dump-factors PREFIX]\n");
 }
 
@@ -142,11 +144,20 @@ int main(int argc, char **argv) {
     else if (k == "inner") o.inner_iters = (uint32_t)atoi(v);
     else if (k == "precision") o.precision = strcmp(v, "mixed") == 0 ? CTC_PREC_MIXED : CTC_PREC_FP32;
     else if (k == "refine") o.refine = atoi(v);
-    else if (k == "sgd-update")
-      o.sgd_update = strcmp(v, "atomic") == 0 ? CTC_SGD_ATOMIC : strcmp(v, "racy") == 0 ? CTC_SGD_RACY
-                     : strcmp(v, "next-racy") == 0                                       ? CTC_SGD_NEXT_RACY
-                     : strcmp(v, "hot-atomic") == 0                                      ? CTC_SGD_HOT_ATOMIC
-                                                                                         : CTC_SGD_AUTO;
+    else if (k == "sgd-update") {
+      static const struct { const char *name; int32_t v; } pol[] = {
+          {"sync", CTC_SGD_SYNC},   {"auto", CTC_SGD_AUTO},           {"atomic", CTC_SGD_ATOMIC},
+          {"racy", CTC_SGD_RACY},   {"next-racy", CTC_SGD_NEXT_RACY}, {"hot-atomic", CTC_SGD_HOT_ATOMIC}};
+      int found = -1;
+      for (const auto &q : pol)
+        if (!strcmp(v, q.name)) found = q.v;
+      if (found < 0) {
+        fprintf(stderr, "ctc: unknown --sgd-update %s\n", v);
+        return 1;
+      }
+      o.sgd_update = found;
+    } else if (k == "sgd-substep")
+      o.sgd_substep = (uint32_t)atoi(v);
     else if (k == "sgd-hot-rows") o.sgd_hot_rows = (uint32_t)strtoul(v, nullptr, 0);
     else if (k == "dump-factors") dump = v;
     else {
