@@ -430,7 +430,8 @@ class Model:
         o = opt.to_c()
         r = _lib.TuneResult()
         check(lib().ctc_autotune(self._h, C.byref(o), C.byref(r)))
-        return {"tile_len": r.tile_len, "block": r.block, "ms": r.ms, "trials": r.trials}
+        return {"tile_len": r.tile_len, "block": r.block, "ms": r.ms, "trials": r.trials,
+                "sgd_substep": r.sgd_substep}
 
 
 # ---------------------------------------------------------------------------

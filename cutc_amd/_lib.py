@@ -146,6 +146,7 @@ class TuneResult(C.Structure):
         ("block", C.c_uint32),
         ("ms", C.c_double),
         ("trials", C.c_uint32),
+        ("sgd_substep", C.c_uint32),
     ]
 
 

@@ -1221,59 +1221,83 @@ extern "C" ctc_status ctc_autotune(ctc_model *m, const ctc_options *o, ctc_tune_
   if (!m || !o || !out) return fail(CTC_ERR_INVALID_ARG, "ctc_autotune");
   CTC_HIP(hipSetDevice(m->ctx->device));
   const uint32_t N = m->nmodes;
-  // save factors: tuning sweeps modify them
-  std::vector<std::vector<float>> saved(N);
+  hipStream_t st = m->ctx->stream;
+  // save the factors and the best-model copy: tuning sweeps modify the
+  // factors, and the restore must leave the best copy as it was
+  std::vector<std::vector<float>> saved(N), saved_best(N);
   for (uint32_t md = 0; md < N; ++md) {
     saved[md].resize(m->dims[md] * m->rank);
+    saved_best[md].resize(m->dims[md] * m->rank);
     CTC_TRY(ctc_model_get_factor(m, md, 0, saved[md].data()));
+    CTC_TRY(ctc_model_get_factor(m, md, 1, saved_best[md].data()));
   }
-  const uint32_t tiles[] = {64, 128, 256, 512, 1024, 2048, 4096};
-  const unsigned blocks[] = {64, 128, 256};
-  hipStream_t st = m->ctx->stream;
+  // search space: (tile_len, block), the GA's two knobs (GA/fitness.c:67-79);
+  // the bulk-synchronous SGD schedule also tunes its sub-step, and its walker
+  // span and sub-step define the schedule, so the tuned shape is what a later
+  // solve with these options computes
+  const bool sync = o->algorithm == CTC_SGD && o->sgd_update == CTC_SGD_SYNC;
+  std::vector<uint32_t> tiles = {64, 128, 256, 512, 1024, 2048, 4096};
+  std::vector<uint32_t> subs = {o->sgd_substep};
+  std::vector<unsigned> blocks = {64, 128, 256};
+  if (sync) {
+    tiles = {1024, 2048, 4096, 8192};
+    subs = {16, 32, 64};
+    blocks = {64, 256};
+  }
   hipEvent_t e0, e1;
   CTC_HIP(hipEventCreate(&e0));
   CTC_HIP(hipEventCreate(&e1));
   double best = 1e300;
-  ctc_tune_result r{o->tile_len, o->block, 0.0, 0};
+  ctc_tune_result r{o->tile_len, o->block, 0.0, 0, o->sgd_substep};
   ctc_status s = CTC_OK;
   for (uint32_t tl : tiles) {
-    for (unsigned b : blocks) {
-      ctc_options t = *o;
-      t.tile_len = tl;
-      t.block = b;
-      double tot = 0.0;
-      const int reps = 3;
-      for (int rep = 0; rep <= reps && s == CTC_OK; ++rep) {
-        (void)hipEventRecord(e0, st);
-        if (o->algorithm == CTC_SGD)
-          s = sgd_epoch(m, &t, 0.0);  // lr 0: same memory traffic, factors unchanged
-        else if (o->algorithm == CTC_ALS)
-          for (uint32_t md = 0; md < N && s == CTC_OK; ++md) s = als_mode(m, md, &t);
-        else
-          s = ccd_epoch(m, &t);
-        (void)hipEventRecord(e1, st);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (rep > 0) tot += ms;
+    for (uint32_t sb : subs) {
+      if (sync && sb > tl) continue;
+      for (unsigned b : blocks) {
+        ctc_options t = *o;
+        t.tile_len = tl;
+        t.block = b;
+        t.sgd_substep = sb;
+        double tot = 0.0;
+        const int reps = 3;
+        for (int rep = 0; rep <= reps && s == CTC_OK; ++rep) {  // rep 0: warm-up (builds tables)
+          (void)hipEventRecord(e0, st);
+          if (o->algorithm == CTC_SGD)
+            s = sgd_epoch(m, &t, 0.0);  // lr 0: same memory traffic, factors unchanged
+          else if (o->algorithm == CTC_ALS)
+            for (uint32_t md = 0; md < N && s == CTC_OK; ++md) s = als_mode(m, md, &t);
+          else
+            s = ccd_epoch(m, &t);
+          (void)hipEventRecord(e1, st);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) tot += ms;
+        }
+        if (s != CTC_OK) break;
+        r.trials++;
+        const double avg = tot / reps;
+        if (avg < best) {
+          best = avg;
+          r.tile_len = tl;
+          r.block = b;
+          r.sgd_substep = sb;
+        }
       }
       if (s != CTC_OK) break;
-      r.trials++;
-      const double avg = tot / reps;
-      if (avg < best) {
-        best = avg;
-        r.tile_len = tl;
-        r.block = b;
-      }
     }
     if (s != CTC_OK) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (uint32_t md = 0; md < N; ++md) {
-    ctc_status s2 = ctc_model_set_factor(m, md, saved[md].data());
-    if (s == CTC_OK) s = s2;
+    const size_t bytes = saved[md].size() * sizeof(float);
+    hipError_t e = hipMemcpyAsync(m->F[md], saved[md].data(), bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->best[md], saved_best[md].data(), bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess && s == CTC_OK) s = fail(CTC_ERR_HIP, "autotune restore: %s", hipGetErrorString(e));
   }
+  m->resid_valid = false;
   // the trial sweeps' status flags do not describe the restored model
   (void)hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), st);
   (void)hipStreamSynchronize(st);

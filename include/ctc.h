@@ -402,7 +402,16 @@ typedef struct ctc_tune_result {
   uint32_t block;
   double ms;       /* best measured sweep time */
   uint32_t trials; /* configurations measured */
+  uint32_t sgd_substep; /* bulk-synchronous SGD: the tuned sub-step (else opt's) */
 } ctc_tune_result;
+
+/* Measured search over the work-unit shape of opt->algorithm's sweep, the
+ * GA's fitness (mean epoch time over (blocksize, tile length),
+ * GA/fitness.c:31-100): ALS / CCD++ / lock-free SGD over tile_len 64..4096 and
+ * block 64..256; the default SGD schedule over walker spans 1024..8192,
+ * sub-steps 16..64 and blocks 64 / 256 (span and sub-step define that
+ * schedule, so the tuned shape is what later solves with it compute).  The
+ * factors and the best-model copy are restored afterwards. */
 
 ctc_status ctc_autotune(ctc_model *m, const ctc_options *opt, ctc_tune_result *out);
 
