@@ -161,22 +161,59 @@ def als_cfg1_leg(ct, ctx, a):
 
 
 # ------------------------------------------------------ config 2 (others)
-def mttkrp_leg(ct, ctx, model, nnz, R):
+def mttkrp_leg(ct, ctx, model, train, R, no_cpu):
+    """GPU MTTKRP per mode (k_mttkrp); beside it the reference's CSF layout and
+    serial MTTKRP (SG/csf.c, SG/mttkrp.c) as restated by the oracle, run with
+    OpenMP over the 64 groups of level-0 nodes (FP32, all cores and one core),
+    and the GPU result against the FP64 CSF MTTKRP of the same factors."""
+    import numpy as np
+    nnz = train.nnz
     ctx.reset_stats()
     for md in range(3):
         model.mttkrp(md)  # warm-up (builds the CISS of each mode)
     ctx.reset_stats()
-    per_mode = []
+    per_mode, outs = [], []
     for md in range(3):
         for _ in range(3):
-            model.mttkrp(md)
+            out = model.mttkrp(md)
+        outs.append(out)
         n_m, ms_m = ctx.kernel_stats("mttkrp")
         per_mode.append(ms_m / max(n_m, 1))
         ctx.reset_stats()
     mb = 12 + 8 * R  # stream + two gathered rows per nonzero
+    res = {"kernel_ms_per_mode": per_mode, "alg_bytes_per_nnz": mb,
+           "alg_gbs_per_mode": [mb * nnz / (t * 1e-3) / 1e9 for t in per_mode]}
     log(f"MTTKRP kernel ms per mode {[round(t, 3) for t in per_mode]}")
-    return {"kernel_ms_per_mode": per_mode, "alg_bytes_per_nnz": mb,
-            "alg_gbs_per_mode": [mb * nnz / (t * 1e-3) / 1e9 for t in per_mode]}
+    if no_cpu:
+        return res
+    import oracle as O
+    nth = os.cpu_count() or 1
+    F = model.factors()
+    ind, vals = train.arrays()
+    ot = O.Coo(ind, vals, train.dims)
+    del ind, vals
+    F64 = [f.astype(np.float64) for f in F]
+    cpu = []
+    for md in range(3):
+        t0 = time.perf_counter()
+        c = O.Csf(ot, md)
+        build = time.perf_counter() - t0
+        ref = O.mttkrp_csf(c, F64, nthreads=nth)
+        err = float(np.linalg.norm(outs[md].astype(np.float64) - ref) / np.linalg.norm(ref))
+        row = {"mode": md, "csf_build_s": build, "gpu_rel_err_vs_fp64": err}
+        for cores in (nth, 1):
+            t0 = time.perf_counter()
+            O.mttkrp_csf(c, F, nthreads=cores)
+            dt = time.perf_counter() - t0
+            row[f"cpu_{cores}_cores_s"] = dt
+            row[f"cpu_{cores}_cores_nnz_per_s"] = nnz / dt
+        cpu.append(row)
+        del c
+        log(f"MTTKRP mode {md}: CSF CPU {row[f'cpu_{nth}_cores_s'] * 1e3:.0f} ms ({nth} cores), "
+            f"{row['cpu_1_cores_s'] * 1e3:.0f} ms (1 core); GPU {per_mode[md]:.2f} ms; GPU vs FP64 rel err {err:.1e}")
+    res["cpu_csf"] = cpu
+    res["host_cores"] = nth
+    return res
 
 
 def ccd_leg(ct, ctx, model, nnz, R, lam, F0, epochs, tile_len):
@@ -650,7 +687,7 @@ def main():
         gc.collect()
 
     if "mttkrp" in algs:
-        run_leg("mttkrp", lambda: mttkrp_leg(ct, ctx, model, train.nnz, R))
+        run_leg("mttkrp", lambda: mttkrp_leg(ct, ctx, model, train, R, a.no_cpu))
     if "ccd" in algs:
         run_leg("ccd", lambda: ccd_leg(ct, ctx, model, train.nnz, R, lam, F0, a.ccd_epochs, a.tile_len))
     if "als" in algs:

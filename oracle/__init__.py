@@ -14,12 +14,16 @@ Reference files restated (SG = ``software/single GPU/sgpu``):
                              averaging)               -> sgd_sync_epoch / sgd_sync
   SG/ccd.cu:532-554, NC/ccd.cu:28-180  CCD++       -> ccd_epoch / ccd
   SG/completion.c:123-214,296-362      loss/RMSE   -> loss_sq / rmse / converge
+  SG/csf.c:11-331, SG/mttkrp.c:16-93   CSF + serial MTTKRP -> Csf / mttkrp_csf
 
 Pins (each function is checked against something it is not a retyping of):
   tns_read                 hand-written files (tests/golden/tns/*)
   ciss_reference_layout    hand-worked layouts (tests/golden/ciss_*.json) and a
                            brute-force decode of every bitmap
   mttkrp / gram_rows       dense numpy einsum over the densified tensor
+  csf_build / mttkrp_csf   the CSF expands back to the COO (round trip), its levels are
+                           strictly increasing within their parent, blockref by hand;
+                           mttkrp_csf equals mttkrp (itself pinned by einsum)
   cholesky / solve         scipy.linalg.cho_factor / cho_solve
   als_mode                 per-row ridge regression via numpy.linalg.lstsq on the
                            stacked [H; sqrt(lambda) I] system; exact rank-1 recovery
@@ -53,6 +57,20 @@ class OrcCoo(C.Structure):
         ("ind", C.POINTER(C.c_uint32) * MAX_MODES),
         ("vals", C.POINTER(C.c_double)),
         ("offset", C.c_uint64 * MAX_MODES),
+    ]
+
+
+class OrcCsf(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32),
+        ("level_mode", C.c_uint32 * 3),
+        ("nnz", C.c_uint64),
+        ("dims", C.c_uint64 * 3),
+        ("nfibs", C.c_uint64 * 3),
+        ("fptr", C.POINTER(C.c_uint64) * 2),
+        ("fids", C.POINTER(C.c_uint32) * 3),
+        ("vals", C.POINTER(C.c_double)),
+        ("blockref", C.c_uint64 * 65),
     ]
 
 
@@ -300,6 +318,58 @@ def sgd_sync_epoch(t: Coo, F, lr, lam, span, sub, hot_rows, scale=1.0, order=Non
     _fn("orc_sgd_sync_epoch", F)(t.c(), op, C.c_uint32(R), _ptrs(F, _prec(F)), C.c_double(lr),
                                  C.c_double(lam), C.c_double(scale), C.c_uint64(span), C.c_uint64(sub),
                                  C.c_uint64(hot_rows), C.c_int(nthreads))
+
+
+class Csf:
+    """CSF of one mode of a 3-way tensor (SG/csf.c:11-331), built by the
+    oracle; numpy views of its arrays."""
+
+    def __init__(self, t: Coo, mode: int):
+        self.s = OrcCsf()
+        self._t = t
+        st = lib().orc_csf_build(t.c(), C.c_uint32(mode), C.byref(self.s))
+        assert st == 0, st
+
+    def __del__(self):
+        if getattr(self, "s", None) is not None and _LIB is not None:
+            _LIB.orc_csf_free(C.byref(self.s))
+            self.s = None
+
+    def _arr(self, p, n):
+        return np.ctypeslib.as_array(p, shape=(int(n),)).copy() if n else np.zeros(0)
+
+    @property
+    def level_mode(self):
+        return tuple(int(x) for x in self.s.level_mode)
+
+    @property
+    def nfibs(self):
+        return tuple(int(x) for x in self.s.nfibs)
+
+    def fptr(self, level):
+        return self._arr(self.s.fptr[level], self.s.nfibs[level] + 1)
+
+    def fids(self, level):
+        return self._arr(self.s.fids[level], self.s.nfibs[level])
+
+    @property
+    def vals(self):
+        return self._arr(self.s.vals, self.s.nnz)
+
+    @property
+    def blockref(self):
+        return np.array(self.s.blockref[:], dtype=np.int64)
+
+
+def mttkrp_csf(csf: Csf, F, nthreads=1):
+    """MTTKRP of the CSF's mode (SG/mttkrp.c:16-93) -> (dims[mode], R) array
+    in F's precision."""
+    R = F[0].shape[1]
+    out = np.empty((int(csf.s.dims[csf.s.mode]), R), F[0].dtype)
+    p = _fp(out) if out.dtype == np.float32 else _dp(out)
+    st = _fn("orc_mttkrp_csf", F)(C.byref(csf.s), C.c_uint32(R), _ptrs(F, _prec(F)), p, C.c_int(nthreads))
+    assert st == 0, st
+    return out
 
 
 def residual(t: Coo, F):

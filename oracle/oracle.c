@@ -417,6 +417,107 @@ int orc_converge(orc_conv *c, double converge_rmse, uint64_t epoch, int *improve
 }
 
 /* ------------------------------------------------------------------------ */
+/* CSF (SG/csf.c:11-331)                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* stable counting sort of perm[0..n) by key[perm[.]] in [0, dim) */
+static int csf_sort_by(const uint32_t *key, uint64_t dim, uint64_t n, uint32_t *perm, uint32_t *tmp) {
+  uint64_t *cnt = (uint64_t *)calloc(dim + 1, sizeof(uint64_t));
+  if (!cnt) return 1;
+  for (uint64_t q = 0; q < n; ++q) cnt[key[perm[q]] + 1]++;
+  for (uint64_t d = 0; d < dim; ++d) cnt[d + 1] += cnt[d];
+  for (uint64_t q = 0; q < n; ++q) tmp[cnt[key[perm[q]]]++] = perm[q];
+  memcpy(perm, tmp, n * sizeof(uint32_t));
+  free(cnt);
+  return 0;
+}
+
+int orc_csf_build(const orc_coo *t, uint32_t mode, orc_csf *out) {
+  memset(out, 0, sizeof(*out));
+  if (t->nmodes != 3 || mode > 2) return 1;
+  /* level order: the reference's firstrank / secondrank / thirdrank
+   * (SG/csf.c:30-34, SG/mttkrp.c:27-49) */
+  static const uint32_t lv[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+  const uint32_t l0 = lv[mode][0], l1 = lv[mode][1], l2 = lv[mode][2];
+  const uint64_t n = t->nnz;
+  out->mode = mode;
+  out->level_mode[0] = l0;
+  out->level_mode[1] = l1;
+  out->level_mode[2] = l2;
+  out->nnz = n;
+  for (int m = 0; m < 3; ++m) out->dims[m] = t->dims[m];
+  uint32_t *perm = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  uint32_t *tmp = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  for (uint64_t q = 0; q < n; ++q) perm[q] = (uint32_t)q;
+  /* lexicographic (level 0, level 1), leaves in input order: LSD, least
+   * significant key first (p_rearrange, SG/csf.c:148-186) */
+  csf_sort_by(t->ind[l1], t->dims[l1], n, perm, tmp);
+  csf_sort_by(t->ind[l0], t->dims[l0], n, perm, tmp);
+  free(tmp);
+  /* count nodes and fibers (p_modecounter, SG/csf.c:11-146) */
+  uint64_t n0 = 0, n1 = 0;
+  for (uint64_t q = 0; q < n; ++q) {
+    const uint32_t p = perm[q];
+    const int new0 = q == 0 || t->ind[l0][p] != t->ind[l0][perm[q - 1]];
+    const int new1 = new0 || t->ind[l1][p] != t->ind[l1][perm[q - 1]];
+    n0 += new0;
+    n1 += new1;
+  }
+  out->nfibs[0] = n0;
+  out->nfibs[1] = n1;
+  out->nfibs[2] = n;
+  out->fids[0] = (uint32_t *)malloc((n0 ? n0 : 1) * sizeof(uint32_t));
+  out->fids[1] = (uint32_t *)malloc((n1 ? n1 : 1) * sizeof(uint32_t));
+  out->fids[2] = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  out->fptr[0] = (uint64_t *)malloc((n0 + 1) * sizeof(uint64_t));
+  out->fptr[1] = (uint64_t *)malloc((n1 + 1) * sizeof(uint64_t));
+  out->vals = (double *)malloc((n ? n : 1) * sizeof(double));
+  uint64_t a = 0, b = 0;
+  for (uint64_t q = 0; q < n; ++q) {
+    const uint32_t p = perm[q];
+    const int new0 = q == 0 || t->ind[l0][p] != t->ind[l0][perm[q - 1]];
+    const int new1 = new0 || t->ind[l1][p] != t->ind[l1][perm[q - 1]];
+    if (new0) {
+      out->fids[0][a] = t->ind[l0][p];
+      out->fptr[0][a++] = b;
+    }
+    if (new1) {
+      out->fids[1][b] = t->ind[l1][p];
+      out->fptr[1][b++] = q;
+    }
+    out->fids[2][q] = t->ind[l2][p];
+    out->vals[q] = t->vals[p];
+  }
+  out->fptr[0][n0] = n1;
+  out->fptr[1][n1] = n;
+  free(perm);
+  /* 64 groups of ceil(n0 / 64) level-0 nodes (SG/csf.c:75-92) */
+  const uint64_t per = (n0 + 63) / 64;
+  for (int g = 0; g < 65; ++g) out->blockref[g] = n0;
+  if (n0 < 64) {
+    for (uint64_t g = 0; g <= n0; ++g) out->blockref[g] = g;
+  } else {
+    out->blockref[0] = 0;
+    for (int g = 1; g < 65; ++g) {
+      if (per * (uint64_t)g < n0) {
+        out->blockref[g] = per * (uint64_t)g;
+      } else {
+        out->blockref[g] = n0;
+        break;
+      }
+    }
+  }
+  return 0;
+}
+
+void orc_csf_free(orc_csf *c) {
+  for (int l = 0; l < 3; ++l) free(c->fids[l]);
+  for (int l = 0; l < 2; ++l) free(c->fptr[l]);
+  free(c->vals);
+  memset(c, 0, sizeof(*c));
+}
+
+/* ------------------------------------------------------------------------ */
 /* precision-templated kernels                                               */
 /* ------------------------------------------------------------------------ */
 #define REAL double

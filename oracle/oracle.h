@@ -63,6 +63,34 @@ int orc_ciss_reference_layout(const orc_coo *t, uint32_t mode, uint32_t tile_len
                               uint64_t *dcounter, double *entries,
                               uint64_t *dlength, uint64_t *size);
 
+/* ---- CSF (compressed sparse fiber) layout, 3-way (SG/csf.c:11-331) --------
+ * Level order of the CSF for `mode` (the reference's p_rearrange /
+ * p_modecounter, SG/csf.c:60-195): mode 0 (i, j, k), mode 1 (j, i, k),
+ * mode 2 (k, i, j).  fids[0] are the distinct level-0 indices, fptr[0][n] the
+ * first level-1 fiber of level-0 node n; fids[1] the level-1 index of each
+ * fiber, fptr[1][f] its first nonzero; fids[2] and vals the nonzeros.  Ids are
+ * 0-based (the reference stores 1-based ids).  blockref[0..64] splits the
+ * level-0 nodes into 64 groups of ceil(nfibs[0] / 64) nodes
+ * (SG/csf.c:75-92): blockref[g] is the first node of group g, and the entries
+ * after the last group hold nfibs[0] (the reference leaves them 0; either way
+ * those groups are empty); nodes [blockref[g], blockref[g + 1]) of a group are
+ * processed by one thread.  Leaves keep the input order within a
+ * fiber (the reference's quicksort leaves it unspecified). */
+typedef struct orc_csf {
+  uint32_t mode;
+  uint32_t level_mode[3];
+  uint64_t nnz;
+  uint64_t dims[3];
+  uint64_t nfibs[3];
+  uint64_t *fptr[2];
+  uint32_t *fids[3];
+  double *vals;
+  uint64_t blockref[65];
+} orc_csf;
+/* Returns 0, or 1 if t is not 3-way. */
+int orc_csf_build(const orc_coo *t, uint32_t mode, orc_csf *out);
+void orc_csf_free(orc_csf *c);
+
 /* ---- precision-templated kernels ------------------------------------------ */
 #define ORC_DECLARE(REAL, SFX)                                                         \
   /* loss = sum (x - <a_i,b_j,c_k,...>)^2 over t (SG/completion.c:166-189) */          \
@@ -97,6 +125,13 @@ int orc_ciss_reference_layout(const orc_coo *t, uint32_t mode, uint32_t tile_len
                                REAL **F, double lr, double lambda, double scale,       \
                                uint64_t span, uint64_t sub, uint64_t hot_rows,         \
                                int nthreads);                                          \
+  /* MTTKRP of the CSF's mode into out (dims[mode] x R, overwritten), the         \
+   * reference's serial mttkrp_csf (SG/mttkrp.c:16-93, with the free of the       \
+   * undeclared `sumbuffer` at :91 read as the free of `sum`): per level-1 fiber   \
+   * s = sum of x * C(k,:) over its nonzeros, then out(i,:) += s * B(j,:); the     \
+   * 64 groups of level-0 nodes run on nthreads threads */                          \
+  int orc_mttkrp_csf##SFX(const orc_csf *c, uint32_t R, REAL *const *F, REAL *out,    \
+                          int nthreads);                                               \
   /* residual r = x - prediction (CCD++ set-up) */                                     \
   void orc_residual##SFX(const orc_coo *t, uint32_t R, REAL *const *F, REAL *res);     \
   /* one CCD++ epoch (canonical, SG/ccd.cu:532-554, NC/ccd.cu:28-180).               \

@@ -336,6 +336,33 @@ void SFX(orc_sgd_sync_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R
   }
 }
 
+int SFX(orc_mttkrp_csf)(const orc_csf *c, uint32_t R, REAL *const *F, REAL *out, int nthreads) {
+  if (R == 0 || R > 512) return 1;
+  const REAL *B = F[c->level_mode[1]], *Cm = F[c->level_mode[2]];
+  memset(out, 0, (size_t)c->dims[c->mode] * R * sizeof(REAL));
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int g = 0; g < 64; ++g) {
+    REAL sum[512];
+    for (uint64_t a = c->blockref[g]; a < c->blockref[g + 1]; ++a) {
+      REAL *o = out + (size_t)c->fids[0][a] * R;
+      for (uint64_t f = c->fptr[0][a]; f < c->fptr[0][a + 1]; ++f) {
+        for (uint32_t r = 0; r < R; ++r) sum[r] = 0; /* SG/mttkrp.c:67-78 */
+        for (uint64_t q = c->fptr[1][f]; q < c->fptr[1][f + 1]; ++q) {
+          const REAL x = (REAL)c->vals[q];
+          const REAL *cr = Cm + (size_t)c->fids[2][q] * R;
+          for (uint32_t r = 0; r < R; ++r) sum[r] += x * cr[r];
+        }
+        const REAL *br = B + (size_t)c->fids[1][f] * R; /* :79-82 */
+        for (uint32_t r = 0; r < R; ++r) o[r] += sum[r] * br[r];
+      }
+    }
+  }
+  return 0;
+}
+
 void SFX(orc_residual)(const orc_coo *t, uint32_t R, REAL *const *F, REAL *res) {
   for (uint64_t p = 0; p < t->nnz; ++p)
     res[p] = (REAL)t->vals[p] - SFX(orc_predict)(t, R, F, p);

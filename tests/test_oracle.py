@@ -675,3 +675,56 @@ def test_mt19937_64_standard_check_value():
     for _ in range(9999):
         g()
     assert g() == 9981545732273789042
+
+
+# --------------------------------------------------------------------------
+# CSF layout and its MTTKRP (SG/csf.c, SG/mttkrp.c)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_csf_expands_back_to_the_tensor(mode):
+    """Level order (i,j,k) / (j,i,k) / (k,i,j) (SG/csf.c:30-34); walking the
+    tree gives back every (index, value) of the COO exactly once, level-0 ids
+    strictly increase, and fiber ids strictly increase within their node."""
+    rng = np.random.default_rng(50 + mode)
+    dims = (17, 13, 11)
+    t = rand_coo(rng, dims, 600)
+    c = O.Csf(t, mode)
+    lv = c.level_mode
+    assert lv == ((0, 1, 2), (1, 0, 2), (2, 0, 1))[mode]
+    f0, p0, f1, p1, f2, v = c.fids(0), c.fptr(0), c.fids(1), c.fptr(1), c.fids(2), c.vals
+    assert np.all(np.diff(f0.astype(np.int64)) > 0)
+    got = set()
+    for a in range(c.nfibs[0]):
+        fib = f1[p0[a]:p0[a + 1]]
+        assert len(fib) > 0 and np.all(np.diff(fib.astype(np.int64)) > 0)
+        for f in range(p0[a], p0[a + 1]):
+            assert p1[f + 1] > p1[f]
+            for q in range(p1[f], p1[f + 1]):
+                idx = [0, 0, 0]
+                idx[lv[0]], idx[lv[1]], idx[lv[2]] = int(f0[a]), int(f1[f]), int(f2[q])
+                got.add((*idx, float(v[q])))
+    want = {(int(t.ind[0][p]), int(t.ind[1][p]), int(t.ind[2][p]), float(t.vals[p])) for p in range(t.nnz)}
+    assert got == want and c.nfibs[2] == t.nnz
+
+
+@pytest.mark.parametrize("n0,expect", [(5, [0, 1, 2, 3, 4, 5] + [5] * 59),
+                                       (130, [3 * g for g in range(44)] + [130] * 21)])
+def test_csf_blockref_groups(n0, expect):
+    """64 groups of ceil(n0 / 64) level-0 nodes (SG/csf.c:75-92): n0 = 5 gives
+    one node per group; n0 = 130 groups of 3 (44 groups, the last partial)."""
+    t = O.Coo([np.arange(n0), np.zeros(n0, int), np.zeros(n0, int)], np.ones(n0), (n0, 1, 1))
+    assert list(O.Csf(t, 0).blockref) == expect
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mttkrp_csf_equals_coo_mttkrp(mode):
+    rng = np.random.default_rng(60 + mode)
+    dims = (40, 30, 20)
+    t = rand_coo(rng, dims, 3000)
+    F = rand_factors(rng, dims, 8)
+    c = O.Csf(t, mode)
+    perm, rowptr = O.mode_order(t, mode)
+    ref = O.mttkrp(t, mode, F, order=(perm, rowptr))
+    np.testing.assert_allclose(O.mttkrp_csf(c, F, nthreads=4), ref, rtol=1e-12, atol=1e-12)
+    F32 = [f.astype(np.float32) for f in F]
+    np.testing.assert_allclose(O.mttkrp_csf(c, F32, nthreads=1), ref, rtol=2e-5, atol=2e-5)
