@@ -170,13 +170,21 @@ struct ctc_model {
   long long *sgd_Dh[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_ch[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t sgd_nhot[CTC_MAX_MODES] = {0, 0, 0, 0};
-  // staged modes (2..N-1): staging slots (walkers x sub x R) and, per
-  // sub-step, the rows that took deltas with their slots in walk order
+  // modes 1..N-1: staging slots (runs of the largest sub-step x R), each
+  // chunk's first slot (sub-steps x walkers) and, per sub-step, the rows that
+  // took runs with their slots in walk order
   float *sgd_stage[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_sbase[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_seg_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_seg_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_slots[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<uint64_t> sgd_seg_off[CTC_MAX_MODES];  // per sub-step: first segment (host)
+  // each segment's slots cut into pieces: first piece per segment, first slot
+  // per piece, pieces' partial sums, and per sub-step the first piece (host)
+  uint32_t *sgd_seg_piece[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_piece_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  double *sgd_part[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint64_t> sgd_piece_off[CTC_MAX_MODES];
   uint32_t *sgd_long = nullptr;
   uint64_t sgd_nlong = 0;
   uint32_t sgd_long_span = 0, sgd_long_sub = 0;

@@ -214,23 +214,25 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
 // ==========================================================================
 // Bulk-synchronous SGD (the default schedule).  One launch of k_sgd_sync is
 // one sub-step: walker w runs nonzeros [w*span + s*sub, +sub) of its span
-// from the factors as they stood when the sub-step began, and one launch of
-// k_sgd_merge then folds every row's handed-in deltas into the factors.
+// from the factors as they stood when the sub-step began; k_sgd_merge and
+// k_sgd_stage_merge then fold every row's handed-in deltas into the factors.
 //   * Along a run of consecutive nonzeros with the same index in a mode the
 //     walker keeps its copy of that row in registers (read where the run
 //     starts) and applies the reference step to it (SG/sgd.cu:86-99).
-//   * When a run ends its delta is handed in: as 64-bit fixed-point integers
-//     (2^-32 units) added atomically into the row's delta sum D, and a count
-//     of runs.  Integer addition is associative, so the sums do not depend on
-//     the order the walkers finish in and the sweep is bitwise reproducible.
-//   * A mode-0 row of at most long_min - 1 nonzeros cannot meet a second
+//   * Modes 1..: when a run ends its delta is written to the run's staging
+//     slot (a static table numbers every chunk's runs in walk order), and
+//     k_sgd_stage_merge adds, per row, the sum of its slots (a cold row: the
+//     reference's atomicAdd, SG/sgd.cu:96-98) or their mean (a hot row:
+//     replica averaging, NC/sgd.cu:229-262 with the sum of :523-525), summed
+//     in a fixed order.  No atomics: the sweep is bitwise reproducible.
+//   * Mode 0: a row of at most long_min - 1 nonzeros cannot meet a second
 //     walker within one sub-step (two walkers' nonzeros of one sub-step are at
 //     least span - sub + 1 positions apart), so its walker stores its copy in
-//     place; longer mode-0 rows go through D like every other mode's rows.
-//   * k_sgd_merge: a cold row adds its delta sum (the reference's atomicAdd,
-//     SG/sgd.cu:96-98), a hot row (bit set in the mode's hot-row bitmap) adds
-//     the mean of its runs' deltas (replica averaging, NC/sgd.cu:229-262 with
-//     the sum of :523-525); both then clear D and the count.
+//     place.  A longer row's run deltas are added as 64-bit fixed-point
+//     integers (2^-32 units) into its delta sum D (integer addition is
+//     associative, so the sum does not depend on the order the walkers finish
+//     in), a hot row's into kSyncReplicas replicas with a count of runs, and
+//     k_sgd_merge adds the sum (cold) or the mean (hot) and clears them.
 // ==========================================================================
 constexpr double kFixScale = 4294967296.0;        // 2^32
 constexpr double kFixInv = 1.0 / 4294967296.0;    // 2^-32
@@ -269,26 +271,34 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
   }
   uint32_t slot = 0;
   bool own0 = false;  // the current mode-0 row is stored in place (no other walker meets it)
-  bool staged[N];     // the current row of mode k writes its deltas to the staging slots
+  // modes 1..: the next staging slot of this chunk's runs
+  uint32_t rslot[N];
 #pragma unroll
-  for (int k = 0; k < N; ++k) staged[k] = false;
+  for (int k = 0; k < N; ++k)
+    rslot[k] = (k >= 1 && walker < nspans) ? sy.sbase[k][(size_t)s * nspans + walker] : 0u;
   auto hand_in = [&](int q) {
-    if (staged[q]) return;  // every delta of the run is in its staging slots
-    if (q == 0 && own0) {
+    if (q >= 1) {  // the run's delta goes to its staging slot
+      float *dst = sy.stage[q] + (size_t)rslot[q]++ * R;
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) dst[wl + k * W] = del[q][k];
+      return;
+    }
+    if (own0) {
 #pragma unroll
       for (int k = 0; k < CPL; ++k) Fm[0][(size_t)cur[0] * R + wl + k * W] = v[0][k];
       return;
     }
-    const uint32_t h = sy.hidx[q][cur[q]];
+    // a long mode-0 row: fixed-point sums
+    const uint32_t h = sy.hidx[0][cur[0]];
     unsigned long long *d;
     if (h == 0xFFFFFFFFu) {
-      d = (unsigned long long *)sy.D[q] + (size_t)cur[q] * R;
+      d = (unsigned long long *)sy.D[0] + (size_t)cur[0] * R;
     } else {
       // a hot row's mean needs its count of runs; a cold row adds its delta
       // sum, so the merge needs no count (an untouched row's sum is 0)
       const size_t r = (size_t)h * kSyncReplicas + walker % kSyncReplicas;
-      d = (unsigned long long *)sy.Dh[q] + r * R;
-      if (wl == 0) atomicAdd(sy.ch[q] + r, 1u);
+      d = (unsigned long long *)sy.Dh[0] + r * R;
+      if (wl == 0) atomicAdd(sy.ch[0] + r, 1u);
     }
 #pragma unroll
     for (int k = 0; k < CPL; ++k)
@@ -338,7 +348,6 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
           if (idx[k] != cur[k]) {
             if (cur[k] != NONE) hand_in(k);
             if (k == 0) own0 = c.rowptr[slot + 1] - c.rowptr[slot] < long_min;
-            if (k >= 2) staged[k] = sy.stage[k] != nullptr && sy.hidx[k][idx[k]] == 0xFFFFFFFFu;
             cur[k] = idx[k];
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
@@ -370,10 +379,7 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
 #pragma unroll
           for (int k = 0; k < N; ++k) {
             v[k][j] += d[k];
-            if (k >= 2 && staged[k])
-              sy.stage[k][(walker * sub + (e - e0)) * R + wl + j * W] = d[k];
-            else
-              del[k][j] += d[k];
+            del[k][j] += d[k];
           }
         }
       }
@@ -437,20 +443,20 @@ __global__ void __launch_bounds__(256) k_sgd_merge(MergeArgs a) {
   }
 }
 
-// One workgroup per staged row: its walkers take the row's staging slots in
-// turn (FP64 partial sums), the partials are added in walker order, and the
-// row adds the total.  The order is fixed, so the result is reproducible.
+// One workgroup per piece of a staged row's slots: its walkers take the
+// piece's slots in turn (FP64 partial sums) and the partials are added in
+// walker order.  The order is fixed, so the result is reproducible.
 template <int R>
-__global__ void __launch_bounds__(256) k_sgd_stage_merge(StageArgs a) {
+__global__ void __launch_bounds__(256) k_sgd_stage_part(StageArgs a) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int NW = 256 / W;  // walkers per workgroup
   __shared__ double part[NW][R];
   const uint64_t g = blockIdx.x;
-  if (g >= a.nseg) return;
+  if (g >= a.npiece) return;
   const int wid = threadIdx.x / W;
   const int wl = threadIdx.x % W;
-  const uint32_t b = a.seg_start[g], e = a.seg_start[g + 1];
+  const uint32_t b = a.piece_start[a.piece0 + g], e = a.piece_start[a.piece0 + g + 1];
   double acc[CPL];
 #pragma unroll
   for (int j = 0; j < CPL; ++j) acc[j] = 0.0;
@@ -479,9 +485,31 @@ __global__ void __launch_bounds__(256) k_sgd_stage_merge(StageArgs a) {
   if (threadIdx.x < R) {
     double t = 0.0;
     for (int w = 0; w < NW; ++w) t += part[w][threadIdx.x];
-    float *gr = a.F + (size_t)a.seg_row[g] * R;
-    gr[threadIdx.x] = (float)((double)gr[threadIdx.x] + t);
+    a.part[g * R + threadIdx.x] = t;
   }
+}
+
+// One walker per staged row: its pieces' partial sums in order, then the row
+// adds the total (a cold row) or the total over its count of runs (a hot row).
+template <int R>
+__global__ void __launch_bounds__(256) k_sgd_stage_fold(StageArgs a) {
+  constexpr int W = R < 64 ? R : 64;
+  constexpr int CPL = R / W;
+  constexpr int WPW = 64 / W;
+  const int wl = lane_id() % W;
+  const uint64_t g = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * WPW + lane_id() / W;
+  if (g >= a.nseg) return;
+  double t[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) t[j] = 0.0;
+  for (uint32_t q = a.seg_piece[g]; q < a.seg_piece[g + 1]; ++q)
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) t[j] += a.part[(size_t)(q - a.piece0) * R + wl + j * W];
+  const uint32_t row = a.seg_row[g];
+  const double sc = ((a.hot[row >> 5] >> (row & 31)) & 1u) ? 1.0 / (double)(a.seg_start[g + 1] - a.seg_start[g]) : 1.0;
+  float *gr = a.F + (size_t)row * R;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) gr[wl + j * W] = (float)((double)gr[wl + j * W] + t[j] * sc);
 }
 
 // ==========================================================================
@@ -1245,7 +1273,10 @@ hipError_t launch_sgd_merge(hipStream_t st, const MergeArgs &a, int R) {
 
 hipError_t launch_sgd_stage_merge(hipStream_t st, const StageArgs &a, int R) {
   if (a.nseg == 0) return hipSuccess;
-  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_stage_merge<RR>), dim3((unsigned)a.nseg), dim3(256), 0, st, a));
+  const int W = R < 64 ? R : 64;
+  const unsigned fold_grid = grid_for((a.nseg + (64 / W) - 1) / (64 / W), 256);
+  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_stage_part<RR>), dim3((unsigned)a.npiece), dim3(256), 0, st, a);
+                  hipLaunchKernelGGL((k_sgd_stage_fold<RR>), dim3(fold_grid), dim3(256), 0, st, a));
   return hipGetLastError();
 }
 

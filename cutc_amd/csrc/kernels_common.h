@@ -50,28 +50,39 @@ struct HotRows {
 // the walkers sharing it do not all queue on one memory channel; hidx[row] is
 // the row's index among its mode's hot rows (0xFFFFFFFF: cold).
 constexpr int kSyncReplicas = 32;
-// Modes 2.. (CISS-relative) change row at almost every nonzero; for their
-// cold rows a walker writes each nonzero's delta to stage[k] at slot
-// walker * sub + (position within the sub-step) instead of adding run sums
-// atomically, and k_sgd_stage_merge sums every row's slots in a fixed order.
+// Modes 1.. (CISS-relative) hand in each run's delta to a staging slot: the
+// runs of walker w's chunk in sub-step s take slots sbase[k][s * nspans + w],
+// +1, ... in walk order, and k_sgd_stage_merge sums (cold row) or averages
+// (hot row) every row's slots in a fixed order.  Mode 0's long rows add
+// fixed-point sums (cold) or replicated sums (hot) through D / Dh.
 struct SyncArgs {
   long long *D[4];
   const uint32_t *hidx[4];
   long long *Dh[4];   // hot rows x kSyncReplicas x R
   uint32_t *ch[4];    // hot rows x kSyncReplicas
-  float *stage[4];    // null: the mode hands in run sums through D
+  float *stage[4];    // modes 1..: runs of one sub-step x R
+  const uint32_t *sbase[4];  // modes 1..: first slot of each (sub-step, walker) chunk
 };
 
-// k_sgd_stage_merge: the rows of one staged mode that took deltas in one
-// sub-step; segment g is row seg_row[g], whose deltas are in the staging
-// slots slots[seg_start[g] .. seg_start[g + 1]).
+// Staged merge of one mode for one sub-step.  Segment g is row seg_row[g],
+// whose runs' deltas are in the staging slots slots[seg_start[g] ..
+// seg_start[g + 1]); the segment's slots are cut into pieces of at most
+// kStagePiece slots, pieces seg_piece[g] .. seg_piece[g + 1] (piece p covers
+// slots[piece_start[p] .. piece_start[p + 1])).  k_sgd_stage_part sums each
+// piece into part[p - piece0]; k_sgd_stage_fold adds each row's pieces in
+// order and applies the sum (cold row) or the mean over its runs (hot row).
+constexpr uint32_t kStagePiece = 256;
 struct StageArgs {
   float *F;
+  const uint32_t *hot;  // the mode's hot-row bitmap
   const float *stage;
   const uint32_t *seg_row;
   const uint32_t *seg_start;
+  const uint32_t *seg_piece;
+  const uint32_t *piece_start;
   const uint32_t *slots;
-  uint64_t nseg;
+  double *part;
+  uint64_t nseg, npiece, piece0;
 };
 
 // k_sgd_merge: segment g covers merge items [off[g], off[g + 1]) of mode

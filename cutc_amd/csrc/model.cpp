@@ -418,6 +418,10 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->sgd_seg_row[md]);
     dev_free(m->sgd_seg_start[md]);
     dev_free(m->sgd_slots[md]);
+    dev_free(m->sgd_sbase[md]);
+    dev_free(m->sgd_seg_piece[md]);
+    dev_free(m->sgd_piece_start[md]);
+    dev_free(m->sgd_part[md]);
     dev_free(m->sgd_hidx[md]);
     dev_free(m->sgd_hrow[md]);
     dev_free(m->sgd_Dh[md]);
@@ -675,64 +679,101 @@ static ctc_status sync_shape(const ctc_options *o, uint32_t *span, uint32_t *sub
 // two walkers can meet in one sub-step (length >= span - sub + 2).
 static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
   hipStream_t st = m->ctx->stream;
-  for (uint32_t md = 0; md < m->nmodes; ++md) {
-    if (m->sgd_D[md]) continue;
-    CTC_TRY(dev_alloc(&m->sgd_D[md], (size_t)m->dims[md] * m->rank));
-    CTC_HIP(hipMemsetAsync(m->sgd_D[md], 0, (size_t)m->dims[md] * m->rank * sizeof(long long), st));
+  if (!m->sgd_D[0]) {  // mode 0's long rows: fixed-point delta sums
+    CTC_TRY(dev_alloc(&m->sgd_D[0], (size_t)m->dims[0] * m->rank));
+    CTC_HIP(hipMemsetAsync(m->sgd_D[0], 0, (size_t)m->dims[0] * m->rank * sizeof(long long), st));
   }
   if (m->sgd_long && m->sgd_long_span == span && m->sgd_long_sub == sub && m->sgd_long_thr == m->hot_threshold)
     return CTC_OK;
   const DevCiss &d = m->ciss[0];
-  // staged modes: slot of position p is walker * sub + (p - walker * span) % sub
-  // in sub-step (p - walker * span) / sub; per sub-step the cold rows' slots
-  // are grouped by row, each group in walk order
+  // Modes 1.. hand in each run's delta to a staging slot.  The runs of a
+  // walker's chunk (walker w, sub-step s) are the maximal runs of equal index
+  // in it, numbered in walk order from base[s][w] (the runs of the walkers
+  // before w in the same sub-step); per sub-step the slots are grouped by row
+  // in slot (= walk) order for the merge.
   const uint64_t nnz = d.nnz, nspans = (nnz + span - 1) / span, nsub = (span + sub - 1) / sub;
-  for (uint32_t k = 2; k < m->nmodes; ++k) {
+  for (uint32_t k = 1; k < m->nmodes; ++k) {
     for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_seg_row[k], (void *)m->sgd_seg_start[k],
-                    (void *)m->sgd_slots[k]})
+                    (void *)m->sgd_slots[k], (void *)m->sgd_sbase[k], (void *)m->sgd_seg_piece[k],
+                    (void *)m->sgd_piece_start[k], (void *)m->sgd_part[k]})
       dev_free(p);
     m->sgd_stage[k] = nullptr;
-    m->sgd_seg_row[k] = m->sgd_seg_start[k] = m->sgd_slots[k] = nullptr;
-    std::vector<uint32_t> idx(nnz), hidx(m->dims[k]);
+    m->sgd_part[k] = nullptr;
+    m->sgd_seg_row[k] = m->sgd_seg_start[k] = m->sgd_slots[k] = m->sgd_sbase[k] = nullptr;
+    m->sgd_seg_piece[k] = m->sgd_piece_start[k] = nullptr;
+    std::vector<uint32_t> idx(nnz);
     CTC_HIP(hipMemcpy(idx.data(), d.oidx[k - 1], nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    CTC_HIP(hipMemcpy(hidx.data(), m->sgd_hidx[k], hidx.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    // bucket the cold positions by sub-step (stable), then sort each bucket by row
+    auto starts_run = [&](uint64_t p) { return (p % span) % sub == 0 || idx[p] != idx[p - 1]; };
+    // runs per (sub-step, walker) -> bases, and per sub-step
+    std::vector<uint32_t> base(nsub * nspans, 0);
     std::vector<uint64_t> boff(nsub + 1, 0);
     for (uint64_t p = 0; p < nnz; ++p)
-      if (hidx[idx[p]] == 0xFFFFFFFFu) boff[(p % span) / sub + 1]++;
-    for (uint64_t s2 = 0; s2 < nsub; ++s2) boff[s2 + 1] += boff[s2];
+      if (starts_run(p)) {
+        const uint64_t w = p / span, s2 = (p - w * span) / sub;
+        base[s2 * nspans + w]++;
+      }
+    uint64_t max_runs = 0;
+    for (uint64_t s2 = 0; s2 < nsub; ++s2) {
+      uint64_t acc = 0;
+      for (uint64_t w = 0; w < nspans; ++w) {
+        const uint32_t c = base[s2 * nspans + w];
+        base[s2 * nspans + w] = (uint32_t)acc;
+        acc += c;
+      }
+      boff[s2 + 1] = boff[s2] + acc;
+      max_runs = std::max(max_runs, acc);
+    }
+    // (row, slot) keys of every run, bucketed by sub-step and sorted by row
     std::vector<uint64_t> key(boff[nsub]);
     {
-      std::vector<uint64_t> fill(boff.begin(), boff.end() - 1);
+      uint64_t w_prev = ~0ull, s_prev = ~0ull, run = 0;
       for (uint64_t p = 0; p < nnz; ++p) {
-        if (hidx[idx[p]] != 0xFFFFFFFFu) continue;
-        const uint64_t w = p / span, in = p - w * span, slotp = w * sub + in % sub;
-        key[fill[in / sub]++] = ((uint64_t)idx[p] << 32) | slotp;
+        if (!starts_run(p)) continue;
+        const uint64_t w = p / span, s2 = (p - w * span) / sub;
+        if (w != w_prev || s2 != s_prev) run = 0, w_prev = w, s_prev = s2;
+        const uint64_t slotr = (uint64_t)base[s2 * nspans + w] + run++;
+        key[boff[s2] + slotr] = ((uint64_t)idx[p] << 32) | slotr;
       }
     }
     parallel_for(nsub, host_threads(), [&](uint64_t b, uint64_t e, unsigned) {
       for (uint64_t s2 = b; s2 < e; ++s2) std::sort(key.begin() + boff[s2], key.begin() + boff[s2 + 1]);
     });
-    std::vector<uint32_t> seg_row, seg_start, slots(key.size());
-    std::vector<uint64_t> &soff = m->sgd_seg_off[k];
+    // segments (one per row of a sub-step) cut into pieces of at most
+    // kStagePiece slots
+    std::vector<uint32_t> seg_row, seg_start, seg_piece, piece_start, slots(key.size());
+    std::vector<uint64_t> &soff = m->sgd_seg_off[k], &poff = m->sgd_piece_off[k];
     soff.assign(nsub + 1, 0);
+    poff.assign(nsub + 1, 0);
+    uint64_t max_pieces = 0;
     for (uint64_t s2 = 0; s2 < nsub; ++s2) {
       soff[s2] = seg_row.size();
+      poff[s2] = piece_start.size();
       for (uint64_t q = boff[s2]; q < boff[s2 + 1]; ++q) {
         const uint32_t row = (uint32_t)(key[q] >> 32);
-        if (q == boff[s2] || row != (uint32_t)(key[q - 1] >> 32)) {
+        const bool new_seg = q == boff[s2] || row != (uint32_t)(key[q - 1] >> 32);
+        if (new_seg) {
           seg_row.push_back(row);
           seg_start.push_back((uint32_t)q);
+          seg_piece.push_back((uint32_t)piece_start.size());
         }
+        if ((q - seg_start.back()) % kStagePiece == 0) piece_start.push_back((uint32_t)q);
         slots[q] = (uint32_t)key[q];
       }
+      max_pieces = std::max<uint64_t>(max_pieces, piece_start.size() - poff[s2]);
     }
     soff[nsub] = seg_row.size();
+    poff[nsub] = piece_start.size();
     seg_start.push_back((uint32_t)key.size());
+    seg_piece.push_back((uint32_t)piece_start.size());
+    piece_start.push_back((uint32_t)key.size());
     CTC_TRY(upload(&m->sgd_seg_row[k], seg_row.data(), seg_row.size(), st));
     CTC_TRY(upload(&m->sgd_seg_start[k], seg_start.data(), seg_start.size(), st));
+    CTC_TRY(upload(&m->sgd_seg_piece[k], seg_piece.data(), seg_piece.size(), st));
+    CTC_TRY(upload(&m->sgd_piece_start[k], piece_start.data(), piece_start.size(), st));
     CTC_TRY(upload(&m->sgd_slots[k], slots.data(), slots.size(), st));
-    CTC_TRY(dev_alloc(&m->sgd_stage[k], (size_t)nspans * sub * m->rank));
+    CTC_TRY(upload(&m->sgd_sbase[k], base.data(), base.size(), st));
+    CTC_TRY(dev_alloc(&m->sgd_stage[k], (size_t)std::max<uint64_t>(1, max_runs) * m->rank));
+    CTC_TRY(dev_alloc(&m->sgd_part[k], (size_t)std::max<uint64_t>(1, max_pieces) * m->rank));
     CTC_HIP(hipStreamSynchronize(st));
   }
   const uint64_t long_min = (uint64_t)span - sub + 2;
@@ -771,20 +812,19 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
     sy.Dh[k] = m->sgd_Dh[k];
     sy.ch[k] = m->sgd_ch[k];
     sy.stage[k] = m->sgd_stage[k];
+    sy.sbase[k] = m->sgd_sbase[k];
     mg.F[k] = m->F[k];
     mg.D[k] = m->sgd_D[k];
     mg.Dh[k] = m->sgd_Dh[k];
     mg.ch[k] = m->sgd_ch[k];
     mg.hrow[k] = m->sgd_hrow[k];
-    // cold segment: mode 0's long rows, every row of mode 1 (the staged
-    // modes' cold rows are summed by k_sgd_stage_merge)
-    if (k < 2) {
-      mg.mode[mg.nseg] = (int)k;
-      mg.is_hot[mg.nseg] = 0;
-      mg.rows[mg.nseg] = k == 0 ? m->sgd_long : nullptr;
-      mg.off[mg.nseg + 1] = mg.off[mg.nseg] + (k == 0 ? m->sgd_nlong : m->dims[k]);
-      ++mg.nseg;
-    }
+    if (k > 0) continue;  // modes 1.. are merged from their staging slots
+    // mode 0: its long rows' sums, then its hot rows' replicated sums
+    mg.mode[mg.nseg] = 0;
+    mg.is_hot[mg.nseg] = 0;
+    mg.rows[mg.nseg] = m->sgd_long;
+    mg.off[mg.nseg + 1] = mg.off[mg.nseg] + m->sgd_nlong;
+    ++mg.nseg;
     if (m->sgd_nhot[k]) {  // hot segment: the mode's hot rows
       mg.mode[mg.nseg] = (int)k;
       mg.is_hot[mg.nseg] = 1;
@@ -808,14 +848,20 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
     {
       Prof p(m->ctx, "sgd_merge");
       CTC_HIP(launch_sgd_merge(st, mg, (int)R));
-      for (uint32_t k = 2; k < N; ++k) {
+      for (uint32_t k = 1; k < N; ++k) {
         StageArgs sa{};
         sa.F = m->F[k];
+        sa.hot = m->hotbits[k];
         sa.stage = m->sgd_stage[k];
         sa.seg_row = m->sgd_seg_row[k] + m->sgd_seg_off[k][s];
         sa.seg_start = m->sgd_seg_start[k] + m->sgd_seg_off[k][s];
+        sa.seg_piece = m->sgd_seg_piece[k] + m->sgd_seg_off[k][s];
+        sa.piece_start = m->sgd_piece_start[k];
         sa.slots = m->sgd_slots[k];
+        sa.part = m->sgd_part[k];
         sa.nseg = m->sgd_seg_off[k][s + 1] - m->sgd_seg_off[k][s];
+        sa.piece0 = m->sgd_piece_off[k][s];
+        sa.npiece = m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s];
         CTC_HIP(launch_sgd_stage_merge(st, sa, (int)R));
       }
     }
