@@ -13,6 +13,10 @@
 #include "../../include/ctc.h"
 
 namespace ctc {
+struct FoldSeg;  // kernels_common.h
+}
+
+namespace ctc {
 
 // ---------------------------------------------------------------- errors --
 void set_error(const char *fmt, ...);
@@ -171,20 +175,19 @@ struct ctc_model {
   uint32_t *sgd_ch[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t sgd_nhot[CTC_MAX_MODES] = {0, 0, 0, 0};
   // modes 1..N-1: staging slots (runs of the largest sub-step x R), each
-  // chunk's first slot (sub-steps x walkers) and, per sub-step, the rows that
-  // took runs with their slots in walk order
+  // chunk's first slot (sub-steps x walkers), every row's slots grouped per
+  // sub-step and cut into pieces (first slot and whole row of each piece),
+  // the pieces' partial sums, the rows of several pieces, and per sub-step
+  // the first piece and first multi-piece row (host)
   float *sgd_stage[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_sbase[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t *sgd_seg_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t *sgd_seg_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_slots[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
-  std::vector<uint64_t> sgd_seg_off[CTC_MAX_MODES];  // per sub-step: first segment (host)
-  // each segment's slots cut into pieces: first piece per segment, first slot
-  // per piece, pieces' partial sums, and per sub-step the first piece (host)
-  uint32_t *sgd_seg_piece[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_piece_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_piece_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   double *sgd_part[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  ctc::FoldSeg *sgd_fold[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<uint64_t> sgd_piece_off[CTC_MAX_MODES];
+  std::vector<uint64_t> sgd_fold_off[CTC_MAX_MODES];
   uint32_t *sgd_long = nullptr;
   uint64_t sgd_nlong = 0;
   uint32_t sgd_long_span = 0, sgd_long_sub = 0;

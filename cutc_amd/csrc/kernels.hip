@@ -214,14 +214,14 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
 // ==========================================================================
 // Bulk-synchronous SGD (the default schedule).  One launch of k_sgd_sync is
 // one sub-step: walker w runs nonzeros [w*span + s*sub, +sub) of its span
-// from the factors as they stood when the sub-step began; k_sgd_merge and
-// k_sgd_stage_merge then fold every row's handed-in deltas into the factors.
+// from the factors as they stood when the sub-step began; k_sgd_settle and
+// k_sgd_fold then add every row's handed-in deltas to the factors.
 //   * Along a run of consecutive nonzeros with the same index in a mode the
 //     walker keeps its copy of that row in registers (read where the run
 //     starts) and applies the reference step to it (SG/sgd.cu:86-99).
 //   * Modes 1..: when a run ends its delta is written to the run's staging
 //     slot (a static table numbers every chunk's runs in walk order), and
-//     k_sgd_stage_merge adds, per row, the sum of its slots (a cold row: the
+//     k_sgd_settle adds, per row, the sum of its slots (a cold row: the
 //     reference's atomicAdd, SG/sgd.cu:96-98) or their mean (a hot row:
 //     replica averaging, NC/sgd.cu:229-262 with the sum of :523-525), summed
 //     in a fixed order.  No atomics: the sweep is bitwise reproducible.
@@ -232,18 +232,46 @@ __global__ void __launch_bounds__(256) k_sgd(CissArgs c, FactorArgs fa, HotRows 
 //     integers (2^-32 units) into its delta sum D (integer addition is
 //     associative, so the sum does not depend on the order the walkers finish
 //     in), a hot row's into kSyncReplicas replicas with a count of runs, and
-//     k_sgd_merge adds the sum (cold) or the mean (hot) and clears them.
+//     k_sgd_fold adds the sum (cold) or the mean (hot) and clears them.
 // ==========================================================================
 constexpr double kFixScale = 4294967296.0;        // 2^32
 constexpr double kFixInv = 1.0 / 4294967296.0;    // 2^-32
+
+// A walker is W = R / 4 lanes of a wave; lane l holds the row elements
+// 4l .. 4l + 3, so a row moves with one 16-byte access per lane and a wave
+// carries 256 / R walkers (eight at R = 32).  Each walker is a chain of
+// dependent steps whose row reads miss the L2 (the launch starts with it
+// invalidated), so the more walkers a wave carries, the more of those reads
+// are in flight at once.
+template <int R>
+struct SyncShape {
+  static constexpr int CPL = 4;
+  static constexpr int W = R / CPL;
+  static constexpr int WPW = 64 / W;
+};
+
+template <int CPL>
+__device__ inline void load_row4(const float *p, float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(p + j);
+    v[j] = t.x, v[j + 1] = t.y, v[j + 2] = t.z, v[j + 3] = t.w;
+  }
+}
+
+template <int CPL>
+__device__ inline void store_row4(float *p, const float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) *reinterpret_cast<float4 *>(p + j) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
+}
 
 template <int R, int N>
 __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, SyncArgs sy, uint32_t span,
                                                   uint32_t sub, uint32_t s, uint64_t nspans, uint32_t long_min,
                                                   float step, float lam) {
-  constexpr int W = R < 64 ? R : 64;
-  constexpr int CPL = R / W;
-  constexpr int WPW = 64 / W;
+  constexpr int W = SyncShape<R>::W;
+  constexpr int CPL = SyncShape<R>::CPL;
+  constexpr int WPW = SyncShape<R>::WPW;
   constexpr uint32_t NONE = 0xFFFFFFFFu;
   const int lane = lane_id();
   const int sub_w = lane / W;
@@ -278,14 +306,11 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
     rslot[k] = (k >= 1 && walker < nspans) ? sy.sbase[k][(size_t)s * nspans + walker] : 0u;
   auto hand_in = [&](int q) {
     if (q >= 1) {  // the run's delta goes to its staging slot
-      float *dst = sy.stage[q] + (size_t)rslot[q]++ * R;
-#pragma unroll
-      for (int k = 0; k < CPL; ++k) dst[wl + k * W] = del[q][k];
+      store_row4<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
       return;
     }
     if (own0) {
-#pragma unroll
-      for (int k = 0; k < CPL; ++k) Fm[0][(size_t)cur[0] * R + wl + k * W] = v[0][k];
+      store_row4<CPL>(Fm[0] + (size_t)cur[0] * R + wl * CPL, v[0]);
       return;
     }
     // a long mode-0 row: fixed-point sums
@@ -302,7 +327,7 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
     }
 #pragma unroll
     for (int k = 0; k < CPL; ++k)
-      atomicAdd(d + wl + k * W, (unsigned long long)__double2ll_rn((double)del[q][k] * kFixScale));
+      atomicAdd(d + wl * CPL + k, (unsigned long long)__double2ll_rn((double)del[q][k] * kFixScale));
   };
 
   for (uint32_t ch = 0; ch < sub; ch += W) {
@@ -321,6 +346,7 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
       for (int q = 0; q < N - 1; ++q) il[q] = 0;
     }
     const uint64_t hm = __ballot(vl && hl);
+#pragma unroll 1  // rolled: 76 VGPRs, six waves per SIMD
     for (int q = 0; q < W; ++q) {
       const int src = sub_w * W + q;
       uint32_t idx[N];
@@ -349,11 +375,9 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
             if (cur[k] != NONE) hand_in(k);
             if (k == 0) own0 = c.rowptr[slot + 1] - c.rowptr[slot] < long_min;
             cur[k] = idx[k];
+            load_row4<CPL>(Fm[k] + (size_t)idx[k] * R + wl * CPL, v[k]);
 #pragma unroll
-            for (int j = 0; j < CPL; ++j) {
-              v[k][j] = Fm[k][(size_t)idx[k] * R + wl + j * W];
-              del[k][j] = 0.f;
-            }
+            for (int j = 0; j < CPL; ++j) del[k][j] = 0.f;
           }
         }
         float part = 0.f;
@@ -390,14 +414,17 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
     if (cur[k] != NONE) hand_in(k);
 }
 
+// Mode 0's long rows (k_sgd_sync's fixed-point sums) as block `blk` of a
+// 256-thread launch: a cold row adds its delta sum D, a hot row the mean of
+// its replicas' sums Dh over their count of runs ch; both are cleared.
 template <int R>
-__global__ void __launch_bounds__(256) k_sgd_merge(MergeArgs a) {
+__device__ inline void sgd_merge_block(const MergeArgs &a, uint64_t blk) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int WPW = 64 / W;
   const int lane = lane_id();
   const int wl = lane % W;
-  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t wave = blk * 4 + (threadIdx.x >> 6);
   const uint64_t i = wave * WPW + lane / W;
   if (i >= a.off[a.nseg]) return;
   int g = 0;
@@ -443,20 +470,36 @@ __global__ void __launch_bounds__(256) k_sgd_merge(MergeArgs a) {
   }
 }
 
-// One workgroup per piece of a staged row's slots: its walkers take the
-// piece's slots in turn (FP64 partial sums) and the partials are added in
-// walker order.  The order is fixed, so the result is reproducible.
+// One launch per sub-step settles every piece of the staged modes: a
+// workgroup per piece, whose walkers take
+// the piece's slots in turn (FP64 partial sums) and add their partials in
+// walker order.  A piece that holds a whole row applies it; the others leave
+// their sums for k_sgd_fold.  The order is fixed, so the result is
+// reproducible.
 template <int R>
-__global__ void __launch_bounds__(256) k_sgd_stage_part(StageArgs a) {
+__global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int NW = 256 / W;  // walkers per workgroup
   __shared__ double part[NW][R];
-  const uint64_t g = blockIdx.x;
-  if (g >= a.npiece) return;
+  const uint64_t pblk = blockIdx.x;
+  if (pblk >= a.pb[4]) return;
+  int k = 1;
+  while (k < 3 && pblk >= a.pb[k + 1]) ++k;
+  const uint64_t gp = a.piece0[k] + (pblk - a.pb[k]);
   const int wid = threadIdx.x / W;
   const int wl = threadIdx.x % W;
-  const uint32_t b = a.piece_start[a.piece0 + g], e = a.piece_start[a.piece0 + g + 1];
+  const uint32_t *slots = a.slots[k];
+  const float *stage = a.stage[k];
+  const uint32_t b = a.piece_start[k][gp], e = a.piece_start[k][gp + 1];
+  // a whole row's factor entry and hot bit, read while the slots are summed
+  const uint32_t row = a.piece_row[k][gp];
+  float fv = 0.f;
+  uint32_t hw = 0;
+  if (row != 0xFFFFFFFFu && threadIdx.x < R) {
+    fv = a.F[k][(size_t)row * R + threadIdx.x];
+    hw = a.hot[k][row >> 5];
+  }
   double acc[CPL];
 #pragma unroll
   for (int j = 0; j < CPL; ++j) acc[j] = 0.0;
@@ -465,7 +508,7 @@ __global__ void __launch_bounds__(256) k_sgd_stage_part(StageArgs a) {
     float x[4][CPL];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float *src = a.stage + (size_t)a.slots[p + u * NW] * R;
+      const float *src = stage + (size_t)slots[p + u * NW] * R;
 #pragma unroll
       for (int j = 0; j < CPL; ++j) x[u][j] = src[wl + j * W];
     }
@@ -475,7 +518,7 @@ __global__ void __launch_bounds__(256) k_sgd_stage_part(StageArgs a) {
       for (int j = 0; j < CPL; ++j) acc[j] += (double)x[u][j];
   }
   for (; p < e; p += NW) {
-    const float *src = a.stage + (size_t)a.slots[p] * R;
+    const float *src = stage + (size_t)slots[p] * R;
 #pragma unroll
     for (int j = 0; j < CPL; ++j) acc[j] += (double)src[wl + j * W];
   }
@@ -485,29 +528,44 @@ __global__ void __launch_bounds__(256) k_sgd_stage_part(StageArgs a) {
   if (threadIdx.x < R) {
     double t = 0.0;
     for (int w = 0; w < NW; ++w) t += part[w][threadIdx.x];
-    a.part[g * R + threadIdx.x] = t;
+    if (row == 0xFFFFFFFFu) {
+      a.part[k][(gp - a.piece0[k]) * R + threadIdx.x] = t;
+    } else {
+      const double sc = ((hw >> (row & 31)) & 1u) ? 1.0 / (double)(e - b) : 1.0;
+      a.F[k][(size_t)row * R + threadIdx.x] = (float)((double)fv + (0.0 + t) * sc);
+    }
   }
 }
 
-// One walker per staged row: its pieces' partial sums in order, then the row
-// adds the total (a cold row) or the total over its count of runs (a hot row).
+// Blocks 0 .. nb_merge - 1 merge mode 0's long rows (sgd_merge_block); after
+// them one walker per staged row of more than one piece adds its pieces'
+// partial sums in order, then the total (a cold row) or the total over its
+// count of runs (a hot row) to the row.
 template <int R>
-__global__ void __launch_bounds__(256) k_sgd_stage_fold(StageArgs a) {
+__global__ void __launch_bounds__(256) k_sgd_fold(SettleArgs a) {
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int WPW = 64 / W;
+  if (blockIdx.x < a.nb_merge) {
+    sgd_merge_block<R>(a.mg, blockIdx.x);
+    return;
+  }
   const int wl = lane_id() % W;
-  const uint64_t g = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * WPW + lane_id() / W;
-  if (g >= a.nseg) return;
+  const uint64_t i = ((uint64_t)(blockIdx.x - a.nb_merge) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * WPW + lane_id() / W;
+  int k = 1;
+  while (k < 3 && i >= a.fb[k + 1]) ++k;
+  if (i >= a.fb[k + 1]) return;
+  const FoldSeg sg = a.fold[k][i - a.fb[k]];
+  const double *pt = a.part[k];
   double t[CPL];
 #pragma unroll
   for (int j = 0; j < CPL; ++j) t[j] = 0.0;
-  for (uint32_t q = a.seg_piece[g]; q < a.seg_piece[g + 1]; ++q)
+  for (uint32_t q = 0; q < sg.npiece; ++q)
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) t[j] += a.part[(size_t)(q - a.piece0) * R + wl + j * W];
-  const uint32_t row = a.seg_row[g];
-  const double sc = ((a.hot[row >> 5] >> (row & 31)) & 1u) ? 1.0 / (double)(a.seg_start[g + 1] - a.seg_start[g]) : 1.0;
-  float *gr = a.F + (size_t)row * R;
+    for (int j = 0; j < CPL; ++j) t[j] += pt[(size_t)(sg.piece + q) * R + wl + j * W];
+  const uint32_t *hot = a.hot[k];
+  const double sc = ((hot[sg.row >> 5] >> (sg.row & 31)) & 1u) ? 1.0 / (double)sg.runs : 1.0;
+  float *gr = a.F[k] + (size_t)sg.row * R;
 #pragma unroll
   for (int j = 0; j < CPL; ++j) gr[wl + j * W] = (float)((double)gr[wl + j * W] + t[j] * sc);
 }
@@ -1248,8 +1306,8 @@ hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &
                            unsigned block) {
   if (c.nnz == 0) return hipSuccess;
   const uint64_t nspans = (c.nnz + span - 1) / span;
-  const int W = R < 64 ? R : 64;
-  const uint64_t waves = (nspans + (64 / W) - 1) / (64 / W);
+  const int wpw = 256 / R;  // walkers per wave (SyncShape)
+  const uint64_t waves = (nspans + wpw - 1) / wpw;
   const unsigned grid = grid_for(waves, block);
   if (nmodes == 3) {
     CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, 3>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam));
@@ -1261,22 +1319,15 @@ hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &
   return hipGetLastError();
 }
 
-hipError_t launch_sgd_merge(hipStream_t st, const MergeArgs &a, int R) {
-  const uint64_t items = a.off[a.nseg];
-  if (items == 0) return hipSuccess;
+hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R) {
   const int W = R < 64 ? R : 64;
-  const uint64_t waves = (items + (64 / W) - 1) / (64 / W);
-  const unsigned grid = grid_for(waves, 256);
-  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_merge<RR>), dim3(grid), dim3(256), 0, st, a));
-  return hipGetLastError();
-}
-
-hipError_t launch_sgd_stage_merge(hipStream_t st, const StageArgs &a, int R) {
-  if (a.nseg == 0) return hipSuccess;
-  const int W = R < 64 ? R : 64;
-  const unsigned fold_grid = grid_for((a.nseg + (64 / W) - 1) / (64 / W), 256);
-  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_stage_part<RR>), dim3((unsigned)a.npiece), dim3(256), 0, st, a);
-                  hipLaunchKernelGGL((k_sgd_stage_fold<RR>), dim3(fold_grid), dim3(256), 0, st, a));
+  const uint64_t items = a.mg.off[a.mg.nseg];
+  a.nb_merge = grid_for((items + (64 / W) - 1) / (64 / W), 256);
+  if (a.pb[4] > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  if (a.pb[4] > 0)
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+  const uint64_t fg = a.nb_merge + grid_for((a.fb[4] + (64 / W) - 1) / (64 / W), 256);
+  if (fg > 0) CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_fold<RR>), dim3((unsigned)fg), dim3(256), 0, st, a));
   return hipGetLastError();
 }
 

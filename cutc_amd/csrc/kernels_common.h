@@ -52,7 +52,7 @@ struct HotRows {
 constexpr int kSyncReplicas = 32;
 // Modes 1.. (CISS-relative) hand in each run's delta to a staging slot: the
 // runs of walker w's chunk in sub-step s take slots sbase[k][s * nspans + w],
-// +1, ... in walk order, and k_sgd_stage_merge sums (cold row) or averages
+// +1, ... in walk order, and k_sgd_settle sums (cold row) or averages
 // (hot row) every row's slots in a fixed order.  Mode 0's long rows add
 // fixed-point sums (cold) or replicated sums (hot) through D / Dh.
 struct SyncArgs {
@@ -64,28 +64,20 @@ struct SyncArgs {
   const uint32_t *sbase[4];  // modes 1..: first slot of each (sub-step, walker) chunk
 };
 
-// Staged merge of one mode for one sub-step.  Segment g is row seg_row[g],
-// whose runs' deltas are in the staging slots slots[seg_start[g] ..
-// seg_start[g + 1]); the segment's slots are cut into pieces of at most
-// kStagePiece slots, pieces seg_piece[g] .. seg_piece[g + 1] (piece p covers
-// slots[piece_start[p] .. piece_start[p + 1])).  k_sgd_stage_part sums each
-// piece into part[p - piece0]; k_sgd_stage_fold adds each row's pieces in
-// order and applies the sum (cold row) or the mean over its runs (hot row).
+// Staged merge of one sub-step (k_sgd_settle + k_sgd_fold).  Per staged
+// mode k, the sub-step's runs are grouped by row (slots[] lists each row's
+// staging slots in walk order) and every row's slots are cut into pieces of
+// at most kStagePiece slots: piece p covers slots[piece_start[p] ..
+// piece_start[p + 1]).  A row that fits one piece is settled by that piece
+// (piece_row[p] = the row); the pieces of a longer row (piece_row[p] =
+// 0xFFFFFFFF) leave FP64 partial sums in part[p - piece0], and k_sgd_fold adds
+// them in order for each such row (fold[g]).  A cold row adds the sum of its
+// runs' deltas, a hot row their mean.
 constexpr uint32_t kStagePiece = 256;
-struct StageArgs {
-  float *F;
-  const uint32_t *hot;  // the mode's hot-row bitmap
-  const float *stage;
-  const uint32_t *seg_row;
-  const uint32_t *seg_start;
-  const uint32_t *seg_piece;
-  const uint32_t *piece_start;
-  const uint32_t *slots;
-  double *part;
-  uint64_t nseg, npiece, piece0;
+struct FoldSeg {
+  uint32_t row, piece, npiece, runs;  // piece: first piece, sub-step relative
 };
-
-// k_sgd_merge: segment g covers merge items [off[g], off[g + 1]) of mode
+// Mode 0's merge: segment g covers merge items [off[g], off[g + 1]) of mode
 // mode[g].  A cold segment's item k is row rows[g][k] (rows[g] null: row k),
 // whose sum is in D; a hot segment's item k is hot row k of the mode, whose
 // sums are spread over the replicas of Dh / ch.
@@ -100,6 +92,22 @@ struct MergeArgs {
   int is_hot[8];
   uint64_t off[9];
   int nseg;
+};
+
+struct SettleArgs {
+  MergeArgs mg;        // mode 0's long and hot rows
+  uint64_t nb_merge;   // k_sgd_fold's first blocks, which take mg's items
+  float *F[4];
+  const uint32_t *hot[4];  // hot-row bitmaps
+  const float *stage[4];
+  const uint32_t *piece_start[4];
+  const uint32_t *piece_row[4];
+  const uint32_t *slots[4];
+  double *part[4];
+  const FoldSeg *fold[4];  // the sub-step's multi-piece rows
+  uint64_t piece0[4];      // the sub-step's first piece
+  uint64_t pb[5];          // mode k's pieces: k_sgd_settle's blocks pb[k] .. pb[k + 1]
+  uint64_t fb[5];          // mode k's multi-piece rows: k_sgd_fold's walkers fb[k] .. fb[k + 1] after the merge
 };
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }

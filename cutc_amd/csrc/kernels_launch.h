@@ -10,13 +10,14 @@ namespace ctc {
 // code = PA * 16 + PB * 4 + PC with P in {0 racy, 1 atomic, 2 run, 3 hot-row bitmap} (see k_sgd)
 hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, const HotRows &hot, int nmodes, int R,
                       uint32_t span, uint64_t stride, int code, float step, float lam, unsigned block);
-// one sub-step s of the bulk-synchronous SGD schedule (k_sgd_sync), then the
-// merge of the rows' handed-in deltas (k_sgd_merge)
+// one sub-step s of the bulk-synchronous SGD schedule (k_sgd_sync); then
+// launch_sgd_settle merges the rows' handed-in deltas (k_sgd_settle, and
+// k_sgd_fold for the rows of more than one piece)
 hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SyncArgs &sy, int nmodes,
                            int R, uint32_t span, uint32_t sub, uint32_t s, uint32_t long_min, float step, float lam,
                            unsigned block);
-hipError_t launch_sgd_merge(hipStream_t st, const MergeArgs &a, int R);
-hipError_t launch_sgd_stage_merge(hipStream_t st, const StageArgs &a, int R);
+// nb_merge is set here; pb / fb must be prefix sums with pb[0..1] = fb[0..1] = 0
+hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R);
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);

@@ -415,13 +415,12 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->hotbits[md]);
     dev_free(m->sgd_D[md]);
     dev_free(m->sgd_stage[md]);
-    dev_free(m->sgd_seg_row[md]);
-    dev_free(m->sgd_seg_start[md]);
     dev_free(m->sgd_slots[md]);
     dev_free(m->sgd_sbase[md]);
-    dev_free(m->sgd_seg_piece[md]);
     dev_free(m->sgd_piece_start[md]);
+    dev_free(m->sgd_piece_row[md]);
     dev_free(m->sgd_part[md]);
+    dev_free(m->sgd_fold[md]);
     dev_free(m->sgd_hidx[md]);
     dev_free(m->sgd_hrow[md]);
     dev_free(m->sgd_Dh[md]);
@@ -693,14 +692,14 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
   // in slot (= walk) order for the merge.
   const uint64_t nnz = d.nnz, nspans = (nnz + span - 1) / span, nsub = (span + sub - 1) / sub;
   for (uint32_t k = 1; k < m->nmodes; ++k) {
-    for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_seg_row[k], (void *)m->sgd_seg_start[k],
-                    (void *)m->sgd_slots[k], (void *)m->sgd_sbase[k], (void *)m->sgd_seg_piece[k],
-                    (void *)m->sgd_piece_start[k], (void *)m->sgd_part[k]})
+    for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_slots[k], (void *)m->sgd_sbase[k],
+                    (void *)m->sgd_piece_start[k], (void *)m->sgd_piece_row[k], (void *)m->sgd_part[k],
+                    (void *)m->sgd_fold[k]})
       dev_free(p);
     m->sgd_stage[k] = nullptr;
     m->sgd_part[k] = nullptr;
-    m->sgd_seg_row[k] = m->sgd_seg_start[k] = m->sgd_slots[k] = m->sgd_sbase[k] = nullptr;
-    m->sgd_seg_piece[k] = m->sgd_piece_start[k] = nullptr;
+    m->sgd_fold[k] = nullptr;
+    m->sgd_slots[k] = m->sgd_sbase[k] = m->sgd_piece_start[k] = m->sgd_piece_row[k] = nullptr;
     std::vector<uint32_t> idx(nnz);
     CTC_HIP(hipMemcpy(idx.data(), d.oidx[k - 1], nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
     auto starts_run = [&](uint64_t p) { return (p % span) % sub == 0 || idx[p] != idx[p - 1]; };
@@ -738,38 +737,39 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
     parallel_for(nsub, host_threads(), [&](uint64_t b, uint64_t e, unsigned) {
       for (uint64_t s2 = b; s2 < e; ++s2) std::sort(key.begin() + boff[s2], key.begin() + boff[s2 + 1]);
     });
-    // segments (one per row of a sub-step) cut into pieces of at most
-    // kStagePiece slots
-    std::vector<uint32_t> seg_row, seg_start, seg_piece, piece_start, slots(key.size());
-    std::vector<uint64_t> &soff = m->sgd_seg_off[k], &poff = m->sgd_piece_off[k];
-    soff.assign(nsub + 1, 0);
+    // every row's slots of a sub-step cut into pieces of at most kStagePiece
+    // slots; a row of one piece is settled by it, the others are folded
+    std::vector<uint32_t> piece_start, piece_row, slots(key.size());
+    std::vector<FoldSeg> fold;
+    std::vector<uint64_t> &poff = m->sgd_piece_off[k], &foff = m->sgd_fold_off[k];
     poff.assign(nsub + 1, 0);
+    foff.assign(nsub + 1, 0);
     uint64_t max_pieces = 0;
     for (uint64_t s2 = 0; s2 < nsub; ++s2) {
-      soff[s2] = seg_row.size();
       poff[s2] = piece_start.size();
-      for (uint64_t q = boff[s2]; q < boff[s2 + 1]; ++q) {
+      foff[s2] = fold.size();
+      for (uint64_t q = boff[s2]; q < boff[s2 + 1];) {
         const uint32_t row = (uint32_t)(key[q] >> 32);
-        const bool new_seg = q == boff[s2] || row != (uint32_t)(key[q - 1] >> 32);
-        if (new_seg) {
-          seg_row.push_back(row);
-          seg_start.push_back((uint32_t)q);
-          seg_piece.push_back((uint32_t)piece_start.size());
+        uint64_t e = q + 1;
+        while (e < boff[s2 + 1] && (uint32_t)(key[e] >> 32) == row) ++e;
+        const uint64_t np = (e - q + kStagePiece - 1) / kStagePiece;
+        if (np > 1)
+          fold.push_back({row, (uint32_t)(piece_start.size() - poff[s2]), (uint32_t)np, (uint32_t)(e - q)});
+        for (uint64_t t = 0; t < np; ++t) {
+          piece_start.push_back((uint32_t)(q + t * kStagePiece));
+          piece_row.push_back(np == 1 ? row : 0xFFFFFFFFu);
         }
-        if ((q - seg_start.back()) % kStagePiece == 0) piece_start.push_back((uint32_t)q);
-        slots[q] = (uint32_t)key[q];
+        for (uint64_t t = q; t < e; ++t) slots[t] = (uint32_t)key[t];
+        q = e;
       }
       max_pieces = std::max<uint64_t>(max_pieces, piece_start.size() - poff[s2]);
     }
-    soff[nsub] = seg_row.size();
     poff[nsub] = piece_start.size();
-    seg_start.push_back((uint32_t)key.size());
-    seg_piece.push_back((uint32_t)piece_start.size());
+    foff[nsub] = fold.size();
     piece_start.push_back((uint32_t)key.size());
-    CTC_TRY(upload(&m->sgd_seg_row[k], seg_row.data(), seg_row.size(), st));
-    CTC_TRY(upload(&m->sgd_seg_start[k], seg_start.data(), seg_start.size(), st));
-    CTC_TRY(upload(&m->sgd_seg_piece[k], seg_piece.data(), seg_piece.size(), st));
     CTC_TRY(upload(&m->sgd_piece_start[k], piece_start.data(), piece_start.size(), st));
+    CTC_TRY(upload(&m->sgd_piece_row[k], piece_row.data(), piece_row.size(), st));
+    CTC_TRY(upload(&m->sgd_fold[k], fold.data(), fold.size(), st));
     CTC_TRY(upload(&m->sgd_slots[k], slots.data(), slots.size(), st));
     CTC_TRY(upload(&m->sgd_sbase[k], base.data(), base.size(), st));
     CTC_TRY(dev_alloc(&m->sgd_stage[k], (size_t)std::max<uint64_t>(1, max_runs) * m->rank));
@@ -794,8 +794,10 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
 }
 
 // One epoch of the bulk-synchronous schedule: per sub-step, k_sgd_sync over
-// every walker, then k_sgd_merge over mode 0's long rows and every row of the
-// other modes.  Enqueued only (no synchronisation).
+// every walker, then k_sgd_settle over the pieces of the other modes' rows and
+// k_sgd_fold over mode 0's long rows and the other modes' rows of several
+// pieces.
+// Enqueued only (no synchronisation).
 static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) {
   uint32_t span = 0, sub = 0;
   CTC_TRY(sync_shape(o, &span, &sub));
@@ -803,7 +805,8 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
   CTC_TRY(ensure_sync(m, span, sub));
   const uint32_t N = m->nmodes, R = m->rank;
   SyncArgs sy{};
-  MergeArgs mg{};
+  SettleArgs sa{};
+  MergeArgs &mg = sa.mg;
   mg.nseg = 0;
   mg.off[0] = 0;
   for (uint32_t k = 0; k < N; ++k) {
@@ -818,6 +821,13 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
     mg.Dh[k] = m->sgd_Dh[k];
     mg.ch[k] = m->sgd_ch[k];
     mg.hrow[k] = m->sgd_hrow[k];
+    sa.F[k] = m->F[k];
+    sa.hot[k] = m->hotbits[k];
+    sa.stage[k] = m->sgd_stage[k];
+    sa.piece_start[k] = m->sgd_piece_start[k];
+    sa.piece_row[k] = m->sgd_piece_row[k];
+    sa.slots[k] = m->sgd_slots[k];
+    sa.part[k] = m->sgd_part[k];
     if (k > 0) continue;  // modes 1.. are merged from their staging slots
     // mode 0: its long rows' sums, then its hot rows' replicated sums
     mg.mode[mg.nseg] = 0;
@@ -847,23 +857,14 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
     }
     {
       Prof p(m->ctx, "sgd_merge");
-      CTC_HIP(launch_sgd_merge(st, mg, (int)R));
-      for (uint32_t k = 1; k < N; ++k) {
-        StageArgs sa{};
-        sa.F = m->F[k];
-        sa.hot = m->hotbits[k];
-        sa.stage = m->sgd_stage[k];
-        sa.seg_row = m->sgd_seg_row[k] + m->sgd_seg_off[k][s];
-        sa.seg_start = m->sgd_seg_start[k] + m->sgd_seg_off[k][s];
-        sa.seg_piece = m->sgd_seg_piece[k] + m->sgd_seg_off[k][s];
-        sa.piece_start = m->sgd_piece_start[k];
-        sa.slots = m->sgd_slots[k];
-        sa.part = m->sgd_part[k];
-        sa.nseg = m->sgd_seg_off[k][s + 1] - m->sgd_seg_off[k][s];
-        sa.piece0 = m->sgd_piece_off[k][s];
-        sa.npiece = m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s];
-        CTC_HIP(launch_sgd_stage_merge(st, sa, (int)R));
+      for (uint32_t k = 0; k < 4; ++k) {
+        const bool staged = k >= 1 && k < N;
+        sa.piece0[k] = staged ? m->sgd_piece_off[k][s] : 0;
+        sa.fold[k] = staged ? m->sgd_fold[k] + m->sgd_fold_off[k][s] : nullptr;
+        sa.pb[k + 1] = sa.pb[k] + (staged ? m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s] : 0);
+        sa.fb[k + 1] = sa.fb[k] + (staged ? m->sgd_fold_off[k][s + 1] - m->sgd_fold_off[k][s] : 0);
       }
+      CTC_HIP(launch_sgd_settle(st, sa, (int)R));
     }
   }
   m->resid_valid = false;
