@@ -478,10 +478,12 @@ __device__ inline void sgd_merge_block(const MergeArgs &a, uint64_t blk) {
 // reproducible.
 template <int R>
 __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
-  constexpr int W = R < 64 ? R : 64;
-  constexpr int CPL = R / W;
+  constexpr int CPL = SyncShape<R>::CPL;  // a slot's row moves as 16-byte pieces
+  constexpr int W = SyncShape<R>::W;
   constexpr int NW = 256 / W;  // walkers per workgroup
+  constexpr int G = 256 / R;   // first-level groups of the reduction
   __shared__ double part[NW][R];
+  __shared__ double red[G][R];
   const uint64_t pblk = blockIdx.x;
   if (pblk >= a.pb[4]) return;
   int k = 1;
@@ -507,32 +509,40 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   for (; p + 3 * NW < e; p += 4 * NW) {  // four independent slots in flight
     float x[4][CPL];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float *src = stage + (size_t)slots[p + u * NW] * R;
-#pragma unroll
-      for (int j = 0; j < CPL; ++j) x[u][j] = src[wl + j * W];
-    }
+    for (int u = 0; u < 4; ++u) load_row4<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int j = 0; j < CPL; ++j) acc[j] += (double)x[u][j];
   }
   for (; p < e; p += NW) {
-    const float *src = stage + (size_t)slots[p] * R;
+    float x[CPL];
+    load_row4<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] += (double)src[wl + j * W];
+    for (int j = 0; j < CPL; ++j) acc[j] += (double)x[j];
   }
 #pragma unroll
-  for (int j = 0; j < CPL; ++j) part[wid][wl + j * W] = acc[j];
+  for (int j = 0; j < CPL; ++j) part[wid][wl * CPL + j] = acc[j];
+  __syncthreads();
+  // walker partials in a fixed order: group g sums walkers g, g + G, ... of
+  // those that took a slot, then the groups in order
+  const uint32_t used = e - b < (uint32_t)NW ? e - b : (uint32_t)NW;
+  {
+    const int col = threadIdx.x % R, g = threadIdx.x / R;
+    double t = 0.0;
+    for (uint32_t w = g; w < used; w += G) t += part[w][col];
+    red[g][col] = t;
+  }
   __syncthreads();
   if (threadIdx.x < R) {
     double t = 0.0;
-    for (int w = 0; w < NW; ++w) t += part[w][threadIdx.x];
+#pragma unroll
+    for (int g = 0; g < G; ++g) t += red[g][threadIdx.x];
     if (row == 0xFFFFFFFFu) {
       a.part[k][(gp - a.piece0[k]) * R + threadIdx.x] = t;
     } else {
       const double sc = ((hw >> (row & 31)) & 1u) ? 1.0 / (double)(e - b) : 1.0;
-      a.F[k][(size_t)row * R + threadIdx.x] = (float)((double)fv + (0.0 + t) * sc);
+      a.F[k][(size_t)row * R + threadIdx.x] = (float)((double)fv + t * sc);
     }
   }
 }
