@@ -265,6 +265,28 @@ __device__ inline void store_row4(float *p, const float (&v)[CPL]) {
   for (int j = 0; j < CPL; j += 4) *reinterpret_cast<float4 *>(p + j) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
 }
 
+// Staged deltas are written once and read once by the next launch: streamed
+// past the caches.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int CPL>
+__device__ inline void store_row4_nt(float *p, const float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) {
+    f32x4 t = {v[j], v[j + 1], v[j + 2], v[j + 3]};
+    __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(p + j));
+  }
+}
+
+template <int CPL>
+__device__ inline void load_row4_nt(const float *p, float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) {
+    const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p + j));
+    v[j] = t.x, v[j + 1] = t.y, v[j + 2] = t.z, v[j + 3] = t.w;
+  }
+}
+
 template <int R, int N>
 __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, SyncArgs sy, uint32_t span,
                                                   uint32_t sub, uint32_t s, uint64_t nspans, uint32_t long_min,
@@ -306,7 +328,7 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
     rslot[k] = (k >= 1 && walker < nspans) ? sy.sbase[k][(size_t)s * nspans + walker] : 0u;
   auto hand_in = [&](int q) {
     if (q >= 1) {  // the run's delta goes to its staging slot
-      store_row4<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
+      store_row4_nt<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
       return;
     }
     if (own0) {
@@ -509,7 +531,7 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   for (; p + 3 * NW < e; p += 4 * NW) {  // four independent slots in flight
     float x[4][CPL];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) load_row4<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+    for (int u = 0; u < 4; ++u) load_row4_nt<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -517,7 +539,7 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   }
   for (; p < e; p += NW) {
     float x[CPL];
-    load_row4<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
+    load_row4_nt<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
 #pragma unroll
     for (int j = 0; j < CPL; ++j) acc[j] += (double)x[j];
   }
