@@ -66,8 +66,12 @@ void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b, hipS
   *a = *b = nullptr;
   if (!c->profiling) return;
   if (c->event_pool.size() < 2) {
+    // timing only: no system-scope fence (cache write-back and invalidation)
+    // between the launches they bracket
     hipEvent_t e1, e2;
-    if (hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess) return;
+    if (hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&e2, hipEventDisableSystemFence) != hipSuccess)
+      return;
     c->event_pool.push_back(e1);
     c->event_pool.push_back(e2);
   }
