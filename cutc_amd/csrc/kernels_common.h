@@ -73,7 +73,7 @@ struct SyncArgs {
 // 0xFFFFFFFF) leave FP64 partial sums in part[p - piece0], and k_sgd_fold adds
 // them in order for each such row (fold[g]).  A cold row adds the sum of its
 // runs' deltas, a hot row their mean.
-constexpr uint32_t kStagePiece = 256;
+constexpr uint32_t kStagePiece = 1024;
 struct FoldSeg {
   uint32_t row, piece, npiece, runs;  // piece: first piece, sub-step relative
 };
