@@ -64,13 +64,13 @@ def cfg2_tensor(ct, nnz, seed=0x5EED0002):
                     noise_std=0.3, test_fraction=0.01, seed=seed)
 
 
-def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0):
+def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None):
     """SGD epochs with the default schedule; -> (ms list, trajectory, walk ms per launch, walk launches per epoch,
     options)."""
     import numpy as np  # noqa: F401
     model.set_factors(F0)
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3, lr_schedule=ct.LR_EXP_DECAY,
-                     lr_decay=0.95, sgd_scale=1.0)
+                     lr_decay=0.95, sgd_scale=1.0, precision=ct.PREC_FP32 if precision is None else precision)
     t0 = time.time()
     model.prepare(opt)
     model.sgd_epoch(opt, lr=0.0)  # builds the schedule's tables (lr 0: factors unchanged)
@@ -91,6 +91,41 @@ def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0):
     n_w, ms_w = ctx.kernel_stats("sgd_walk")
     n_m, ms_m = ctx.kernel_stats("sgd_merge")
     return times, traj, ms_w / max(n_w, 1), n_w / max(steps, 1), ms_m / max(steps, 1), opt
+
+
+def sgd_mixed_leg(ct, ctx, model, nnz, R, lam, F0, steps, warmup, fp32, full):
+    """The mixed-precision SGD path (run deltas of modes 1.. staged in BF16) on
+    the benchmark tensor: s/epoch against the FP32 path's, and the per-epoch
+    test RMSE against the FP64 oracle of the FP32 schedule and against the
+    oracle's mixed variant (tests/golden/sgd_cfg2_sync_oracle.npz,
+    sgd_cfg2_sync_mixed_oracle.npz)."""
+    import numpy as np
+    times, traj, walk_ms, walks, merge_ms, opt = sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0,
+                                                         precision=ct.PREC_MIXED)
+    s_ep = sum(times) / steps
+    res = {"precision": "run deltas of modes 1.. staged in BF16; FP32 factors and arithmetic, FP64 sums",
+           "s_per_epoch": s_ep, "nnz_per_s": nnz / s_ep, "speedup_vs_fp32": fp32["s_per_epoch"] / s_ep,
+           "walk_kernel_ms_avg": walk_ms, "merge_ms_per_epoch": merge_ms,
+           "rmse_te_per_epoch": [round(x, 6) for x in traj]}
+    c = opt.to_c()
+    for key, name in (("gap_vs_fp64_oracle", "sgd_cfg2_sync_oracle.npz"),
+                      ("gap_vs_mixed_oracle", "sgd_cfg2_sync_mixed_oracle.npz")):
+        fx = os.path.join(ROOT, "tests", "golden", name)
+        res[key] = None
+        if full and R == 32 and os.path.exists(fx):
+            g = np.load(fx, allow_pickle=False)
+            if (int(g["span"]), int(g["sub"]), int(g["hot"])) == (c.tile_len, c.sgd_substep, c.sgd_hot_rows):
+                n = min(len(traj), len(g["rmse_te"]))
+                res[key] = float(np.max(np.abs(np.array(traj[:n]) - g["rmse_te"][:n])))
+    res["rmse_gap_ok"] = res["gap_vs_fp64_oracle"] is not None and res["gap_vs_fp64_oracle"] < 1e-3
+    # the walk's algorithmic bytes per launch: the FP32 figure less the BF16 halves of the staged deltas
+    # would understate what the kernel reads, so the same 12 + 24R per nonzero is used
+    per_launch = (12 + 24 * R) * nnz / walks
+    res["walk_achieved_GBs"] = per_launch / (walk_ms * 1e-3) / 1e9
+    log(f"SGD mixed: {s_ep * 1e3:.2f} ms/epoch ({res['speedup_vs_fp32']:.2f}x the FP32 path), walk {walk_ms:.3f} ms, "
+        f"merges {merge_ms:.2f} ms; gap vs FP64 oracle {res['gap_vs_fp64_oracle']}, vs mixed oracle "
+        f"{res['gap_vs_mixed_oracle']}")
+    return res
 
 
 # ---------------------------------------------------------------- config 1
@@ -553,7 +588,7 @@ def main():
     ap.add_argument("--als-epochs", type=int, default=3)
     ap.add_argument("--tile-len", type=int, default=0, help="0 = per-algorithm default (CCD++ / ALS legs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--algs", default="share,cfg1,mttkrp,ccd,als,batch,adjoint,mixed",
+    ap.add_argument("--algs", default="sgd_mixed,share,cfg1,mttkrp,ccd,als,batch,adjoint,mixed",
                     help="legs after the SGD headline")
     ap.add_argument("--budget", type=float, default=560.0,
                     help="seconds; a leg that would start later is skipped and recorded as such")
@@ -686,6 +721,9 @@ def main():
             log(f"{name}: failed\n{traceback.format_exc()}")
         gc.collect()
 
+    if "sgd_mixed" in algs:
+        run_leg("sgd_mixed", lambda: sgd_mixed_leg(ct, ctx, model, train.nnz, R, lam, F0, a.steps, a.warmup,
+                                                   out_algs["sgd"], a.nnz == 100_000_000))
     if "mttkrp" in algs:
         run_leg("mttkrp", lambda: mttkrp_leg(ct, ctx, model, train, R, a.no_cpu))
     if "ccd" in algs:

@@ -14,7 +14,8 @@
 //   --lr-schedule bold|exp|const  --lr-decay G   --inner I (NUM_INNER, :30)
 //   --autotune (replaces the GA of software/autotuning)      --json (summary line)
 //   --dump-factors PREFIX (writes PREFIX<mode>.txt)
-//   --precision fp32|mixed (BF16-operand ALS)  --refine K   --sgd-update P  --sgd-hot-rows N
+//   --precision fp32|mixed (BF16-operand ALS; SGD with BF16-staged run deltas)
+//   --refine K   --sgd-update P  --sgd-hot-rows N
 //   --sgd-substep N (nonzeros per walker per sub-step of the default SGD schedule)
 //
 //   ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]  (data preparation)
@@ -136,13 +137,22 @@ int main(int argc, char **argv) {
     else if (k == "tile-len") o.tile_len = (uint32_t)atoi(v);
     else if (k == "block") o.block = (uint32_t)atoi(v);
     else if (k == "seed") o.seed = strtoull(v, nullptr, 0);
-    else if (k == "mode-order") o.mode_order = strcmp(v, "rotate") == 0 ? CTC_ORDER_ROTATE : CTC_ORDER_FIXED;
-    else if (k == "lr-schedule")
-      o.lr_schedule = strcmp(v, "exp") == 0 ? CTC_LR_EXP_DECAY : strcmp(v, "const") == 0 ? CTC_LR_CONSTANT
-                                                                                          : CTC_LR_BOLD_DRIVER;
-    else if (k == "lr-decay") o.lr_decay = atof(v);
+    else if (k == "mode-order" || k == "lr-schedule" || k == "precision") {
+      static const struct { const char *flag, *name; int32_t v; } names[] = {
+          {"mode-order", "fixed", CTC_ORDER_FIXED}, {"mode-order", "rotate", CTC_ORDER_ROTATE},
+          {"lr-schedule", "bold", CTC_LR_BOLD_DRIVER}, {"lr-schedule", "exp", CTC_LR_EXP_DECAY},
+          {"lr-schedule", "const", CTC_LR_CONSTANT}, {"precision", "fp32", CTC_PREC_FP32},
+          {"precision", "mixed", CTC_PREC_MIXED}};
+      int found = -1;
+      for (const auto &q : names)
+        if (k == q.flag && !strcmp(v, q.name)) found = q.v;
+      if (found < 0) {
+        fprintf(stderr, "ctc: unknown --%s %s\n", k.c_str(), v);
+        return 1;
+      }
+      (k == "mode-order" ? o.mode_order : k == "lr-schedule" ? o.lr_schedule : o.precision) = found;
+    } else if (k == "lr-decay") o.lr_decay = atof(v);
     else if (k == "inner") o.inner_iters = (uint32_t)atoi(v);
-    else if (k == "precision") o.precision = strcmp(v, "mixed") == 0 ? CTC_PREC_MIXED : CTC_PREC_FP32;
     else if (k == "refine") o.refine = atoi(v);
     else if (k == "sgd-update") {
       static const struct { const char *name; int32_t v; } pol[] = {

@@ -287,7 +287,32 @@ __device__ inline void load_row4_nt(const float *p, float (&v)[CPL]) {
   }
 }
 
-template <int R, int N>
+// Four BF16 entries per lane (8 bytes): the mixed path's staged deltas.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int CPL>
+__device__ inline void load_bf4_nt(const uint16_t *p, float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) {
+    const u32x2 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p + j));
+    v[j] = bf_lo(t.x), v[j + 1] = bf_hi(t.x), v[j + 2] = bf_lo(t.y), v[j + 3] = bf_hi(t.y);
+  }
+}
+
+template <int CPL>
+__device__ inline void store_bf4_nt(uint16_t *p, const float (&v)[CPL]) {
+#pragma unroll
+  for (int j = 0; j < CPL; j += 4) {
+    const u32x2 t = {(uint32_t)f2bf(v[j]) | ((uint32_t)f2bf(v[j + 1]) << 16),
+                     (uint32_t)f2bf(v[j + 2]) | ((uint32_t)f2bf(v[j + 3]) << 16)};
+    __builtin_nontemporal_store(t, reinterpret_cast<u32x2 *>(p + j));
+  }
+}
+
+// MIX (the mixed-precision path): the run deltas of modes 1.. are staged in
+// BF16 (round to nearest even), halving the staging traffic; the factors, the
+// walker's arithmetic and the sums of the deltas stay FP32 / FP64.
+template <int R, int N, bool MIX>
 __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, SyncArgs sy, uint32_t span,
                                                   uint32_t sub, uint32_t s, uint64_t nspans, uint32_t long_min,
                                                   float step, float lam) {
@@ -328,7 +353,10 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
     rslot[k] = (k >= 1 && walker < nspans) ? sy.sbase[k][(size_t)s * nspans + walker] : 0u;
   auto hand_in = [&](int q) {
     if (q >= 1) {  // the run's delta goes to its staging slot
-      store_row4_nt<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
+      if constexpr (MIX)
+        store_bf4_nt<CPL>(reinterpret_cast<uint16_t *>(sy.stage[q]) + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
+      else
+        store_row4_nt<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
       return;
     }
     if (own0) {
@@ -498,7 +526,7 @@ __device__ inline void sgd_merge_block(const MergeArgs &a, uint64_t blk) {
 // walker order.  A piece that holds a whole row applies it; the others leave
 // their sums for k_sgd_fold.  The order is fixed, so the result is
 // reproducible.
-template <int R>
+template <int R, bool MIX>
 __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   constexpr int CPL = SyncShape<R>::CPL;  // a slot's row moves as 16-byte pieces
   constexpr int W = SyncShape<R>::W;
@@ -531,7 +559,12 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   for (; p + 3 * NW < e; p += 4 * NW) {  // four independent slots in flight
     float x[4][CPL];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) load_row4_nt<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+    for (int u = 0; u < 4; ++u) {
+      if constexpr (MIX)
+        load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+      else
+        load_row4_nt<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -539,7 +572,10 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   }
   for (; p < e; p += NW) {
     float x[CPL];
-    load_row4_nt<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
+    if constexpr (MIX)
+      load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + (size_t)slots[p] * R + wl * CPL, x);
+    else
+      load_row4_nt<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
 #pragma unroll
     for (int j = 0; j < CPL; ++j) acc[j] += (double)x[j];
   }
@@ -1335,29 +1371,45 @@ hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, co
 
 hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SyncArgs &sy, int nmodes,
                            int R, uint32_t span, uint32_t sub, uint32_t s, uint32_t long_min, float step, float lam,
-                           unsigned block) {
+                           unsigned block, bool mixed) {
   if (c.nnz == 0) return hipSuccess;
   const uint64_t nspans = (c.nnz + span - 1) / span;
   const int wpw = 256 / R;  // walkers per wave (SyncShape)
   const uint64_t waves = (nspans + wpw - 1) / wpw;
   const unsigned grid = grid_for(waves, block);
+#define CTC_SYNC_LAUNCH(NN, MM) \
+  CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, NN, MM>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam))
   if (nmodes == 3) {
-    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, 3>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam));
+    if (mixed) {
+      CTC_SYNC_LAUNCH(3, true);
+    } else {
+      CTC_SYNC_LAUNCH(3, false);
+    }
   } else if (nmodes == 4) {
-    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_sync<RR, 4>), dim3(grid), dim3(block), 0, st, c, f, sy, span, sub, s, nspans, long_min, step, lam));
+    if (mixed) {
+      CTC_SYNC_LAUNCH(4, true);
+    } else {
+      CTC_SYNC_LAUNCH(4, false);
+    }
   } else {
     return hipErrorInvalidValue;
   }
+#undef CTC_SYNC_LAUNCH
   return hipGetLastError();
 }
 
-hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R) {
+hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R, bool mixed) {
   const int W = R < 64 ? R : 64;
   const uint64_t items = a.mg.off[a.mg.nseg];
   a.nb_merge = grid_for((items + (64 / W) - 1) / (64 / W), 256);
   if (a.pb[4] > 0x7FFFFFFFull) return hipErrorInvalidValue;
-  if (a.pb[4] > 0)
-    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+  if (a.pb[4] > 0) {
+    if (mixed) {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, true>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+    } else {
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, false>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+    }
+  }
   const uint64_t fg = a.nb_merge + grid_for((a.fb[4] + (64 / W) - 1) / (64 / W), 256);
   if (fg > 0) CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_fold<RR>), dim3((unsigned)fg), dim3(256), 0, st, a));
   return hipGetLastError();

@@ -60,7 +60,7 @@ struct SyncArgs {
   const uint32_t *hidx[4];
   long long *Dh[4];   // hot rows x kSyncReplicas x R
   uint32_t *ch[4];    // hot rows x kSyncReplicas
-  float *stage[4];    // modes 1..: runs of one sub-step x R
+  float *stage[4];    // modes 1..: runs of one sub-step x R (BF16 entries on the mixed path)
   const uint32_t *sbase[4];  // modes 1..: first slot of each (sub-step, walker) chunk
 };
 
@@ -111,6 +111,15 @@ struct SettleArgs {
 };
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+
+// FP32 -> BF16, round to nearest even (finite inputs); BF16 halves of a word
+__device__ inline uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ inline float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 
 // 1/x to full FP64 precision: hardware reciprocal estimate + two Newton steps
 __device__ inline double rcp_f64(double x) {

@@ -15,9 +15,9 @@ hipError_t launch_sgd(hipStream_t st, const CissArgs &c, const FactorArgs &f, co
 // k_sgd_fold for the rows of more than one piece)
 hipError_t launch_sgd_sync(hipStream_t st, const CissArgs &c, const FactorArgs &f, const SyncArgs &sy, int nmodes,
                            int R, uint32_t span, uint32_t sub, uint32_t s, uint32_t long_min, float step, float lam,
-                           unsigned block);
+                           unsigned block, bool mixed);
 // nb_merge is set here; pb / fb must be prefix sums with pb[0..1] = fb[0..1] = 0
-hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R);
+hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R, bool mixed);
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);

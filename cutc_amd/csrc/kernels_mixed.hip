@@ -33,13 +33,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ inline float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
-__device__ inline float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 
 // v_mfma_f32_32x32x16_bf16 accumulator v of lane l: row 8(v/4) + 4(l/32) + v%4, column l%32
 __device__ inline int row32(int v, int l) { return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3); }

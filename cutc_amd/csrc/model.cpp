@@ -808,6 +808,8 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
   CTC_TRY(ensure_hotbits(m, o->sgd_hot_rows ? o->sgd_hot_rows : kSgdHotRows));
   CTC_TRY(ensure_sync(m, span, sub));
   const uint32_t N = m->nmodes, R = m->rank;
+  const bool mixed = o->precision == CTC_PREC_MIXED;  // run deltas of modes 1.. staged in BF16
+  hipStream_t st = m->ctx->stream;
   SyncArgs sy{};
   SettleArgs sa{};
   MergeArgs &mg = sa.mg;
@@ -851,13 +853,12 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
   const CissArgs c = ciss_args(m->ciss[0]);
   const FactorArgs f = rel_factors(m->F, N, 0);
   const uint32_t nsub = (span + sub - 1) / sub;
-  hipStream_t st = m->ctx->stream;
   Prof pe(m->ctx, "sgd");
   for (uint32_t s = 0; s < nsub; ++s) {
     {
       Prof p(m->ctx, "sgd_walk");
       CTC_HIP(launch_sgd_sync(st, c, f, sy, (int)N, (int)R, span, sub, s, span - sub + 2, step, (float)o->lambda,
-                              block_of(o)));
+                              block_of(o), mixed));
     }
     {
       Prof p(m->ctx, "sgd_merge");
@@ -868,7 +869,7 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
         sa.pb[k + 1] = sa.pb[k] + (staged ? m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s] : 0);
         sa.fb[k + 1] = sa.fb[k] + (staged ? m->sgd_fold_off[k][s + 1] - m->sgd_fold_off[k][s] : 0);
       }
-      CTC_HIP(launch_sgd_settle(st, sa, (int)R));
+      CTC_HIP(launch_sgd_settle(st, sa, (int)R, mixed));
     }
   }
   m->resid_valid = false;
@@ -878,6 +879,9 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   CTC_TRY(ensure_ciss(m, 0));
   if (o->sgd_update == CTC_SGD_SYNC) return sgd_epoch_sync(m, o, lr);
+  if (o->precision == CTC_PREC_MIXED)
+    return fail(CTC_ERR_INVALID_ARG, "mixed-precision SGD runs the bulk-synchronous schedule (sgd_update %d)",
+                (int)CTC_SGD_SYNC);
   // policy codes of k_sgd: PA * 64 + PB * 8 + PC with 0 racy, 1 atomic, 3 hot
   // bitmap (racy / atomic), 4 hot bitmap (racy / run-accumulated atomic)
   static const int code_of[5] = {0 * 64 + 4 * 8 + 4, 0 * 64 + 1 * 8 + 1, 0, 0 * 64 + 0 * 8 + 1, 0 * 64 + 3 * 8 + 3};

@@ -188,7 +188,10 @@ typedef enum ctc_lr_schedule {
 } ctc_lr_schedule;
 typedef enum ctc_precision {
   CTC_PREC_FP32 = 0,  /* FP32 factors and arithmetic; FP64 solve and RMSE */
-  CTC_PREC_MIXED = 1  /* BF16 MFMA Gram/MTTKRP, FP32 accumulate; FP64 Cholesky + 1 refinement */
+  CTC_PREC_MIXED = 1  /* ALS: BF16 MFMA Gram/MTTKRP, FP32 accumulate; FP64 Cholesky + 1 refinement.
+                         SGD (CTC_SGD_SYNC only): the run deltas of modes 1.. are handed in
+                         (staged) in BF16, halving the schedule's dominant traffic; factors,
+                         the walkers' arithmetic and the sums of deltas stay FP32 / FP64 */
 } ctc_precision;
 typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mode_order;
 /* How concurrent SGD walkers combine updates of a factor row they share.  A
@@ -226,8 +229,10 @@ typedef enum ctc_sgd_update {
                              deltas (the reference's atomicAdd, SG/sgd.cu:96-98), or their
                              mean if it is hot (more than sgd_hot_rows nonzeros and more
                              than 4x its mode's mean; replica averaging,
-                             NC/sgd.cu:229-262,523-525).  Sums are exact (fixed point),
-                             so every run of a sweep gives the same bits. */
+                             NC/sgd.cu:229-262,523-525).  Every sum is taken in a fixed
+                             order (a row's runs in FP64 in walk order; a mode-0 row that
+                             several walkers share in 64-bit fixed point), so every run of
+                             a sweep gives the same bits. */
 } ctc_sgd_update;
 
 typedef struct ctc_options {

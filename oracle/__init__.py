@@ -306,18 +306,20 @@ def sgd_epoch(t: Coo, F, lr, lam, scale=1.0, order=None):
                             C.c_double(lam), C.c_double(scale))
 
 
-def sgd_sync_epoch(t: Coo, F, lr, lam, span, sub, hot_rows, scale=1.0, order=None, nthreads=1):
+def sgd_sync_epoch(t: Coo, F, lr, lam, span, sub, hot_rows, scale=1.0, order=None, nthreads=1, mixed=False):
     """One epoch of the bulk-synchronous walker schedule (libctc's default SGD):
     walkers of `span` positions of `order`, sub-steps of `sub` nonzeros from
     the sub-step's factors, per-run deltas summed on cold rows and averaged on
     hot rows (more than hot_rows and 4x the mode's mean nonzeros).  Restates
     SG/sgd.cu:21-143 (walker per tile, atomicAdd merge) with the replica
-    averaging of NC/sgd.cu:229-262,523-525 for hot rows."""
+    averaging of NC/sgd.cu:229-262,523-525 for hot rows.  mixed=True: the
+    mixed-precision variant, whose run deltas of modes 1.. are rounded
+    (through FP32) to BF16 before they are summed."""
     R = F[0].shape[1]
     op = _u32p(order) if order is not None else None
     _fn("orc_sgd_sync_epoch", F)(t.c(), op, C.c_uint32(R), _ptrs(F, _prec(F)), C.c_double(lr),
                                  C.c_double(lam), C.c_double(scale), C.c_uint64(span), C.c_uint64(sub),
-                                 C.c_uint64(hot_rows), C.c_int(nthreads))
+                                 C.c_uint64(hot_rows), C.c_int(1 if mixed else 0), C.c_int(nthreads))
 
 
 class Csf:
@@ -476,14 +478,14 @@ def sgd(t: Coo, F, lr, lam, epochs, test: Coo | None = None, scale=1.0, decay=No
 
 
 def sgd_sync(t: Coo, F, lr, lam, epochs, span, sub, hot_rows, test: Coo | None = None, scale=1.0, decay=None,
-             order=None, nthreads=1):
+             order=None, nthreads=1, mixed=False):
     """Epochs of sgd_sync_epoch in mode-0 order with an exponential lr decay."""
     if order is None:
         order, _ = mode_order(t, 0)
     rm, lrs = [], []
     for _ in range(epochs):
         lrs.append(lr)
-        sgd_sync_epoch(t, F, lr, lam, span, sub, hot_rows, scale, order=order, nthreads=nthreads)
+        sgd_sync_epoch(t, F, lr, lam, span, sub, hot_rows, scale, order=order, nthreads=nthreads, mixed=mixed)
         if test is not None:
             rm.append(rmse(test, F))
         if decay is not None:

@@ -120,11 +120,13 @@ void orc_csf_free(orc_csf *c);
   /* one SGD epoch of the bulk-synchronous walker schedule (libctc's default):      \
    * walkers of `span` positions of `order`, sub-steps of `sub` nonzeros from the      \
    * sub-step's factors, per-run deltas summed (cold rows) or averaged (rows with     \
-   * more than hot_rows and 4x the mean nonzeros); see oracle_impl.h */               \
+   * more than hot_rows and 4x the mean nonzeros); bf16 != 0: the mixed-precision   \
+   * variant (run deltas of modes 1.. rounded through FP32 to BF16 before they are    \
+   * summed); see oracle_impl.h */                                                    \
   void orc_sgd_sync_epoch##SFX(const orc_coo *t, const uint32_t *order, uint32_t R,    \
                                REAL **F, double lr, double lambda, double scale,       \
                                uint64_t span, uint64_t sub, uint64_t hot_rows,         \
-                               int nthreads);                                          \
+                               int bf16, int nthreads);                                \
   /* MTTKRP of the CSF's mode into out (dims[mode] x R, overwritten), the         \
    * reference's serial mttkrp_csf (SG/mttkrp.c:16-93, with the free of the       \
    * undeclared `sumbuffer` at :91 read as the free of `sum`): per level-1 fiber   \

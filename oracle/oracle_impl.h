@@ -216,7 +216,7 @@ void SFX(orc_sgd_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R, REA
  * then added in FP64 in thread order, a difference of FP64 rounding only). */
 void SFX(orc_sgd_sync_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R, REAL **F,
                              double lr, double lambda, double scale, uint64_t span, uint64_t sub,
-                             uint64_t hot_rows, int nthreads) {
+                             uint64_t hot_rows, int bf16, int nthreads) {
   const uint32_t N = t->nmodes;
   const uint64_t n = t->nnz;
   if (n == 0 || span == 0 || sub == 0 || R > 512) return;
@@ -257,10 +257,11 @@ void SFX(orc_sgd_sync_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R
           if (live[m]) { /* the run on cur[m] ends: hand in its delta */
             double *d = D[m] + (size_t)cur[m] * R;
             for (uint32_t f = 0; f < R; ++f) {
+              const double dv = bf16 && m > 0 ? (double)orc_bf16_round((float)del[m][f]) : (double)del[m][f];
 #ifdef _OPENMP
 #pragma omp atomic
 #endif
-              d[f] += (double)del[m][f];
+              d[f] += dv;
             }
 #ifdef _OPENMP
 #pragma omp atomic
@@ -299,10 +300,11 @@ void SFX(orc_sgd_sync_epoch)(const orc_coo *t, const uint32_t *order, uint32_t R
         if (!live[m]) continue;
         double *d = D[m] + (size_t)cur[m] * R;
         for (uint32_t f = 0; f < R; ++f) {
+          const double dv = bf16 && m > 0 ? (double)orc_bf16_round((float)del[m][f]) : (double)del[m][f];
 #ifdef _OPENMP
 #pragma omp atomic
 #endif
-          d[f] += (double)del[m][f];
+          d[f] += dv;
         }
 #ifdef _OPENMP
 #pragma omp atomic

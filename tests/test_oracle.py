@@ -537,6 +537,71 @@ def test_sgd_sync_threads_agree():
         np.testing.assert_allclose(g, f, rtol=1e-12, atol=1e-14)
 
 
+def _bf16(x):
+    """FP64 -> FP32 -> BF16 (round to nearest even) -> FP64, elementwise."""
+    u = np.asarray(x, dtype=np.float64).astype(np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_bf16_rounding_known_values():
+    """Round to nearest even at the BF16 half-way points, 8 significant bits."""
+    one_ulp = 2.0 ** -7  # BF16 spacing in [1, 2)
+    assert _bf16(1.0 + one_ulp / 2) == 1.0                   # tie -> even (1.0)
+    assert _bf16(1.0 + 3 * one_ulp / 2) == 1.0 + 2 * one_ulp  # tie -> even (1 + 2 ulp)
+    assert _bf16(1.0 + 0.6 * one_ulp) == 1.0 + one_ulp
+    assert _bf16(-3.0) == -3.0 and _bf16(0.0) == 0.0
+    assert O.bf16_round(1.0 + 3 * one_ulp / 2) == 1.0 + 2 * one_ulp  # the oracle's C rounding agrees
+
+
+def test_sgd_sync_mixed_rounding_points():
+    """The mixed-precision variant, one walker and one sub-step over two
+    nonzeros that share their mode-0 and mode-1 rows: rows are read unrounded;
+    each mode's run delta is accumulated unrounded along its run and, for modes
+    1 and 2, rounded to BF16 once, when the run is handed in (mode 1: one run
+    over both nonzeros; mode 2: two runs); mode 0's delta is never rounded.
+    Worked step by step with numpy's own rounding."""
+    rng = np.random.default_rng(47)
+    dims, R = (1, 1, 2), 4
+    t = O.Coo([[0, 0], [0, 0], [0, 1]], [2.5, -1.25], dims)
+    F = rand_factors(rng, dims, R)
+    F0 = [f.copy() for f in F]
+    lr, lam = 0.3, 0.05
+    # nonzero 0 from the sub-step's factors
+    v0, v1, v2a = F0[0][0].copy(), F0[1][0].copy(), F0[2][0].copy()
+    d = _sgd_step(2.5, [v0, v1, v2a], lr, lam)
+    v0, v1 = v0 + d[0], v1 + d[1]
+    del0, del1, del2a = d[0].copy(), d[1].copy(), d[2].copy()
+    # nonzero 1: the mode-2 run ends (row 0 handed in), row 1 read
+    v2b = F0[2][1].copy()
+    d = _sgd_step(-1.25, [v0, v1, v2b], lr, lam)
+    del0, del1, del2b = del0 + d[0], del1 + d[1], d[2]
+    O.sgd_sync_epoch(t, F, lr, lam, span=2, sub=2, hot_rows=NO_HOT, mixed=True)
+    np.testing.assert_allclose(F[0][0], F0[0][0] + del0, rtol=1e-14)
+    np.testing.assert_allclose(F[1][0], F0[1][0] + _bf16(del1), rtol=1e-14)
+    np.testing.assert_allclose(F[2][0], F0[2][0] + _bf16(del2a), rtol=1e-14)
+    np.testing.assert_allclose(F[2][1], F0[2][1] + _bf16(del2b), rtol=1e-14)
+    # the roundings matter in this case: the FP64 variant gives other values
+    G = [f.copy() for f in F0]
+    O.sgd_sync_epoch(t, G, lr, lam, span=2, sub=2, hot_rows=NO_HOT)
+    assert np.abs(G[1][0] - F[1][0]).max() > 1e-6
+
+
+def test_sgd_sync_mixed_is_fp64_on_bf16_exact_data():
+    """Factors of BF16 numbers and a step (a power of two) that keeps every
+    delta on the BF16 grid: the roundings change nothing, so the mixed variant
+    equals the FP64 one."""
+    dims, R = (3, 2, 2), 2
+    t = O.Coo([[0, 1, 2], [0, 1, 1], [1, 0, 1]], [1.0, 2.0, 0.5], dims)
+    F = [np.full((d, R), 0.5) for d in dims]
+    G = [f.copy() for f in F]
+    # lambda 0, unit values and rows of 0.5: err * others and step are powers of two
+    O.sgd_sync_epoch(t, F, 2.0 ** -4, 0.0, span=1, sub=1, hot_rows=NO_HOT, mixed=True)
+    O.sgd_sync_epoch(t, G, 2.0 ** -4, 0.0, span=1, sub=1, hot_rows=NO_HOT)
+    for f, g in zip(F, G):
+        np.testing.assert_array_equal(f, g)
+
+
 # --------------------------------------------------------------------------
 # CCD++
 # --------------------------------------------------------------------------
