@@ -781,7 +781,13 @@ def main():
         try:
             tj = json.load(open(tpath))
             if tj.get("nnz") == a.nnz and tj.get("rank") == R:
-                line["roofline"]["traffic"] = tj.get("hbm_bytes_per_launch")
+                # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs, over this run's
+                # HIP-event launch time: the counter-based rate beside the algorithmic one
+                tb = tj.get("hbm_bytes_per_launch")
+                line["roofline"]["traffic"] = tb
+                line["roofline"]["traffic_GBs"] = tb / (walk_ms * 1e-3) / 1e9
+                line["roofline"]["traffic_frac"] = tb / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                line["roofline"]["traffic_frac_of_copy"] = tb / (walk_ms * 1e-3) / 1e9 / copy_gbs
         except (OSError, ValueError):
             pass
     print(json.dumps(line), flush=True)
