@@ -15,6 +15,7 @@ TILE_LEN = 64
 ALS, SGD, CCD = 0, 1, 2
 LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
+QUIRK_TILE_HEAD, QUIRK_LAST_LANE, QUIRK_TRUNC_VALUES = 1, 2, 4  # ctc_ref_quirk
 ORDER_FIXED, ORDER_ROTATE = 0, 1
 SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY, SGD_HOT_ATOMIC, SGD_SYNC = 0, 1, 2, 3, 4, 5
 ALS_AUTO, ALS_SPANS, ALS_ROWS = 0, 1, 2
@@ -102,6 +103,7 @@ class Options(C.Structure):
         ("sgd_hot_rows", C.c_uint32),
         ("als_kernel", C.c_int32),
         ("sgd_substep", C.c_uint32),
+        ("ref_quirks", C.c_uint32),
     ]
 
 
@@ -197,6 +199,7 @@ SIGNATURES = {
                                P(C.c_double), P(C.c_double)]),
     "ctc_grad": (C.c_int, [vp, P(vp), P(vp), C.c_double, C.c_int, P(C.c_double), vp]),
     "ctc_grad_noalloc": (C.c_int, [vp, P(vp), P(vp), C.c_double, C.c_int, vp, vp]),
+    "ctc_ref_quirk_weights": (C.c_int, [vp, C.c_uint32, C.c_uint32, P(C.c_uint32)]),
     "ctc_tc_main_ciss": (C.c_int, [P(Coo), P(Coo), P(Coo), P(Coo), P(Coo), C.c_int, P(Options)]),
     "ctc_tc_als": (C.c_int, [P(Coo), P(Coo), P(Coo), P(Coo), P(Coo), P(P(Matrix)), P(P(Matrix)),
                              C.c_int, C.c_double, P(C.c_double), P(C.c_double), P(C.c_uint64),

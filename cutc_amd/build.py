@@ -16,7 +16,7 @@ CLI = os.path.join(HERE, "ctc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["host_coo.cpp", "host_ciss.cpp", "model.cpp", "ccd.cpp", "batch.cpp", "mixed.cpp", "kernels.hip", "kernels_ccd.hip", "kernels_mixed.hip", "grad_noalloc.hip"]
+SOURCES = ["host_coo.cpp", "host_ciss.cpp", "model.cpp", "ccd.cpp", "batch.cpp", "mixed.cpp", "kernels.hip", "kernels_ccd.hip", "kernels_mixed.hip", "grad_noalloc.hip", "quirks.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unknown-pragmas",
           # packed FP32 ops are disabled: they are avoided on this hardware pool

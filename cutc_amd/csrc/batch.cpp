@@ -347,6 +347,7 @@ extern "C" ctc_status ctc_batch_als(ctc_batch *b, const ctc_options *o, uint32_t
   if (!b || !o) return fail(CTC_ERR_INVALID_ARG, "ctc_batch_als");
   if (o->precision != CTC_PREC_FP32 && o->precision != CTC_PREC_MIXED)
     return fail(CTC_ERR_INVALID_ARG, "ctc_batch_als: unknown precision %d", (int)o->precision);
+  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ctc_batch_als: ref_quirks applies to single models");
   CTC_HIP(hipSetDevice(b->ctx->device));
   for (auto &c : b->cls) {
     const ctc_options t = class_options(c, o);

@@ -17,6 +17,7 @@
 //   --precision fp32|mixed (BF16-operand ALS; SGD with BF16-staged run deltas)
 //   --refine K   --sgd-update P  --sgd-hot-rows N
 //   --sgd-substep N (nonzeros per walker per sub-step of the default SGD schedule)
+//   --ref-quirks B (ALS: the reference's reduction defects, bits 1 Q1, 2 Q2, 4 Q5)
 //
 //   ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]  (data preparation)
 #include <cstdio>
@@ -35,7 +36,7 @@ static void usage() {
           "           [--seed S] [--mode-order fixed|rotate] [--lr-schedule bold|exp|const]\n"
           "           [--lr-decay G] [--inner I] [--precision fp32|mixed] [--refine K]\n"
           "           [--sgd-update sync|auto|atomic|racy|next-racy|hot-atomic] [--sgd-hot-rows N]\n"
-          "           [--sgd-substep N]\n"
+          "           [--sgd-substep N] [--ref-quirks B]\n"
           "           [--autotune] [--json] [--dump-factors PREFIX]\n");
 }
 
@@ -169,6 +170,7 @@ int main(int argc, char **argv) {
     } else if (k == "sgd-substep")
       o.sgd_substep = (uint32_t)atoi(v);
     else if (k == "sgd-hot-rows") o.sgd_hot_rows = (uint32_t)strtoul(v, nullptr, 0);
+    else if (k == "ref-quirks") o.ref_quirks = (uint32_t)strtoul(v, nullptr, 0);
     else if (k == "dump-factors") dump = v;
     else {
       fprintf(stderr, "ctc: unknown flag --%s\n", k.c_str());

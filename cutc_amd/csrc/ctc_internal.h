@@ -54,6 +54,8 @@ struct HostCiss {
   std::vector<uint64_t> tile_mask;
 };
 ctc_status build_host_ciss(const ctc_coo *t, uint32_t mode, HostCiss &out);
+// the nonzeros of t repeated by their ref_quirks weights for `mode` (quirks.cpp)
+ctc_status quirk_multiset(const ctc_coo *t, uint32_t mode, uint32_t flags, ctc_coo **out);
 
 // ----------------------------------------------------------- device CISS --
 // Device copy of one mode's CISS plus the per-span piece tables used by the
@@ -142,6 +144,10 @@ struct ctc_model {
   std::unique_ptr<ctc_coo, void (*)(ctc_coo *)> train{nullptr, ctc_coo_free};
   ctc::DevCiss ciss[CTC_MAX_MODES];
   bool ciss_ready[CTC_MAX_MODES] = {false, false, false, false};
+  // ref_quirks: each mode's CISS of the nonzeros repeated by their weights,
+  // and the quirk bits it was built for (0: not built)
+  ctc::DevCiss qciss[CTC_MAX_MODES];
+  uint32_t qciss_flags[CTC_MAX_MODES] = {0, 0, 0, 0};
   ctc::DevCoo valid, test;
   float *F[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   float *best[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};

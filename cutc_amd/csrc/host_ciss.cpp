@@ -16,6 +16,24 @@
 
 namespace ctc {
 
+// Stable LSD counting sort: least significant mode first, the CISS mode last,
+// giving lexicographic order over (mode, mode+1, ..., mode+N-1).  The
+// reference leaves sorting to its caller (SG/ciss.c:74, quirk Q12).
+void mode_order_perm(const ctc_coo *t, uint32_t mode, std::vector<uint32_t> &order) {
+  const uint32_t N = t->nmodes;
+  const uint64_t nnz = t->nnz;
+  std::vector<uint32_t> tmp;
+  order.resize(nnz);
+  unsigned T = host_threads();
+  parallel_for(nnz, T, [&](uint64_t lo, uint64_t hi, unsigned) {
+    for (uint64_t p = lo; p < hi; ++p) order[p] = (uint32_t)p;
+  });
+  for (int s = (int)N - 1; s >= 0; --s) {
+    uint32_t m = (mode + (uint32_t)s) % N;
+    counting_sort(t->ind[m], t->dims[m], order, tmp);
+  }
+}
+
 ctc_status build_host_ciss(const ctc_coo *t, uint32_t mode, HostCiss &c) {
   if (!t || mode >= t->nmodes) return fail(CTC_ERR_INVALID_ARG, "ciss: bad mode %u", mode);
   if (t->nnz == 0) return fail(CTC_ERR_INVALID_ARG, "ciss: empty tensor");
@@ -29,21 +47,10 @@ ctc_status build_host_ciss(const ctc_coo *t, uint32_t mode, HostCiss &c) {
     if (t->dims[m] == 0 || t->dims[m] > 0xFFFFFFFFull)
       return fail(CTC_ERR_INVALID_ARG, "ciss: dims[%u] out of range", m);
   }
-  // Stable LSD counting sort: least significant mode first, the CISS mode
-  // last, giving lexicographic order over (mode, mode+1, ..., mode+N-1).
-  // The reference leaves sorting to its caller (SG/ciss.c:74, quirk Q12).
-  std::vector<uint32_t> order(nnz), tmp;
-  unsigned T = host_threads();
-  parallel_for(nnz, T, [&](uint64_t lo, uint64_t hi, unsigned) {
-    for (uint64_t p = lo; p < hi; ++p) order[p] = (uint32_t)p;
-  });
-  for (int s = (int)N - 1; s >= 0; --s) {
-    uint32_t m = (mode + (uint32_t)s) % N;
-    counting_sort(t->ind[m], t->dims[m], order, tmp);
-  }
-  tmp.clear();
-  tmp.shrink_to_fit();
+  std::vector<uint32_t> order;
+  mode_order_perm(t, mode, order);
   c.perm.swap(order);
+  unsigned T = host_threads();
 
   // streams
   for (uint32_t s = 0; s + 1 < N; ++s) c.oidx[s].resize(nnz);

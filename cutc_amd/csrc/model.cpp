@@ -119,12 +119,7 @@ ctc_status ensure_scratch(ctc_model *m, size_t bytes) {
 }
 
 // ------------------------------------------------------------ CISS upload --
-ctc_status ensure_ciss(ctc_model *m, uint32_t mode) {
-  if (mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "mode %u out of range", mode);
-  if (m->ciss_ready[mode]) return CTC_OK;
-  HostCiss h;
-  CTC_TRY(build_host_ciss(m->train.get(), mode, h));
-  DevCiss &d = m->ciss[mode];
+static ctc_status upload_ciss(ctc_model *m, HostCiss &h, DevCiss &d, uint32_t mode) {
   hipStream_t st = m->ctx->stream;
   d.nmodes = h.nmodes;
   d.mode = mode;
@@ -142,7 +137,44 @@ ctc_status ensure_ciss(ctc_model *m, uint32_t mode) {
   d.h_tile_mask.swap(h.tile_mask);
   d.h_rowptr.swap(h.rowptr);
   d.span_tiles = 0;
+  return CTC_OK;
+}
+
+ctc_status ensure_ciss(ctc_model *m, uint32_t mode) {
+  if (mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "mode %u out of range", mode);
+  if (m->ciss_ready[mode]) return CTC_OK;
+  HostCiss h;
+  CTC_TRY(build_host_ciss(m->train.get(), mode, h));
+  CTC_TRY(upload_ciss(m, h, m->ciss[mode], mode));
   m->ciss_ready[mode] = true;
+  return CTC_OK;
+}
+
+static void free_dev_ciss(DevCiss &d) {
+  for (int s = 0; s < 3; ++s) dev_free(d.oidx[s]);
+  for (void *p : {(void *)d.vals, (void *)d.tile_slot, (void *)d.tile_mask, (void *)d.directory, (void *)d.rowptr,
+                  (void *)d.span_off, (void *)d.slot_pb, (void *)d.grp_piece, (void *)d.grp_n, (void *)d.slot_gb,
+                  (void *)d.mix_items, (void *)d.mix_heavy_slot, (void *)d.mix_heavy_pb, (void *)d.mix_grp,
+                  (void *)d.mix_heavy_gb})
+    dev_free(p);
+  d = DevCiss();
+}
+
+// ref_quirks: the CISS of the mode's nonzeros repeated by their weights
+// (quirks.cpp), built once per set of quirk bits
+static ctc_status ensure_quirk_ciss(ctc_model *m, uint32_t mode, uint32_t flags) {
+  if (m->qciss_flags[mode] == flags) return CTC_OK;
+  CTC_HIP(hipStreamSynchronize(m->ctx->stream));
+  free_dev_ciss(m->qciss[mode]);
+  m->qciss_flags[mode] = 0;
+  ctc_coo *q = nullptr;
+  CTC_TRY(quirk_multiset(m->train.get(), mode, flags, &q));
+  HostCiss h;
+  const ctc_status s = build_host_ciss(q, mode, h);
+  ctc_coo_free(q);
+  CTC_TRY(s);
+  CTC_TRY(upload_ciss(m, h, m->qciss[mode], mode));
+  m->qciss_flags[mode] = flags;
   return CTC_OK;
 }
 
@@ -393,23 +425,8 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
   (void)hipSetDevice(m->ctx->device);
   (void)hipStreamSynchronize(m->ctx->stream);
   for (uint32_t md = 0; md < CTC_MAX_MODES; ++md) {
-    DevCiss &d = m->ciss[md];
-    for (int s = 0; s < 3; ++s) dev_free(d.oidx[s]);
-    dev_free(d.vals);
-    dev_free(d.tile_slot);
-    dev_free(d.tile_mask);
-    dev_free(d.directory);
-    dev_free(d.rowptr);
-    dev_free(d.span_off);
-    dev_free(d.slot_pb);
-    dev_free(d.grp_piece);
-    dev_free(d.grp_n);
-    dev_free(d.slot_gb);
-    dev_free(d.mix_items);
-    dev_free(d.mix_heavy_slot);
-    dev_free(d.mix_heavy_pb);
-    dev_free(d.mix_grp);
-    dev_free(d.mix_heavy_gb);
+    free_dev_ciss(m->ciss[md]);
+    free_dev_ciss(m->qciss[md]);
     dev_free(m->Fbf[md]);
     dev_free(m->F[md]);
     dev_free(m->best[md]);
@@ -555,7 +572,7 @@ extern "C" ctc_status ctc_model_prepare(ctc_model *m, const ctc_options *o) {
 // =========================================================================
 // single steps
 // =========================================================================
-static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
+static ctc_status als_mode_plain(ctc_model *m, uint32_t mode, const ctc_options *o) {
   const uint32_t R = m->rank;
   if (als_uses_rows(o, R)) return als_mode_rows(m, mode, o, o->precision == CTC_PREC_MIXED);
   if (R > 64) return fail(CTC_ERR_UNSUPPORTED, "the span ALS kernels support ranks up to 64 (got %u)", R);
@@ -589,6 +606,21 @@ static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
   }
   m->resid_valid = false;
   return CTC_OK;
+}
+
+// With ref_quirks the update runs on the mode's quirk CISS (the nonzeros
+// repeated by their weights), swapped in for the update and out after it.
+static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
+  const uint32_t q = o->ref_quirks & (CTC_QUIRK_TILE_HEAD | CTC_QUIRK_LAST_LANE | CTC_QUIRK_TRUNC_VALUES);
+  if (o->ref_quirks != q) return fail(CTC_ERR_INVALID_ARG, "ref_quirks: unknown bits 0x%x", o->ref_quirks & ~q);
+  if (!q) return als_mode_plain(m, mode, o);
+  if (mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "mode %u out of range", mode);
+  CTC_TRY(ensure_ciss(m, mode));  // the plain CISS stays resident for the other algorithms
+  CTC_TRY(ensure_quirk_ciss(m, mode, q));
+  std::swap(m->ciss[mode], m->qciss[mode]);
+  const ctc_status s = als_mode_plain(m, mode, o);
+  std::swap(m->ciss[mode], m->qciss[mode]);
+  return s;
 }
 
 static ctc_status check_flag(ctc_model *m) {
@@ -877,6 +909,7 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
 }
 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
+  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ref_quirks reproduces the reference's ALS reductions only");
   CTC_TRY(ensure_ciss(m, 0));
   if (o->sgd_update == CTC_SGD_SYNC) return sgd_epoch_sync(m, o, lr);
   if (o->precision == CTC_PREC_MIXED)
@@ -905,7 +938,10 @@ static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   return CTC_OK;
 }
 
-static ctc_status ccd_epoch(ctc_model *m, const ctc_options *o) { return ctc::ccd_epoch_impl(m, o); }
+static ctc_status ccd_epoch(ctc_model *m, const ctc_options *o) {
+  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ref_quirks reproduces the reference's ALS reductions only");
+  return ctc::ccd_epoch_impl(m, o);
+}
 
 extern "C" ctc_status ctc_als_mode_update(ctc_model *m, uint32_t mode, const ctc_options *o) {
   if (!m || !o || mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "ctc_als_mode_update");

@@ -186,6 +186,34 @@ def mode_order(t: Coo, mode: int):
     return perm, rowptr
 
 
+QUIRK_TILE_HEAD = 1     # Q1: a tile's first row segment is not written
+QUIRK_LAST_LANE = 2     # Q2: the last lane's term re-added in a full tile
+QUIRK_TRUNC_VALUES = 4  # Q5: values stored through an integer (SG/ciss.c:123,157,176)
+
+
+def quirk_weights(t: Coo, mode: int, flags: int, tile_len: int = 32):
+    """Per-nonzero weights (indexed like t) of the reference's tile-reduction
+    defects Q1 / Q2 in the mode-`mode` walk (SG/als.cu:226-312)."""
+    perm, _ = mode_order(t, mode)
+    w = np.empty(t.nnz, np.uint32)
+    st = lib().orc_quirk_weights(t.c(), C.c_uint32(mode), _u32p(perm), C.c_uint32(tile_len),
+                                 C.c_uint32(flags), _u32p(w))
+    assert st == 0, st
+    return w
+
+
+def quirk_multiset(t: Coo, mode: int, flags: int, tile_len: int = 32) -> Coo:
+    """The tensor whose plain ALS update of `mode` equals the reference's
+    update with the given defects: each nonzero repeated by its weight (0
+    drops it), values truncated toward zero under QUIRK_TRUNC_VALUES."""
+    w = quirk_weights(t, mode, flags & (QUIRK_TILE_HEAD | QUIRK_LAST_LANE), tile_len).astype(np.int64)
+    ind = [np.repeat(np.asarray(x), w) for x in t.ind]
+    vals = np.repeat(np.asarray(t.vals, dtype=np.float64), w)
+    if flags & QUIRK_TRUNC_VALUES:
+        vals = np.trunc(vals)
+    return Coo(ind, vals, t.dims)
+
+
 def sort_by_mode(t: Coo, mode: int) -> Coo:
     perm, _ = mode_order(t, mode)
     return t.permuted(perm)

@@ -168,6 +168,37 @@ int orc_mode_order(const orc_coo *t, uint32_t mode, uint32_t *perm, uint64_t *ro
 }
 
 /* ------------------------------------------------------------------------ */
+/* the reference's tile-reduction defects (SG/als.cu:226-312) as weights     */
+/* ------------------------------------------------------------------------ */
+
+int orc_quirk_weights(const orc_coo *t, uint32_t mode, const uint32_t *perm, uint32_t tile_len,
+                      uint32_t flags, uint32_t *w) {
+  if (mode >= t->nmodes || tile_len == 0 || tile_len > 64) return 1;
+  const uint64_t n = t->nnz;
+  const uint32_t *row = t->ind[mode];
+  for (uint64_t b = 0; b < n; b += tile_len) {
+    const uint64_t e = b + tile_len < n ? b + tile_len : n;
+    const uint32_t L = (uint32_t)(e - b);
+    int cont[64];
+    uint32_t iters = 0; /* continuation lanes 1..L-1 (itercounter) */
+    for (uint32_t l = 0; l < L; ++l) {
+      const uint64_t p = b + l;
+      cont[l] = p > 0 && row[perm[p]] == row[perm[p - 1]];
+      if (l > 0) iters += (uint32_t)cont[l];
+      w[perm[p]] = 1;
+    }
+    if (flags & 1) /* Q1: lane 0 through the lane before the next head */
+      for (uint32_t l = 0; l < L && (l == 0 || cont[l]); ++l) w[perm[b + l]] = 0;
+    if ((flags & 2) && L == tile_len && cont[L - 1]) { /* Q2 */
+      uint32_t h = L - 1;
+      while (h > 0 && cont[h]) --h;
+      if (!((flags & 1) && h == 0)) w[perm[b + L - 1]] += iters - (L - 1 - h);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* reference CISS layout (SG/ciss.c:68-195)                                  */
 /* ------------------------------------------------------------------------ */
 

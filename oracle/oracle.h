@@ -47,6 +47,24 @@ void orc_coo_free(orc_coo *t);
  * (CSR over all rows, empty rows included). */
 int orc_mode_order(const orc_coo *t, uint32_t mode, uint32_t *perm, uint64_t *rowptr);
 
+/* ---- the reference's tile-reduction defects as weights ---------------------
+ * How many times each nonzero's term reaches its row's sums in the
+ * reference's warp-per-tile reductions (tiles of tile_len = 32 consecutive
+ * nonzeros of the mode-m order perm, one lane each; SG/als.cu:226-312):
+ *   flags & 1 (Q1): lane 0's mybit is forced to 1 and only lanes with
+ *     mybit = 0 write (SG/als.cu:245-249,285-288,307-310), so the tile's
+ *     first row segment is never written: weight 0;
+ *   flags & 2 (Q2): __shfl_down_sync from the last lane returns its own value
+ *     (SG/als.cu:279-284), so in a full tile whose last lane continues a row
+ *     segment that starts at lane h, the head receives the last lane's term
+ *     itercounter - (tile_len - 1 - h) extra times, itercounter being the
+ *     number of continuation lanes 1..tile_len-1 (SG/als.cu:233).
+ * A partial last tile's last lane reads an inactive lane (undefined in the
+ * reference); it gets no extra.  w[p] is indexed by the nonzero's position in
+ * t.  Returns 0, or 1 on bad arguments. */
+int orc_quirk_weights(const orc_coo *t, uint32_t mode, const uint32_t *perm, uint32_t tile_len,
+                      uint32_t flags, uint32_t *w);
+
 /* ---- reference CISS (TB-COO) layout (SG/ciss.c:68-195) --------------------
  * Input: t, already sorted by `mode` (the reference requires it, SG/ciss.c:74).
  * Writes the reference's arrays exactly as ciss_alloc fills them:

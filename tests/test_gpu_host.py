@@ -13,6 +13,6 @@ from test_host import (test_binary_reader_known_answer, test_binary_reader_zero_
                        test_coo_split_fractions_cover_and_seed, test_library_exports_every_header_symbol,
                        test_options_defaults_follow_reference, test_status_strings, test_synth_affine_range,
                        test_synth_is_seeded_distinct_and_exact, test_tns_reader_errors, test_tns_reader_matches_oracle,
-                       test_tns_write_roundtrip, test_work_unit_defaults)
+                       test_tns_write_roundtrip, test_work_unit_defaults, test_ref_quirk_weights_equal_oracle)
 
 pytestmark = pytest.mark.gpu

@@ -138,6 +138,50 @@ def test_als_mode_update_four_modes(ctx, R, kernel):
         m.set_factors([f.astype(np.float32) for f in F64])
 
 
+@pytest.mark.parametrize("R,kernel", [(16, ct.ALS_SPANS), (32, ct.ALS_SPANS), (32, ct.ALS_ROWS), (64, ct.ALS_ROWS)])
+@pytest.mark.parametrize("flags", [ct.QUIRK_TILE_HEAD | ct.QUIRK_LAST_LANE,
+                                   ct.QUIRK_TILE_HEAD | ct.QUIRK_LAST_LANE | ct.QUIRK_TRUNC_VALUES])
+def test_als_ref_quirks_match_oracle(ctx, R, kernel, flags):
+    """ALS mode updates with the reference's reduction defects
+    (options.ref_quirks) against the oracle's plain ALS on the nonzeros
+    repeated by the oracle's quirk weights (values truncated under Q5),
+    1e-5 per factor; the update differs from the intended one."""
+    tr, te = small_problem(3, nnz=30000, seed=90 + R)
+    say(f"[als_quirks] R={R} kernel={kernel} flags={flags}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 8)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    F64 = [f.astype(np.float64) for f in F]
+    opt = ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, tile_len=256, als_kernel=kernel, ref_quirks=flags)
+    for mode in range(3):
+        plain = [f.copy() for f in F64]
+        m.als_mode_update(mode, opt)
+        assert O.als_mode(O.quirk_multiset(oc, mode, flags), mode, F64, 0.05) == 0
+        e = relerr(m.factor(mode), F64[mode])
+        assert e < 1e-5, (mode, e)
+        assert O.als_mode(oc, mode, plain, 0.05) == 0
+        assert relerr(F64[mode], plain[mode]) > 1e-3, mode  # the defects change the update
+        F64 = [f.astype(np.float32).astype(np.float64) for f in F64]
+        m.set_factors([f.astype(np.float32) for f in F64])
+    # the plain CISS is untouched: an ordinary update afterwards matches the intended rule
+    opt.ref_quirks = 0
+    m.als_mode_update(0, opt)
+    assert O.als_mode(oc, 0, F64, 0.05) == 0
+    assert relerr(m.factor(0), F64[0]) < 1e-5
+
+
+def test_ref_quirks_are_for_als_only(ctx):
+    tr, _ = small_problem(3, nnz=5000, seed=4)
+    m = ct.Model(ctx, tr, None, None, rank=16)
+    m.set_factors(init_factors(tr.dims, 16, 1))
+    for alg, call in ((ct.SGD, lambda o: m.sgd_epoch(o, lr=1e-3)), (ct.CCD, lambda o: m.ccd_epoch(o))):
+        with pytest.raises(ct.CtcError, match="ALS reductions only"):
+            call(ct.Options(algorithm=alg, rank=16, lambda_=0.05, ref_quirks=ct.QUIRK_TILE_HEAD))
+    with pytest.raises(ct.CtcError, match="unknown bits"):
+        m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.05, ref_quirks=8))
+
+
 def test_als_deterministic(ctx):
     tr, te = small_problem(3, nnz=30000, seed=77)
     say("[als_deterministic]")

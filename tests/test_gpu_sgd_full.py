@@ -32,6 +32,7 @@ import cutc_amd as ct
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "sgd_cfg2_sync_oracle.npz")
 GOLD_MIXED = os.path.join(os.path.dirname(__file__), "golden", "sgd_cfg2_sync_mixed_oracle.npz")
+GOLD_CCD = os.path.join(os.path.dirname(__file__), "golden", "ccd_cfg2_oracle.npz")
 DIMS = (480189, 17770, 2182)
 
 
@@ -99,3 +100,34 @@ def test_sgd_mixed_full_size_matches_oracles(full_model):
     say(f"  max test-RMSE gap: {gap_mixed:.2e} to the mixed oracle, {gap_fp64:.2e} to the FP64 oracle")
     assert gap_mixed < 1e-3
     assert gap_fp64 < 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_ccd_full_size_matches_oracle(full_model):
+    """CCD++ (32 columns x 3 modes per epoch, one inner iteration, lambda 0.05)
+    on the same tensor from the same float32-rounded init against the FP64
+    oracle (ccd_cfg2_oracle.npz from tests/golden/make_ccd_cfg2.py): 64 sampled
+    rows of every factor after epoch 1 and the last, and the per-epoch test
+    RMSE.  The GPU keeps FP32 factors and residuals with FP64 row sums."""
+    gold = np.load(GOLD_CCD, allow_pickle=False)
+    m = full_model
+    rng = np.random.default_rng(1)
+    m.set_factors([rng.uniform(-1, 1, (d, 32)).astype(np.float32) for d in DIMS])
+    assert abs(m.evaluate()["rmse_te"] - float(gold["rmse_te_initial"])) < 1e-6
+    opt = ct.Options(algorithm=ct.CCD, rank=32, lambda_=float(gold["lam"]))
+    E = len(gold["rmse_te"])
+    rm = []
+    for e in range(1, E + 1):
+        m.ccd_epoch(opt)
+        rm.append(m.evaluate()["rmse_te"])
+        say(f"  epoch {e}: gpu {rm[-1]:.6f} oracle {gold['rmse_te'][e - 1]:.6f}")
+        if e in (1, E):
+            for k in range(3):
+                got = m.factor(k)[gold[f"rows{k}"]].astype(np.float64)
+                ref = gold[f"f{k}_e{e}"]
+                err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+                say(f"    factor {k} sampled rows rel err {err:.2e}")
+                assert err < 1e-4, (e, k, err)
+    gap = np.max(np.abs(np.array(rm) - gold["rmse_te"]))
+    say(f"  max test-RMSE gap {gap:.2e}")
+    assert gap < 1e-4

@@ -340,3 +340,18 @@ def test_context_without_gpu_reports_error():
     with pytest.raises(ct.CtcError) as e:
         ct.Context(0)
     assert e.value.status in (6, 9)
+
+
+@pytest.mark.parametrize("flags", [1, 2, 3])
+def test_ref_quirk_weights_equal_oracle(flags):
+    """The library's weights of the reference's tile-reduction defects
+    (ctc_ref_quirk_weights) equal the oracle's, which the lane-by-lane walk
+    of test_oracle.py pins, on a skewed tensor in every mode."""
+    tr, _ = ct.synth((300, 120, 40), 30000, zipf=(1.0, 0.8, 0.0), true_rank=4, value_map=ct.VALUES_AFFINE_1_5,
+                     noise_std=0.1, test_fraction=0.0, seed=17)
+    ind, vals = tr.arrays()
+    oc = O.Coo(ind, vals, tr.dims)
+    for mode in range(3):
+        w = tr.ref_quirk_weights(mode, flags)
+        assert np.array_equal(w, O.quirk_weights(oc, mode, flags)), mode
+        assert (w == 0).any() if flags & 1 else (w >= 1).all()

@@ -793,3 +793,92 @@ def test_mttkrp_csf_equals_coo_mttkrp(mode):
     np.testing.assert_allclose(O.mttkrp_csf(c, F, nthreads=4), ref, rtol=1e-12, atol=1e-12)
     F32 = [f.astype(np.float32) for f in F]
     np.testing.assert_allclose(O.mttkrp_csf(c, F32, nthreads=1), ref, rtol=2e-5, atol=2e-5)
+
+
+# --------------------------------------------------------------------------
+# The reference's tile-reduction defects (quirks Q1, Q2) as weights
+# --------------------------------------------------------------------------
+def _reference_tile_walk(rows, terms, q1, q2, L=32):
+    """The reference's warp-per-tile segmented reduction (SG/als.cu:226-312)
+    run lane by lane: per tile of L consecutive nonzeros, mybit = the lane
+    continues the previous nonzero's row (lane 0: forced to 1 under Q1, else
+    a head), mylbit = mybit except lane 0's 0; itercounter rounds of
+    t = shfl_down(nt, 1) + (1 - mylbit) t, nt = mybit t, where the last lane
+    of a full tile reads its own nt under Q2 (0 otherwise) and the last lane of
+    a partial tile reads 0; lanes with mybit = 0 add t to their row."""
+    out = {}
+    n = len(rows)
+    for b in range(0, n, L):
+        nl = min(L, n - b)
+        cont = [int(b + l > 0 and rows[b + l] == rows[b + l - 1]) for l in range(nl)]
+        iters = sum(cont[1:])
+        mybit, mylbit = list(cont), list(cont)
+        mybit[0], mylbit[0] = (1, 0) if q1 else (0, 0)
+        t = [terms[b + l] for l in range(nl)]
+        nt = [mybit[l] * t[l] for l in range(nl)]
+        for _ in range(iters):
+            inc = [nt[l + 1] if l + 1 < nl else (nt[l] if (q2 and nl == L) else 0) for l in range(nl)]
+            t = [inc[l] + (1 - mylbit[l]) * t[l] for l in range(nl)]
+            nt = [mybit[l] * t[l] for l in range(nl)]
+        for l in range(nl):
+            if not mybit[l]:
+                out[rows[b + l]] = out.get(rows[b + l], 0) + t[l]
+    return out
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_quirk_weights_match_the_lane_by_lane_walk(flags):
+    """Per-row sums of integer terms weighted by orc_quirk_weights equal the
+    lane-by-lane simulation of the reference's reduction, for every flag
+    combination, on tensors whose rows run across tiles, end exactly at lane
+    31, fill whole tiles, and end in a partial tile."""
+    rng = np.random.default_rng(70 + flags)
+    q1, q2 = bool(flags & 1), bool(flags & 2)
+    for trial in range(40):
+        dims = (int(rng.integers(3, 40)), 7, 5)
+        lens = rng.geometric(1.0 / rng.choice([2, 8, 40]), size=dims[0])
+        lens[rng.random(dims[0]) < 0.2] = 0
+        rows = np.repeat(np.arange(dims[0]), lens)
+        n = len(rows)
+        if n == 0:
+            continue
+        t = O.Coo([rows, rng.integers(0, 7, n), rng.integers(0, 5, n)], np.ones(n), dims)
+        perm, _ = O.mode_order(t, 0)
+        terms = rng.integers(-50, 50, n)  # by nonzero position in t
+        w = O.quirk_weights(t, 0, flags)
+        want = _reference_tile_walk([int(t.ind[0][p]) for p in perm], [int(terms[p]) for p in perm], q1, q2)
+        got = {}
+        for p in range(n):
+            if w[p]:
+                got[int(t.ind[0][p])] = got.get(int(t.ind[0][p]), 0) + int(w[p]) * int(terms[p])
+        want = {k: v for k, v in want.items() if v != 0}
+        got = {k: v for k, v in got.items() if v != 0}
+        assert got == want, (trial, flags)
+    if flags == 0:
+        assert np.all(w == 1)
+
+
+def test_quirk_weights_hand_worked_tile():
+    """One full tile: lanes 0-4 on row 1 (lane 0 continuing row 1 from the
+    previous tile), lanes 5-31 on row 2.  Q1 drops lanes 0-4; under Q2 the
+    continuation lane 31 (segment head 5) is re-added itercounter - (31 - 5)
+    = (4 + 26) - 26 = 4 extra times, weight 5.  The previous tile (row 0 on
+    lanes 0-30, row 1 on lane 31) has a single-lane head at 31: no extra."""
+    rows = np.array([0] * 31 + [1] + [1] * 5 + [2] * 27)
+    n = len(rows)
+    # mode-1 ids increase with the position, so the mode-0 order is t's order
+    t = O.Coo([rows, np.arange(n), np.zeros(n, int)], np.ones(n), (3, n, 1))
+    w = O.quirk_weights(t, 0, 3)
+    assert list(w[:32]) == [0] * 31 + [1]
+    assert list(w[32:37]) == [0] * 5
+    assert list(w[37:63]) == [1] * 26 and w[63] == 5
+    assert list(O.quirk_weights(t, 0, 1)[32:]) == [0] * 5 + [1] * 27
+    assert O.quirk_weights(t, 0, 2)[63] == 5
+
+
+def test_quirk_multiset_truncates_and_repeats():
+    t = O.Coo([[0, 0, 1], [0, 1, 0], [0, 0, 0]], [2.7, 3.2, -1.5], (2, 2, 1))
+    q = O.quirk_multiset(t, 0, O.QUIRK_TRUNC_VALUES)
+    assert list(q.vals) == [2.0, 3.0, -1.0] and q.nnz == 3
+    q = O.quirk_multiset(t, 0, O.QUIRK_TILE_HEAD)  # one partial tile: its first segment (row 0) dropped
+    assert q.nnz == 1 and list(q.ind[0]) == [1]
