@@ -23,7 +23,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALS, ALS_AUTO, ALS_ROWS, ALS_SPANS, CCD, LR_BOLD_DRIVER, LR_CONSTANT, LR_EXP_DECAY, ORDER_FIXED,  # noqa: F401
-                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, QUIRK_LAST_LANE, QUIRK_TILE_HEAD, QUIRK_TRUNC_VALUES, SGD,
+                   ORDER_ROTATE, PREC_FP32, PREC_MIXED, QUIRK_CCD_GRID, QUIRK_LAST_LANE, QUIRK_NO_ADDBACK,
+                   QUIRK_TILE_HEAD, QUIRK_TRUNC_VALUES, SGD,
                    SGD_ATOMIC, SGD_AUTO, SGD_HOT_ATOMIC, SGD_NEXT_RACY, SGD_RACY, SGD_SYNC,
                    VALUES_AFFINE_1_5, VALUES_INT_1_5, VALUES_RAW, CtcError, check, lib)
 
@@ -76,7 +77,7 @@ class Coo:
 
     def ref_quirk_weights(self, mode, flags):
         """How many times each nonzero's term reaches its row's sums in the
-        reference's ALS reductions of `mode` under QUIRK_* `flags`
+        reference's ALS and CCD reductions of `mode` under QUIRK_* `flags`
         (ctc_ref_quirk_weights)."""
         w = np.empty(self.nnz, np.uint32)
         check(lib().ctc_ref_quirk_weights(self._p, mode, flags, w.ctypes.data_as(C.POINTER(C.c_uint32))))
@@ -267,7 +268,7 @@ class Options:
     sgd_hot_rows: int = None  # None: the library default (65536)
     als_kernel: int = 0
     sgd_substep: int = None
-    ref_quirks: int = None  # QUIRK_* bits: the reference's ALS reduction defects (diagnostic)
+    ref_quirks: int = None  # QUIRK_* bits: the reference's ALS / CCD kernel defects (diagnostic)
 
     def to_c(self):
         o = _lib.Options()

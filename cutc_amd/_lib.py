@@ -15,7 +15,7 @@ TILE_LEN = 64
 ALS, SGD, CCD = 0, 1, 2
 LR_BOLD_DRIVER, LR_EXP_DECAY, LR_CONSTANT = 0, 1, 2
 PREC_FP32, PREC_MIXED = 0, 1
-QUIRK_TILE_HEAD, QUIRK_LAST_LANE, QUIRK_TRUNC_VALUES = 1, 2, 4  # ctc_ref_quirk
+QUIRK_TILE_HEAD, QUIRK_LAST_LANE, QUIRK_TRUNC_VALUES, QUIRK_NO_ADDBACK, QUIRK_CCD_GRID = 1, 2, 4, 8, 16  # ctc_ref_quirk
 ORDER_FIXED, ORDER_ROTATE = 0, 1
 SGD_AUTO, SGD_ATOMIC, SGD_RACY, SGD_NEXT_RACY, SGD_HOT_ATOMIC, SGD_SYNC = 0, 1, 2, 3, 4, 5
 ALS_AUTO, ALS_SPANS, ALS_ROWS = 0, 1, 2

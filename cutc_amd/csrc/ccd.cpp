@@ -8,6 +8,9 @@
 // epoch's column 0) so each (column, mode) costs one pass over the tensor.
 // At the start of column f every mode's per-row records are packed from the
 // column-major mirror: (pending column's new value, f saved, f current).
+//
+// ccd_epoch_jacobi is the reference's own CCD form, run on request only.
+#include <algorithm>
 #include <vector>
 
 #include "ctc_internal.h"
@@ -81,6 +84,39 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
   }
   for (uint32_t md = 0; md < N; ++md)
     CTC_HIP(launch_transpose(st, m->FT[md], m->F[md], m->dims[md], (int)R, false));
+  return CTC_OK;
+}
+
+// The reference's CCD (tc_ccd, SG/ccd.cu:504-560), for ref_quirks: one mode
+// after another, num / den zeroed (:548-549), the Jacobi walk sums e h and
+// h * h per row and column from the factors as they are (update_frac_gpu_as,
+// with the tile defects as the weights of the quirk CISS), then every row of
+// the mode's directory takes num / (lambda + den) (update_ccd_gpu).  A row
+// whose every term was dropped becomes 0; rows without nonzeros are kept.
+ctc_status ccd_epoch_jacobi(ctc_model *m, const ctc_options *o, uint32_t start) {
+  const uint32_t N = m->nmodes, R = m->rank;
+  const uint32_t wflags =
+      o->ref_quirks & (CTC_QUIRK_TILE_HEAD | CTC_QUIRK_LAST_LANE | CTC_QUIRK_TRUNC_VALUES | CTC_QUIRK_CCD_GRID);
+  hipStream_t st = m->ctx->stream;
+  uint64_t maxdim = 0;
+  for (uint32_t md = 0; md < N; ++md) maxdim = std::max<uint64_t>(maxdim, m->dims[md]);
+  CTC_TRY(ensure_scratch(m, 2 * maxdim * R * sizeof(float)));
+  float *num = (float *)m->scratch, *den = num + maxdim * R;
+  for (uint32_t k = 0; k < N; ++k) {
+    const uint32_t md = (start + k) % N;
+    CTC_TRY(ensure_ciss(m, md));
+    if (wflags) CTC_TRY(ensure_quirk_ciss(m, md, wflags));
+    const DevCiss &walk = wflags ? m->qciss[md] : m->ciss[md];
+    const DevCiss &d = m->ciss[md];
+    CTC_HIP(launch_fill(st, num, m->dims[md] * R, 0.f));
+    CTC_HIP(launch_fill(st, den, m->dims[md] * R, 0.f));
+    {
+      Prof p(m->ctx, "ccd_jacobi");
+      CTC_HIP(launch_ccd_jacobi(st, ciss_args(walk), rel_factors(m->F, N, md), (int)N, (int)R, 256, num, den, 256));
+    }
+    CTC_HIP(launch_ccd_jacobi_rows(st, d.directory, d.dlength, (int)R, num, den, o->lambda, m->F[md]));
+  }
+  m->resid_valid = false;
   return CTC_OK;
 }
 

@@ -17,7 +17,8 @@
 //   --precision fp32|mixed (BF16-operand ALS; SGD with BF16-staged run deltas)
 //   --refine K   --sgd-update P  --sgd-hot-rows N
 //   --sgd-substep N (nonzeros per walker per sub-step of the default SGD schedule)
-//   --ref-quirks B (ALS: the reference's reduction defects, bits 1 Q1, 2 Q2, 4 Q5)
+//   --ref-quirks B (the reference's kernel defects: ALS and CCD bits 1 Q1, 2 Q2,
+//                   4 Q5; CCD bits 8 Q4, its update form, and 16 Q3, its grid)
 //
 //   ctc prep <tensor> <prefix> [--seed S] [--decimals D] [--binary]  (data preparation)
 #include <cstdio>

@@ -54,7 +54,8 @@ struct HostCiss {
   std::vector<uint64_t> tile_mask;
 };
 ctc_status build_host_ciss(const ctc_coo *t, uint32_t mode, HostCiss &out);
-// the nonzeros of t repeated by their ref_quirks weights for `mode` (quirks.cpp)
+// the nonzeros of t repeated by their ref_quirks weights for `mode`
+// (quirks.cpp); *out = nullptr when every weight is 0
 ctc_status quirk_multiset(const ctc_coo *t, uint32_t mode, uint32_t flags, ctc_coo **out);
 
 // ----------------------------------------------------------- device CISS --
@@ -205,8 +206,18 @@ namespace ctc {
 ctc_status ensure_scratch(ctc_model *m, size_t bytes);
 ctc_status ensure_ciss(ctc_model *m, uint32_t mode);
 ctc_status ensure_pieces(ctc_model *m, uint32_t mode, uint32_t span_tiles);
+// every ctc_ref_quirk bit
+constexpr uint32_t kQuirkAll = CTC_QUIRK_TILE_HEAD | CTC_QUIRK_LAST_LANE | CTC_QUIRK_TRUNC_VALUES |
+                               CTC_QUIRK_NO_ADDBACK | CTC_QUIRK_CCD_GRID;
+// the device CISS of `mode`'s nonzeros repeated by their quirk weights under
+// `flags` (model.cpp), kept in m->qciss[mode] until other flags are asked for;
+// it has nnz = 0 (and no arrays) when every weight is 0
+ctc_status ensure_quirk_ciss(ctc_model *m, uint32_t mode, uint32_t flags);
 // one CCD++ epoch on the device (ccd.cpp)
 ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o);
+// one epoch of the reference's CCD form (ref_quirks with CTC_QUIRK_NO_ADDBACK;
+// ccd.cpp), modes start, start + 1, ... in turn
+ctc_status ccd_epoch_jacobi(ctc_model *m, const ctc_options *o, uint32_t start);
 // one row-fused ALS mode update (ranks 32, 64, 128), enqueued on the model's
 // stream: BF16-operand Gram (bf16) or FP64-accumulated Gram from FP32 factors
 ctc_status als_mode_rows(ctc_model *m, uint32_t mode, const ctc_options *o, bool bf16);

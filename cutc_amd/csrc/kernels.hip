@@ -641,15 +641,21 @@ __global__ void __launch_bounds__(256) k_sgd_fold(SettleArgs a) {
 // ==========================================================================
 // MTTKRP / adjoint walk: per row, acc = sum over its nonzeros of w * h with
 // h = hadamard of the other modes' rows and
-//   w = x                       (MTTKRP, SG/als.cu:295-312 intent), or
-//   w = x - <F_mode(row), h>    (loss adjoint: residual-weighted MTTKRP).
+//   w = x                       (kWalkMttkrp: SG/als.cu:295-312 intent), or
+//   w = x - <F_mode(row), h>    (kWalkAdjoint, kWalkJacobi: the residual).
 // Rows may span walkers, so each walker adds its partial row with atomics
 // into a zeroed output.  For the adjoint, out receives -acc and loss_part
-// the per-block sum of squared residuals.
+// the per-block sum of squared residuals.  kWalkJacobi is the reference's
+// CCD sums (SG/ccd.cu:157-195): out receives acc = sum e h and den the
+// sum of h * h, per column, all from the factors as they were at launch.
 // ==========================================================================
-template <int R, int N, bool ADJ>
+enum : int { kWalkMttkrp = 0, kWalkAdjoint = 1, kWalkJacobi = 2 };
+
+template <int R, int N, int KIND>
 __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint32_t span,
-                                                float *out, double *loss_part) {
+                                                float *out, double *loss_part, float *den) {
+  constexpr bool ADJ = KIND != kWalkMttkrp;  // w is the residual
+  constexpr bool SQ = KIND == kWalkJacobi;
   constexpr int W = R < 64 ? R : 64;
   constexpr int CPL = R / W;
   constexpr int WPW = 64 / W;
@@ -664,11 +670,11 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
   const uint64_t s1 = s0 + span < c.nnz ? s0 + span : c.nnz;
   const bool wave_idle = wave * WPW * (uint64_t)span >= c.nnz;
 
-  float acc[CPL], own[CPL];
+  float acc[CPL], own[CPL], sq[CPL];
   double lsum = 0.0;
   uint32_t arow = NONE, slot = 0;
 #pragma unroll
-  for (int q = 0; q < CPL; ++q) acc[q] = 0.f, own[q] = 0.f;
+  for (int q = 0; q < CPL; ++q) acc[q] = 0.f, own[q] = 0.f, sq[q] = 0.f;
 
   if (!wave_idle) {
     for (uint32_t ch = 0; ch < span; ch += W) {
@@ -699,8 +705,10 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
           if (arow == NONE || head) {
             if (arow != NONE) {
 #pragma unroll
-              for (int q2 = 0; q2 < CPL; ++q2)
-                atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, ADJ ? -acc[q2] : acc[q2]);
+              for (int q2 = 0; q2 < CPL; ++q2) {
+                atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, KIND == kWalkAdjoint ? -acc[q2] : acc[q2]);
+                if (SQ) atomic_add_f32(den + (size_t)arow * R + wl + q2 * W, sq[q2]);
+              }
               ++slot;
             } else {
               slot = c.tile_slot[s0 >> 6];
@@ -709,6 +717,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
 #pragma unroll
             for (int q2 = 0; q2 < CPL; ++q2) {
               acc[q2] = 0.f;
+              sq[q2] = 0.f;
               if (ADJ) own[q2] = fa.f[0][(size_t)arow * R + wl + q2 * W];
             }
           }
@@ -726,20 +735,25 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
 #pragma unroll
             for (int q2 = 0; q2 < CPL; ++q2) part += own[q2] * h[q2];
             w = x - group_sum<W>(part);
-            if (wl == 0) lsum += (double)w * (double)w;
+            if (KIND == kWalkAdjoint && wl == 0) lsum += (double)w * (double)w;
           }
 #pragma unroll
-          for (int q2 = 0; q2 < CPL; ++q2) acc[q2] += w * h[q2];
+          for (int q2 = 0; q2 < CPL; ++q2) {
+            acc[q2] += w * h[q2];
+            if (SQ) sq[q2] += h[q2] * h[q2];
+          }
         }
       }
     }
     if (arow != NONE) {
 #pragma unroll
-      for (int q2 = 0; q2 < CPL; ++q2)
-        atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, ADJ ? -acc[q2] : acc[q2]);
+      for (int q2 = 0; q2 < CPL; ++q2) {
+        atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, KIND == kWalkAdjoint ? -acc[q2] : acc[q2]);
+        if (SQ) atomic_add_f32(den + (size_t)arow * R + wl + q2 * W, sq[q2]);
+      }
     }
   }
-  if (ADJ) {
+  if (KIND == kWalkAdjoint) {
     double t = wave_sum_d(lsum);
     if (lane == 0) red[threadIdx.x >> 6] = t;
     __syncthreads();
@@ -1424,18 +1438,34 @@ hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f,
   const unsigned grid = grid_for(waves, block);
   *grid_out = grid;
   if (c.nnz == 0) return hipSuccess;
+  float *none = nullptr;
   if (nmodes == 3) {
     if (adjoint) {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, true>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, kWalkAdjoint>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part, none));
     } else {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, false>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, kWalkMttkrp>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part, none));
     }
   } else if (nmodes == 4) {
     if (adjoint) {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, true>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, kWalkAdjoint>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part, none));
     } else {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, false>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, kWalkMttkrp>), dim3(grid), dim3(block), 0, st, c, f, span, out, loss_part, none));
     }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ccd_jacobi(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                             uint32_t span, float *num, float *den, unsigned block) {
+  if (c.nnz == 0) return hipSuccess;
+  const unsigned grid = (unsigned)mttkrp_grid(c.nnz, R, span, block);
+  double *none = nullptr;
+  if (nmodes == 3) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, kWalkJacobi>), dim3(grid), dim3(block), 0, st, c, f, span, num, none, den));
+  } else if (nmodes == 4) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, kWalkJacobi>), dim3(grid), dim3(block), 0, st, c, f, span, num, none, den));
   } else {
     return hipErrorInvalidValue;
   }

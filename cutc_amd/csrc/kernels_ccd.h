@@ -25,6 +25,10 @@ hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_
 hipError_t launch_ccd_pack(hipStream_t st, const float *ft, uint64_t rows, int col, int sub_col, float4 *x);
 hipError_t launch_ccd_resid(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                             float *resid);
+// F(row, :) <- num(row, :) / (lambda + den(row, :)) for the rows of a
+// directory (SG/ccd.cu:318-348)
+hipError_t launch_ccd_jacobi_rows(hipStream_t st, const uint32_t *directory, uint64_t dlength, int R,
+                                  const float *num, const float *den, double lam, float *F);
 hipError_t launch_transpose(hipStream_t st, const float *in, float *out, uint64_t rows, int R,
                             bool to_colmajor);
 

@@ -208,6 +208,19 @@ __global__ void __launch_bounds__(256) k_ccd_resid(CissArgs c, FactorArgs fa, in
   }
 }
 
+// The reference's CCD update of a mode (SG/ccd.cu:318-348): every column of
+// each directory row from the sums of the Jacobi walk, in place; rows outside
+// the directory keep their values.
+__global__ void __launch_bounds__(256) k_ccd_jacobi_rows(const uint32_t *directory, uint64_t dlength, int R,
+                                                         const float *num, const float *den, double lam, float *F) {
+  const uint64_t n = dlength * (uint64_t)R;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const size_t e = (size_t)directory[p / R] * R + p % R;
+    F[e] = (float)((double)num[e] / (lam + (double)den[e]));
+  }
+}
+
 // out (R x rows, column-major) <-> in (rows x R, row-major)
 __global__ void __launch_bounds__(256) k_transpose(const float *in, float *out, uint64_t rows, int R,
                                                    int to_colmajor) {
@@ -264,6 +277,17 @@ hipError_t launch_ccd_resid(hipStream_t st, const CissArgs &c, const FactorArgs 
     hipLaunchKernelGGL((k_ccd_resid<4>), dim3((unsigned)blocks), dim3(256), 0, st, c, f, R, resid);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_ccd_jacobi_rows(hipStream_t st, const uint32_t *directory, uint64_t dlength, int R,
+                                  const float *num, const float *den, double lam, float *F) {
+  const uint64_t n = dlength * (uint64_t)R;
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_ccd_jacobi_rows, dim3((unsigned)blocks), dim3(256), 0, st, directory, dlength, R, num, den,
+                     lam, F);
   return hipGetLastError();
 }
 

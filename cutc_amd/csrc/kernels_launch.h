@@ -22,6 +22,11 @@ hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);
 uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block);
+// the reference's CCD sums of one mode (SG/ccd.cu:157-195) over the CISS c:
+// num += sum of e h, den += sum of h * h per row and column, e the residual
+// of the factors as they are; num / den are zeroed dims x R arrays
+hipError_t launch_ccd_jacobi(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                             uint32_t span, float *num, float *den, unsigned block);
 hipError_t launch_als_gram(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                            uint32_t span_tiles, uint64_t nspans, const uint32_t *span_off,
                            double *pieces, unsigned block);

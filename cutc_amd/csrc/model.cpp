@@ -162,13 +162,17 @@ static void free_dev_ciss(DevCiss &d) {
 
 // ref_quirks: the CISS of the mode's nonzeros repeated by their weights
 // (quirks.cpp), built once per set of quirk bits
-static ctc_status ensure_quirk_ciss(ctc_model *m, uint32_t mode, uint32_t flags) {
+ctc_status ensure_quirk_ciss(ctc_model *m, uint32_t mode, uint32_t flags) {
   if (m->qciss_flags[mode] == flags) return CTC_OK;
   CTC_HIP(hipStreamSynchronize(m->ctx->stream));
   free_dev_ciss(m->qciss[mode]);
   m->qciss_flags[mode] = 0;
   ctc_coo *q = nullptr;
   CTC_TRY(quirk_multiset(m->train.get(), mode, flags, &q));
+  if (!q) {  // every term dropped
+    m->qciss_flags[mode] = flags;
+    return CTC_OK;
+  }
   HostCiss h;
   const ctc_status s = build_host_ciss(q, mode, h);
   ctc_coo_free(q);
@@ -610,13 +614,19 @@ static ctc_status als_mode_plain(ctc_model *m, uint32_t mode, const ctc_options 
 
 // With ref_quirks the update runs on the mode's quirk CISS (the nonzeros
 // repeated by their weights), swapped in for the update and out after it.
+// The CCD bits (Q3, Q4) do not concern ALS and are ignored here.
 static ctc_status als_mode(ctc_model *m, uint32_t mode, const ctc_options *o) {
+  if (o->ref_quirks & ~kQuirkAll) return fail(CTC_ERR_INVALID_ARG, "ref_quirks: unknown bits 0x%x", o->ref_quirks & ~kQuirkAll);
   const uint32_t q = o->ref_quirks & (CTC_QUIRK_TILE_HEAD | CTC_QUIRK_LAST_LANE | CTC_QUIRK_TRUNC_VALUES);
-  if (o->ref_quirks != q) return fail(CTC_ERR_INVALID_ARG, "ref_quirks: unknown bits 0x%x", o->ref_quirks & ~q);
   if (!q) return als_mode_plain(m, mode, o);
   if (mode >= m->nmodes) return fail(CTC_ERR_INVALID_ARG, "mode %u out of range", mode);
   CTC_TRY(ensure_ciss(m, mode));  // the plain CISS stays resident for the other algorithms
   CTC_TRY(ensure_quirk_ciss(m, mode, q));
+  if (m->qciss[mode].nnz == 0) {  // the reference zeroes the factor and solves lambda I a = 0
+    CTC_HIP(launch_fill(m->ctx->stream, m->F[mode], m->dims[mode] * m->rank, 0.f));
+    m->resid_valid = false;
+    return CTC_OK;
+  }
   std::swap(m->ciss[mode], m->qciss[mode]);
   const ctc_status s = als_mode_plain(m, mode, o);
   std::swap(m->ciss[mode], m->qciss[mode]);
@@ -909,7 +919,7 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
 }
 
 static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
-  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ref_quirks reproduces the reference's ALS reductions only");
+  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ref_quirks reproduces the reference's ALS and CCD kernels only");
   CTC_TRY(ensure_ciss(m, 0));
   if (o->sgd_update == CTC_SGD_SYNC) return sgd_epoch_sync(m, o, lr);
   if (o->precision == CTC_PREC_MIXED)
@@ -938,9 +948,14 @@ static ctc_status sgd_epoch(ctc_model *m, const ctc_options *o, double lr) {
   return CTC_OK;
 }
 
-static ctc_status ccd_epoch(ctc_model *m, const ctc_options *o) {
-  if (o->ref_quirks) return fail(CTC_ERR_UNSUPPORTED, "ref_quirks reproduces the reference's ALS reductions only");
-  return ctc::ccd_epoch_impl(m, o);
+// `start`: the first mode of the reference's CCD form (ref_quirks), which
+// updates whole modes in turn; CCD++ visits every mode per column.
+static ctc_status ccd_epoch(ctc_model *m, const ctc_options *o, uint32_t start = 0) {
+  if (!o->ref_quirks) return ctc::ccd_epoch_impl(m, o);
+  if (o->ref_quirks & ~kQuirkAll) return fail(CTC_ERR_INVALID_ARG, "ref_quirks: unknown bits 0x%x", o->ref_quirks & ~kQuirkAll);
+  if (!(o->ref_quirks & CTC_QUIRK_NO_ADDBACK))
+    return fail(CTC_ERR_INVALID_ARG, "ref_quirks for CCD need CTC_QUIRK_NO_ADDBACK: the reference's CCD has that form only");
+  return ctc::ccd_epoch_jacobi(m, o, start % m->nmodes);
 }
 
 extern "C" ctc_status ctc_als_mode_update(ctc_model *m, uint32_t mode, const ctc_options *o) {
@@ -1157,13 +1172,13 @@ extern "C" ctc_status ctc_solve(ctc_model *m, const ctc_options *o, ctc_result *
   CTC_HIP(hipMemsetAsync(m->dflag, 0, sizeof(uint32_t), st));
   for (uint32_t e = 1; e <= o->max_epochs; ++e) {
     CTC_HIP(hipEventRecord(e0, st));
+    const uint32_t start = o->mode_order == CTC_ORDER_ROTATE ? (uint32_t)(rot() % N) : 0;
     if (o->algorithm == CTC_ALS) {
-      const uint32_t start = o->mode_order == CTC_ORDER_ROTATE ? (uint32_t)(rot() % N) : 0;
       for (uint32_t k = 0; k < N; ++k) CTC_TRY(als_mode(m, (start + k) % N, o));
     } else if (o->algorithm == CTC_SGD) {
       CTC_TRY(sgd_epoch(m, o, lr));
     } else {
-      CTC_TRY(ccd_epoch(m, o));
+      CTC_TRY(ccd_epoch(m, o, start));
     }
     CTC_HIP(hipEventRecord(e1, st));
     CTC_HIP(hipEventSynchronize(e1));

@@ -15,9 +15,12 @@
 //       itercounter - (31 - h) extra times, itercounter being the number of
 //       continuation lanes 1..31 (SG/als.cu:233).
 // The last lane of a partial tile reads an inactive lane (undefined in the
-// reference) and gets no extra.  An ALS update with these defects is then the
-// plain update on a multiset of the nonzeros, each repeated by its weight;
-// Q5 stores the values through an integer (SG/ciss.c:123,157,176).
+// reference) and gets no extra.  The CCD kernel reduces its sums the same way
+// (SG/ccd.cu:120-195), and its launch covers only 8 * (T / 256 + 1) of the
+// T = nnz / 32 + 1 tiles (Q3, SG/ccd.cu:520): the others have weight 0.  An
+// ALS or CCD update with these defects is then the plain update on a multiset
+// of the nonzeros, each repeated by its weight; Q5 stores the values through
+// an integer (SG/ciss.c:123,157,176).
 #include <cmath>
 #include <vector>
 
@@ -27,7 +30,8 @@ namespace ctc {
 
 void mode_order_perm(const ctc_coo *t, uint32_t mode, std::vector<uint32_t> &order);
 
-static constexpr uint32_t kRefTile = 32;  // DEFAULT_T_TILE_LENGTH = ALS_WARPSIZE (SG/base.h:47,62)
+static constexpr uint32_t kRefTile = 32;    // DEFAULT_T_TILE_LENGTH = ALS_WARPSIZE (SG/base.h:47,62)
+static constexpr uint32_t kRefBlock = 256;  // DEFAULT_BLOCKSIZE (SG/base.h:61)
 
 // w[p] for nonzero p of t (its position in t), given the mode's order
 static void quirk_weights(const ctc_coo *t, uint32_t mode, uint32_t flags, const std::vector<uint32_t> &order,
@@ -52,6 +56,12 @@ static void quirk_weights(const ctc_coo *t, uint32_t mode, uint32_t flags, const
       if (!((flags & CTC_QUIRK_TILE_HEAD) && h == 0)) w[order[b + L - 1]] += iters - (L - 1 - h);
     }
   }
+  if (flags & CTC_QUIRK_CCD_GRID) {
+    // blocknum_m = tilenum / DEFAULT_BLOCKSIZE + 1 blocks of 256 / 32 warps, one tile each
+    const uint64_t tiles = n / kRefTile + 1;
+    const uint64_t covered = (tiles / kRefBlock + 1) * (kRefBlock / kRefTile);
+    for (uint64_t p = covered * kRefTile; p < n; ++p) w[order[p]] = 0;
+  }
 }
 
 ctc_status quirk_multiset(const ctc_coo *t, uint32_t mode, uint32_t flags, ctc_coo **out) {
@@ -60,7 +70,8 @@ ctc_status quirk_multiset(const ctc_coo *t, uint32_t mode, uint32_t flags, ctc_c
   quirk_weights(t, mode, flags, order, w.data());
   uint64_t n = 0;
   for (uint64_t p = 0; p < t->nnz; ++p) n += w[p];
-  if (n == 0) return fail(CTC_ERR_INVALID_ARG, "ref_quirks: every nonzero of mode %u dropped", mode);
+  *out = nullptr;
+  if (n == 0) return CTC_OK;  // every term dropped: the reference's rows of the mode all become 0
   CTC_TRY(ctc_coo_alloc(t->nmodes, n, t->dims, out));
   uint64_t q = 0;
   for (uint64_t i = 0; i < t->nnz; ++i) {  // in mode order: repeated copies stay together

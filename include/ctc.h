@@ -152,13 +152,14 @@ ctc_status ctc_ciss_reference_layout(const ctc_ciss *c, const ctc_coo *src, uint
                                      int truncate_vals, uint64_t *directory, uint64_t *dcounter,
                                      double *entries, uint64_t *dlength, uint64_t *size);
 void ctc_ciss_free(ctc_ciss *c);
-/* How many times each nonzero's term reaches its row's Gram and MTTKRP sums
- * in the reference's warp-per-tile reductions of `mode` (tiles of 32 nonzeros
- * of the mode's order, SG/als.cu:226-312) under the ctc_ref_quirk bits
- * CTC_QUIRK_TILE_HEAD / CTC_QUIRK_LAST_LANE of `flags`: weights[p] for the
- * nonzero at position p of t (1 everywhere with neither bit).  Replaces
- * nothing in the reference (its defects are implicit in p_mttkrp_gpu_as);
- * ALS with ctc_options.ref_quirks solves on each mode's nonzeros repeated by
+/* How many times each nonzero's term reaches its row's sums in the
+ * reference's warp-per-tile reductions of `mode` (tiles of 32 nonzeros of the
+ * mode's order, SG/als.cu:226-312, SG/ccd.cu:120-195) under the ctc_ref_quirk
+ * bits CTC_QUIRK_TILE_HEAD / CTC_QUIRK_LAST_LANE / CTC_QUIRK_CCD_GRID of
+ * `flags`: weights[p] for the nonzero at position p of t (1 everywhere with
+ * none of them; other bits are ignored).  Replaces nothing in the reference
+ * (its defects are implicit in p_mttkrp_gpu_as and update_frac_gpu_as); ALS
+ * and CCD with ctc_options.ref_quirks run on each mode's nonzeros repeated by
  * these weights. */
 ctc_status ctc_ref_quirk_weights(const ctc_coo *t, uint32_t mode, uint32_t flags, uint32_t *weights);
 
@@ -203,20 +204,34 @@ typedef enum ctc_precision {
                          the walkers' arithmetic and the sums of deltas stay FP32 / FP64 */
 } ctc_precision;
 typedef enum ctc_mode_order { CTC_ORDER_FIXED = 0, CTC_ORDER_ROTATE = 1 } ctc_mode_order;
-/* Diagnostic reproduction of the reference's ALS reductions (bits of
- * ctc_options.ref_quirks; ALS only, 0 = the intended rule).  The reference
- * reduces every tile of 32 nonzeros of a mode's order with one warp
- * (SG/als.cu:226-312):
+/* Diagnostic reproduction of the reference's ALS and CCD kernels (bits of
+ * ctc_options.ref_quirks; 0 = the intended rule).  The reference reduces
+ * every tile of 32 nonzeros of a mode's order with one warp, in ALS
+ * (SG/als.cu:226-312) and CCD (SG/ccd.cu:120-195) alike:
  *   CTC_QUIRK_TILE_HEAD   (Q1) the tile's first row segment is not written
  *                              (lane 0's mybit forced to 1, SG/als.cu:245-249);
  *   CTC_QUIRK_LAST_LANE   (Q2) lane 31's term is re-added by the shuffle that
  *                              reads lane 31 itself (SG/als.cu:279-284);
  *   CTC_QUIRK_TRUNC_VALUES (Q5) values stored through an integer
- *                              (SG/ciss.c:123,157,176). */
+ *                              (SG/ciss.c:123,157,176);
+ * and, for CCD only:
+ *   CTC_QUIRK_NO_ADDBACK  (Q4) the reference's CCD update in place of CCD++:
+ *                              per mode, every column of a row from one residual
+ *                              e = x - <current factors>, a_ir = sum e h_r /
+ *                              (lambda + sum h_r^2), rows of the mode's
+ *                              directory only (SG/ccd.cu:157-195,318-348);
+ *                              the modes in the ALS order (ctc_mode_order);
+ *   CTC_QUIRK_CCD_GRID    (Q3) only the first 8 * (floor(T / 256) + 1) of the
+ *                              T = floor(nnz / 32) + 1 tiles contribute, the
+ *                              tiles the CCD launch covers (SG/ccd.cu:520).
+ * ALS uses Q1, Q2, Q5 and ignores the CCD bits; CCD with any bit requires
+ * Q4 (the reference's CCD has no other form); SGD accepts none. */
 typedef enum ctc_ref_quirk {
   CTC_QUIRK_TILE_HEAD = 1,
   CTC_QUIRK_LAST_LANE = 2,
-  CTC_QUIRK_TRUNC_VALUES = 4
+  CTC_QUIRK_TRUNC_VALUES = 4,
+  CTC_QUIRK_NO_ADDBACK = 8,
+  CTC_QUIRK_CCD_GRID = 16
 } ctc_ref_quirk;
 /* How concurrent SGD walkers combine updates of a factor row they share.  A
  * walker keeps the rows of its current nonzero in registers; "racy" rows are
@@ -292,7 +307,7 @@ typedef struct ctc_options {
   int32_t als_kernel;      /* ctc_als_kernel (FP32 ALS) */
   uint32_t sgd_substep;    /* CTC_SGD_SYNC: nonzeros per walker per sub-step (0 = 32);
                               at most tile_len */
-  uint32_t ref_quirks;     /* ctc_ref_quirk bits (ALS only; 0 = the intended rule) */
+  uint32_t ref_quirks;     /* ctc_ref_quirk bits (ALS and CCD; 0 = the intended rule) */
 } ctc_options;
 
 /* Reference defaults for an algorithm (SG/base.h, SG/completion.c:460-480). */

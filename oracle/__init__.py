@@ -189,11 +189,15 @@ def mode_order(t: Coo, mode: int):
 QUIRK_TILE_HEAD = 1     # Q1: a tile's first row segment is not written
 QUIRK_LAST_LANE = 2     # Q2: the last lane's term re-added in a full tile
 QUIRK_TRUNC_VALUES = 4  # Q5: values stored through an integer (SG/ciss.c:123,157,176)
+QUIRK_NO_ADDBACK = 8    # Q4: the reference's CCD update form (SG/ccd.cu:157-195,318-348)
+QUIRK_CCD_GRID = 16     # Q3: only the tiles the CCD launch covers (SG/ccd.cu:520)
+_WEIGHT_BITS = QUIRK_TILE_HEAD | QUIRK_LAST_LANE | QUIRK_CCD_GRID
 
 
 def quirk_weights(t: Coo, mode: int, flags: int, tile_len: int = 32):
     """Per-nonzero weights (indexed like t) of the reference's tile-reduction
-    defects Q1 / Q2 in the mode-`mode` walk (SG/als.cu:226-312)."""
+    defects Q1 / Q2 in the mode-`mode` walk (SG/als.cu:226-312) and of the
+    CCD grid Q3 (SG/ccd.cu:520)."""
     perm, _ = mode_order(t, mode)
     w = np.empty(t.nnz, np.uint32)
     st = lib().orc_quirk_weights(t.c(), C.c_uint32(mode), _u32p(perm), C.c_uint32(tile_len),
@@ -206,12 +210,31 @@ def quirk_multiset(t: Coo, mode: int, flags: int, tile_len: int = 32) -> Coo:
     """The tensor whose plain ALS update of `mode` equals the reference's
     update with the given defects: each nonzero repeated by its weight (0
     drops it), values truncated toward zero under QUIRK_TRUNC_VALUES."""
-    w = quirk_weights(t, mode, flags & (QUIRK_TILE_HEAD | QUIRK_LAST_LANE), tile_len).astype(np.int64)
+    w = quirk_weights(t, mode, flags & _WEIGHT_BITS, tile_len).astype(np.int64)
     ind = [np.repeat(np.asarray(x), w) for x in t.ind]
     vals = np.repeat(np.asarray(t.vals, dtype=np.float64), w)
     if flags & QUIRK_TRUNC_VALUES:
         vals = np.trunc(vals)
     return Coo(ind, vals, t.dims)
+
+
+def ccd_jacobi_mode(t: Coo, mode: int, F, lam, flags=0, tile_len: int = 32):
+    """In-place FP64 update of F[mode] by the reference's CCD form (quirk Q4,
+    orc_ccd_jacobi_mode), its sums weighted by quirk_weights(flags) and the
+    values truncated under QUIRK_TRUNC_VALUES."""
+    assert F[0].dtype == np.float64
+    perm, rowptr = mode_order(t, mode)
+    w = quirk_weights(t, mode, flags & _WEIGHT_BITS, tile_len)
+    st = lib().orc_ccd_jacobi_mode(t.c(), C.c_uint32(mode), C.c_uint32(F[0].shape[1]), _ptrs(F, "f64"),
+                                   C.c_double(lam), _u32p(w), C.c_int(1 if flags & QUIRK_TRUNC_VALUES else 0),
+                                   _u32p(perm), _u64p(rowptr))
+    assert st == 0, st
+
+
+def ccd_jacobi_epoch(t: Coo, F, lam, flags=0, start=0):
+    """One epoch of the reference's CCD form: modes start, start + 1, ... in turn."""
+    for k in range(t.nmodes):
+        ccd_jacobi_mode(t, (start + k) % t.nmodes, F, lam, flags)
 
 
 def sort_by_mode(t: Coo, mode: int) -> Coo:

@@ -195,6 +195,47 @@ int orc_quirk_weights(const orc_coo *t, uint32_t mode, const uint32_t *perm, uin
       if (!((flags & 1) && h == 0)) w[perm[b + L - 1]] += iters - (L - 1 - h);
     }
   }
+  if (flags & 16) { /* Q3: the CCD grid, tilenum / 256 + 1 blocks of 256 / tile_len warps (SG/ccd.cu:520) */
+    const uint64_t tiles = n / tile_len + 1;
+    const uint64_t covered = (tiles / 256 + 1) * (256 / tile_len);
+    for (uint64_t p = covered * tile_len; p < n; ++p) w[perm[p]] = 0;
+  }
+  return 0;
+}
+
+/* The reference's CCD update of one mode (update_frac_gpu_as +
+ * update_ccd_gpu, SG/ccd.cu:82-206,318-348), in FP64: for every row i with a
+ * nonzero in t, num_f = sum w_p e_p h_f and den_f = sum w_p h_f^2 over its
+ * nonzeros, e_p = x_p - <F_mode(i,:), h> from the row's values before the
+ * update (x truncated toward zero when trunc_vals), then F_mode(i,f) = num_f /
+ * (lambda + den_f) for every f; rows without nonzeros are kept. */
+int orc_ccd_jacobi_mode(const orc_coo *t, uint32_t mode, uint32_t R, double **F, double lambda,
+                        const uint32_t *w, int trunc_vals, const uint32_t *perm, const uint64_t *rowptr) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  double h[512], num[512], den[512];
+  for (uint64_t i = 0; i < t->dims[mode]; ++i) {
+    if (rowptr[i] == rowptr[i + 1]) continue;
+    double *a = F[mode] + (size_t)i * R;
+    for (uint32_t f = 0; f < R; ++f) num[f] = den[f] = 0.0;
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const uint64_t p = perm[q];
+      const double wp = w ? (double)w[p] : 1.0;
+      if (wp == 0.0) continue;
+      for (uint32_t f = 0; f < R; ++f) h[f] = 1.0;
+      for (uint32_t m = 0; m < t->nmodes; ++m) {
+        if (m == mode) continue;
+        const double *row = F[m] + (size_t)t->ind[m][p] * R;
+        for (uint32_t f = 0; f < R; ++f) h[f] *= row[f];
+      }
+      double e = trunc_vals ? trunc(t->vals[p]) : t->vals[p];
+      for (uint32_t f = 0; f < R; ++f) e -= a[f] * h[f];
+      for (uint32_t f = 0; f < R; ++f) {
+        num[f] += wp * e * h[f];
+        den[f] += wp * h[f] * h[f];
+      }
+    }
+    for (uint32_t f = 0; f < R; ++f) a[f] = num[f] / (lambda + den[f]);
+  }
   return 0;
 }
 

@@ -59,11 +59,23 @@ int orc_mode_order(const orc_coo *t, uint32_t mode, uint32_t *perm, uint64_t *ro
  *     segment that starts at lane h, the head receives the last lane's term
  *     itercounter - (tile_len - 1 - h) extra times, itercounter being the
  *     number of continuation lanes 1..tile_len-1 (SG/als.cu:233).
+ *   flags & 16 (Q3, CCD only): the CCD launch has tilenum / 256 + 1 blocks of
+ *     256 / tile_len warps, one tile each, tilenum = nnz / tile_len + 1
+ *     (SG/ccd.cu:520): the tiles beyond have weight 0.
  * A partial last tile's last lane reads an inactive lane (undefined in the
  * reference); it gets no extra.  w[p] is indexed by the nonzero's position in
  * t.  Returns 0, or 1 on bad arguments. */
 int orc_quirk_weights(const orc_coo *t, uint32_t mode, const uint32_t *perm, uint32_t tile_len,
                       uint32_t flags, uint32_t *w);
+/* The reference's CCD update of one mode in FP64 (SG/ccd.cu:82-206,318-348;
+ * quirk Q4: no add-back, every column at once): for each row i with a nonzero
+ * in t, e_p = x_p - <F_mode(i,:), h_p> with the row's values before the update,
+ * F_mode(i,f) = sum w_p e_p h_pf / (lambda + sum w_p h_pf^2) over the row's
+ * nonzeros (w NULL: weights 1; x truncated toward zero when trunc_vals); rows
+ * without nonzeros are kept.  perm / rowptr: orc_mode_order.  Returns 0, or 1
+ * on bad arguments. */
+int orc_ccd_jacobi_mode(const orc_coo *t, uint32_t mode, uint32_t R, double **F, double lambda,
+                        const uint32_t *w, int trunc_vals, const uint32_t *perm, const uint64_t *rowptr);
 
 /* ---- reference CISS (TB-COO) layout (SG/ciss.c:68-195) --------------------
  * Input: t, already sorted by `mode` (the reference requires it, SG/ciss.c:74).

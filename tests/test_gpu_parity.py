@@ -171,15 +171,86 @@ def test_als_ref_quirks_match_oracle(ctx, R, kernel, flags):
     assert relerr(m.factor(0), F64[0]) < 1e-5
 
 
-def test_ref_quirks_are_for_als_only(ctx):
+def test_ref_quirks_scope(ctx):
+    """SGD takes no quirk bits; CCD's need Q4 (the reference's CCD has only
+    that form); ALS ignores the CCD bits; unknown bits are errors."""
     tr, _ = small_problem(3, nnz=5000, seed=4)
     m = ct.Model(ctx, tr, None, None, rank=16)
-    m.set_factors(init_factors(tr.dims, 16, 1))
-    for alg, call in ((ct.SGD, lambda o: m.sgd_epoch(o, lr=1e-3)), (ct.CCD, lambda o: m.ccd_epoch(o))):
-        with pytest.raises(ct.CtcError, match="ALS reductions only"):
-            call(ct.Options(algorithm=alg, rank=16, lambda_=0.05, ref_quirks=ct.QUIRK_TILE_HEAD))
-    with pytest.raises(ct.CtcError, match="unknown bits"):
-        m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.05, ref_quirks=8))
+    F = init_factors(tr.dims, 16, 1)
+    m.set_factors(F)
+    with pytest.raises(ct.CtcError, match="ALS and CCD kernels only"):
+        m.sgd_epoch(ct.Options(algorithm=ct.SGD, rank=16, lambda_=0.05, ref_quirks=ct.QUIRK_TILE_HEAD), lr=1e-3)
+    with pytest.raises(ct.CtcError, match="NO_ADDBACK"):
+        m.ccd_epoch(ct.Options(algorithm=ct.CCD, rank=16, lambda_=0.05, ref_quirks=ct.QUIRK_TILE_HEAD))
+    for alg in (ct.ALS, ct.CCD):
+        with pytest.raises(ct.CtcError, match="unknown bits"):
+            o = ct.Options(algorithm=alg, rank=16, lambda_=0.05, ref_quirks=32 | ct.QUIRK_NO_ADDBACK)
+            m.als_mode_update(0, o) if alg == ct.ALS else m.ccd_epoch(o)
+    m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=16, lambda_=0.05,
+                                    ref_quirks=ct.QUIRK_NO_ADDBACK | ct.QUIRK_CCD_GRID))
+    F64 = [f.astype(np.float64) for f in F]
+    assert O.als_mode(ocoo(tr), 0, F64, 0.05) == 0
+    assert relerr(m.factor(0), F64[0]) < 1e-5  # the intended ALS update
+
+
+def test_ref_quirks_every_term_dropped(ctx):
+    """20 nonzeros, all on row 0 of mode 0: one partial tile whose only row
+    segment Q1 drops.  The reference then zeroes the ALS factor and solves
+    lambda I a = 0 (every row 0), and its CCD writes 0 / lambda to row 0 and
+    keeps the rows without nonzeros; the oracle agrees on both."""
+    rng = np.random.default_rng(5)
+    n, dims, R = 20, (4, 9, 7), 8
+    ind = [np.zeros(n, np.uint32), rng.integers(0, 9, n).astype(np.uint32), rng.integers(0, 7, n).astype(np.uint32)]
+    vals = rng.uniform(1, 5, n)
+    tr = ct.Coo.from_arrays(ind, vals, dims)
+    oc = O.Coo(ind, vals, dims)
+    F = init_factors(dims, R, 3)
+    m = ct.Model(ctx, tr, None, None, rank=R)
+    m.set_factors(F)
+    m.als_mode_update(0, ct.Options(algorithm=ct.ALS, rank=R, lambda_=0.05, ref_quirks=ct.QUIRK_TILE_HEAD))
+    assert np.all(m.factor(0) == 0)
+    F64 = [f.astype(np.float64) for f in F]
+    assert O.als_mode(O.quirk_multiset(oc, 0, ct.QUIRK_TILE_HEAD), 0, F64, 0.05) == 0
+    assert np.all(F64[0] == 0)
+    m.set_factors(F)
+    m.ccd_epoch(ct.Options(algorithm=ct.CCD, rank=R, lambda_=5.0,
+                           ref_quirks=ct.QUIRK_NO_ADDBACK | ct.QUIRK_TILE_HEAD))
+    F64 = [f.astype(np.float64) for f in F]
+    O.ccd_jacobi_epoch(oc, F64, 5.0, ct.QUIRK_NO_ADDBACK | ct.QUIRK_TILE_HEAD)
+    assert np.all(F64[0][0] == 0) and np.array_equal(F64[0][1:], F[0][1:])
+    for mode in range(3):
+        assert relerr(m.factor(mode), F64[mode]) < 1e-6, mode
+
+
+@pytest.mark.parametrize("R", [8, 32])
+@pytest.mark.parametrize("flags", [ct.QUIRK_NO_ADDBACK,
+                                   ct.QUIRK_NO_ADDBACK | ct.QUIRK_TILE_HEAD | ct.QUIRK_LAST_LANE,
+                                   ct.QUIRK_NO_ADDBACK | ct.QUIRK_TILE_HEAD | ct.QUIRK_LAST_LANE
+                                   | ct.QUIRK_TRUNC_VALUES | ct.QUIRK_CCD_GRID])
+def test_ccd_ref_quirks_match_oracle(ctx, R, flags):
+    """CCD epochs in the reference's form (options.ref_quirks with Q4, and the
+    tile defects Q1/Q2, Q5 values and Q3 grid as asked) against the oracle's
+    FP64 restatement from the same factors, two epochs, 1e-5 per factor;
+    the trajectory differs from CCD++'s."""
+    tr, te = small_problem(3, nnz=30000, seed=120 + R)
+    say(f"[ccd_quirks] R={R} flags={flags}")
+    m = ct.Model(ctx, tr, None, te, rank=R)
+    F = init_factors(tr.dims, R, 9)
+    m.set_factors(F)
+    oc = ocoo(tr)
+    F64 = [f.astype(np.float64) for f in F]
+    opt = ct.Options(algorithm=ct.CCD, rank=R, lambda_=5.0, ref_quirks=flags)
+    for ep in range(2):
+        m.ccd_epoch(opt)
+        O.ccd_jacobi_epoch(oc, F64, 5.0, flags)
+        for mode in range(3):
+            e = relerr(m.factor(mode), F64[mode])
+            assert e < 1e-5, (ep, mode, e)
+    plain = ct.Model(ctx, tr, None, te, rank=R)
+    plain.set_factors(F)
+    plain.ccd_epoch(ct.Options(algorithm=ct.CCD, rank=R, lambda_=5.0))
+    plain.ccd_epoch(ct.Options(algorithm=ct.CCD, rank=R, lambda_=5.0))
+    assert relerr(m.factor(0), plain.factor(0)) > 1e-3
 
 
 def test_als_deterministic(ctx):

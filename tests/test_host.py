@@ -342,7 +342,7 @@ def test_context_without_gpu_reports_error():
     assert e.value.status in (6, 9)
 
 
-@pytest.mark.parametrize("flags", [1, 2, 3])
+@pytest.mark.parametrize("flags", [1, 2, 3, 16, 19])
 def test_ref_quirk_weights_equal_oracle(flags):
     """The library's weights of the reference's tile-reduction defects
     (ctc_ref_quirk_weights) equal the oracle's, which the lane-by-lane walk
@@ -354,4 +354,4 @@ def test_ref_quirk_weights_equal_oracle(flags):
     for mode in range(3):
         w = tr.ref_quirk_weights(mode, flags)
         assert np.array_equal(w, O.quirk_weights(oc, mode, flags)), mode
-        assert (w == 0).any() if flags & 1 else (w >= 1).all()
+        assert (w == 0).any() if flags & 17 else (w >= 1).all()

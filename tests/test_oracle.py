@@ -882,3 +882,116 @@ def test_quirk_multiset_truncates_and_repeats():
     assert list(q.vals) == [2.0, 3.0, -1.0] and q.nnz == 3
     q = O.quirk_multiset(t, 0, O.QUIRK_TILE_HEAD)  # one partial tile: its first segment (row 0) dropped
     assert q.nnz == 1 and list(q.ind[0]) == [1]
+
+
+def test_quirk_ccd_grid_weights():
+    """Q3: the CCD launch has tilenum / 256 + 1 blocks of 8 warps, one tile of
+    32 each, tilenum = nnz / 32 + 1 (SG/ccd.cu:520).  1000 nonzeros: 32 tiles,
+    one block, 8 tiles = the first 256 nonzeros of the mode order; 100
+    nonzeros: all 4 tiles covered; 70000: 2188 tiles, 9 blocks, 72 tiles."""
+    rng = np.random.default_rng(12)
+    for n, kept in ((1000, 256), (100, 100), (70000, 72 * 32)):
+        t = O.Coo([rng.integers(0, 50, n), rng.integers(0, 9, n), rng.integers(0, 4, n)], np.ones(n), (50, 9, 4))
+        perm, _ = O.mode_order(t, 0)
+        w = O.quirk_weights(t, 0, O.QUIRK_CCD_GRID)
+        assert list(w[perm]) == [1] * kept + [0] * (n - kept), n
+        # with Q1 as well, both apply: the dropped tiles stay dropped
+        w2 = O.quirk_weights(t, 0, O.QUIRK_CCD_GRID | O.QUIRK_TILE_HEAD)
+        assert np.all(w2[perm][kept:] == 0) and np.array_equal(w2[perm][:kept], O.quirk_weights(t, 0, 1)[perm][:kept])
+
+
+def test_ccd_jacobi_hand_worked():
+    """R = 1, row 0 holds (0,0,0) = 3 and (0,1,0) = 5, B = [1, 2], C = [1],
+    a = 0.5, lambda = 1: h = (1, 2), e = (3 - 0.5, 5 - 1) = (2.5, 4), num =
+    2.5 + 8 = 10.5, den = 1 + 4 = 5, a <- 10.5 / 6 = 1.75.  Row 1 has no
+    nonzero and keeps its value; with Q1 the single partial tile's first (and
+    only) segment is dropped, so row 0 becomes 0 / (1 + 0) = 0."""
+    t = O.Coo([[0, 0], [0, 1], [0, 0]], [3.0, 5.0], (2, 2, 1))
+    F = [np.array([[0.5], [7.0]]), np.array([[1.0], [2.0]]), np.array([[1.0]])]
+    O.ccd_jacobi_mode(t, 0, F, 1.0)
+    assert F[0][0, 0] == 1.75 and F[0][1, 0] == 7.0
+    F[0][0, 0] = 0.5
+    O.ccd_jacobi_mode(t, 0, F, 1.0, O.QUIRK_TILE_HEAD)
+    assert F[0][0, 0] == 0.0 and F[0][1, 0] == 7.0
+
+
+def _reference_ccd_mode(t, mode, F, lam, q1, q2, q3, trunc):
+    """update_frac_gpu_as + update_ccd_gpu (SG/ccd.cu:82-206,318-348) run lane
+    by lane: in the mode's sorted order each lane computes h and localtbuffer[2]
+    = x - <a_row, h> from the factors before the update; per column m the
+    terms h_m^2 and h_m * e go through the tile walk; the launch covers
+    tilenum / 256 + 1 blocks of 8 tiles under Q3 (else every tile); then every
+    row with a nonzero takes num / (lambda + den), 0 / lambda if nothing
+    reached it."""
+    perm, _ = O.mode_order(t, mode)
+    n, R = t.nnz, F[0].shape[1]
+    others = [m for m in range(t.nmodes) if m != mode]
+    rows = [int(t.ind[mode][p]) for p in perm]
+    h = np.ones((n, R))
+    for m in others:
+        h *= F[m][np.asarray(t.ind[m])[perm]]
+    x = np.asarray(t.vals, np.float64)[perm]
+    if trunc:
+        x = np.trunc(x)
+    e = x - np.einsum("pr,pr->p", F[mode][rows], h)
+    if q3:
+        covered = ((n // 32 + 1) // 256 + 1) * 8 * 32
+        rows, h, e = rows[:covered], h[:covered], e[:covered]
+    new = F[mode].copy()
+    for r in set(int(i) for i in t.ind[mode]):
+        new[r] = 0.0
+    for m in range(R):
+        den = _reference_tile_walk(rows, list(h[:, m] * h[:, m]), q1, q2)
+        num = _reference_tile_walk(rows, list(h[:, m] * e), q1, q2)
+        for r in set(int(i) for i in t.ind[mode]):
+            new[r, m] = num.get(r, 0.0) / (lam + den.get(r, 0.0))
+    F[mode][:] = new
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3, 7, 16, 19, 23])
+def test_ccd_jacobi_matches_the_lane_by_lane_kernel(flags):
+    """orc_ccd_jacobi_mode with the quirk weights equals the lane-by-lane
+    simulation of the reference's CCD kernels for every combination of Q1,
+    Q2, Q3, Q5, on tensors with rows across tiles and more tiles than the
+    CCD grid covers, every mode, to 1e-12."""
+    rng = np.random.default_rng(300 + flags)
+    q1, q2, q3, tr = bool(flags & 1), bool(flags & 2), bool(flags & 16), bool(flags & 4)
+    for trial in range(3):
+        dims = (int(rng.integers(5, 30)), int(rng.integers(4, 12)), 6)
+        n = int(rng.choice([70, 400, 9000]))
+        lens = rng.multinomial(n, rng.dirichlet(np.ones(dims[0]) * 0.3))
+        rows = np.repeat(np.arange(dims[0]), lens)
+        t = O.Coo([rows, rng.integers(0, dims[1], n), rng.integers(0, dims[2], n)],
+                  rng.normal(2.0, 1.5, n), dims)
+        R = 3
+        F0 = [rng.uniform(-1, 1, (d, R)) for d in dims]
+        for mode in range(3):
+            want = [f.copy() for f in F0]
+            _reference_ccd_mode(t, mode, want, 2.0, q1, q2, q3, tr)
+            got = [f.copy() for f in F0]
+            O.ccd_jacobi_mode(t, mode, got, 2.0, flags | O.QUIRK_NO_ADDBACK)
+            np.testing.assert_allclose(got[mode], want[mode], rtol=1e-12, atol=1e-12)
+            for m in range(3):
+                if m != mode:
+                    assert np.array_equal(got[m], F0[m])
+
+
+def test_ccd_jacobi_is_a_ridge_step_on_the_residual():
+    """Without weights the update equals (m - G a) / (lambda + diag G) per row,
+    with m the MTTKRP and G the Gram of the oracle's ALS path (the row's own
+    value enters through G a: no add-back)."""
+    rng = np.random.default_rng(8)
+    n, dims, R = 3000, (40, 30, 20), 4
+    t = O.Coo([rng.integers(0, d, n) for d in dims], rng.normal(0, 1, n), dims)
+    F = [rng.uniform(-1, 1, (d, R)) for d in dims]
+    mode = 1
+    perm, rowptr = O.mode_order(t, mode)
+    m = O.mttkrp(t, mode, F)
+    G = O.gram_rows(t, mode, F)
+    a = F[mode]
+    want = a.copy()
+    has = np.diff(rowptr.astype(np.int64)) > 0
+    ga = np.einsum("irs,is->ir", G, a)
+    want[has] = (m[has] - ga[has]) / (0.7 + np.einsum("irr->ir", G)[has])
+    O.ccd_jacobi_mode(t, mode, F, 0.7)
+    np.testing.assert_allclose(F[mode], want, rtol=1e-10, atol=1e-12)
