@@ -189,10 +189,12 @@ struct ctc_model {
   float *sgd_stage[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_sbase[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_slots[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_dest[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_piece_start[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t *sgd_piece_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   double *sgd_part[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   ctc::FoldSeg *sgd_fold[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint64_t> sgd_slot_off[CTC_MAX_MODES];  // first staging position of each sub-step
   std::vector<uint64_t> sgd_piece_off[CTC_MAX_MODES];
   std::vector<uint64_t> sgd_fold_off[CTC_MAX_MODES];
   uint32_t *sgd_long = nullptr;

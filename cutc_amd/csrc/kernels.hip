@@ -346,17 +346,23 @@ __global__ void __launch_bounds__(256) k_sgd_sync(CissArgs c, FactorArgs fa, Syn
   }
   uint32_t slot = 0;
   bool own0 = false;  // the current mode-0 row is stored in place (no other walker meets it)
-  // modes 1..: the next staging slot of this chunk's runs
-  uint32_t rslot[N];
+  // modes 1..: the next slot of this chunk's runs and, on the mixed path, its
+  // staging position, read one run ahead (the table has one entry of padding)
+  uint32_t rslot[N], rpos[N];
 #pragma unroll
-  for (int k = 0; k < N; ++k)
+  for (int k = 0; k < N; ++k) {
     rslot[k] = (k >= 1 && walker < nspans) ? sy.sbase[k][(size_t)s * nspans + walker] : 0u;
+    rpos[k] = (MIX && k >= 1) ? sy.dest[k][rslot[k]] : 0u;
+  }
   auto hand_in = [&](int q) {
-    if (q >= 1) {  // the run's delta goes to its staging slot
-      if constexpr (MIX)
-        store_bf4_nt<CPL>(reinterpret_cast<uint16_t *>(sy.stage[q]) + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
-      else
+    if (q >= 1) {  // the run's delta goes to its staging slot / position
+      if constexpr (MIX) {
+        const uint32_t pos = rpos[q];
+        rpos[q] = sy.dest[q][++rslot[q]];
+        store_bf4_nt<CPL>(reinterpret_cast<uint16_t *>(sy.stage[q]) + (size_t)pos * R + wl * CPL, del[q]);
+      } else {
         store_row4_nt<CPL>(sy.stage[q] + (size_t)rslot[q]++ * R + wl * CPL, del[q]);
+      }
       return;
     }
     if (own0) {
@@ -521,8 +527,8 @@ __device__ inline void sgd_merge_block(const MergeArgs &a, uint64_t blk) {
 }
 
 // One launch per sub-step settles every piece of the staged modes: a
-// workgroup per piece, whose walkers take
-// the piece's slots in turn (FP64 partial sums) and add their partials in
+// workgroup per piece, whose walkers take the piece's positions in turn
+// (FP64 partial sums) and add their partials in
 // walker order.  A piece that holds a whole row applies it; the others leave
 // their sums for k_sgd_fold.  The order is fixed, so the result is
 // reproducible.
@@ -541,9 +547,12 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   const uint64_t gp = a.piece0[k] + (pblk - a.pb[k]);
   const int wid = threadIdx.x / W;
   const int wl = threadIdx.x % W;
-  const uint32_t *slots = a.slots[k];
-  const float *stage = a.stage[k];
   const uint32_t b = a.piece_start[k][gp], e = a.piece_start[k][gp + 1];
+  const float *stage = a.stage[k];
+  const uint32_t s0 = (uint32_t)a.slot0[k];  // positions are numbered over the epoch
+  const uint32_t *slots = a.slots[k];
+  // the staged delta at position q: FP32 at its slot, BF16 at q itself
+  auto at = [&](uint32_t q) -> size_t { return MIX ? (size_t)(q - s0) : (size_t)slots[q]; };
   // a whole row's factor entry and hot bit, read while the slots are summed
   const uint32_t row = a.piece_row[k][gp];
   float fv = 0.f;
@@ -561,9 +570,9 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if constexpr (MIX)
-        load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+        load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + at(p + u * NW) * R + wl * CPL, x[u]);
       else
-        load_row4_nt<CPL>(stage + (size_t)slots[p + u * NW] * R + wl * CPL, x[u]);
+        load_row4_nt<CPL>(stage + at(p + u * NW) * R + wl * CPL, x[u]);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -573,9 +582,9 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   for (; p < e; p += NW) {
     float x[CPL];
     if constexpr (MIX)
-      load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + (size_t)slots[p] * R + wl * CPL, x);
+      load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + at(p) * R + wl * CPL, x);
     else
-      load_row4_nt<CPL>(stage + (size_t)slots[p] * R + wl * CPL, x);
+      load_row4_nt<CPL>(stage + at(p) * R + wl * CPL, x);
 #pragma unroll
     for (int j = 0; j < CPL; ++j) acc[j] += (double)x[j];
   }

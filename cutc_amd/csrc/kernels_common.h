@@ -62,17 +62,24 @@ struct SyncArgs {
   uint32_t *ch[4];    // hot rows x kSyncReplicas
   float *stage[4];    // modes 1..: runs of one sub-step x R (BF16 entries on the mixed path)
   const uint32_t *sbase[4];  // modes 1..: first slot of each (sub-step, walker) chunk
+  const uint32_t *dest[4];   // modes 1.., mixed path: the sub-step's staging position of each slot (padded by one)
 };
 
 // Staged merge of one sub-step (k_sgd_settle + k_sgd_fold).  Per staged
-// mode k, the sub-step's runs are grouped by row (slots[] lists each row's
-// staging slots in walk order) and every row's slots are cut into pieces of
-// at most kStagePiece slots: piece p covers slots[piece_start[p] ..
-// piece_start[p + 1]).  A row that fits one piece is settled by that piece
-// (piece_row[p] = the row); the pieces of a longer row (piece_row[p] =
-// 0xFFFFFFFF) leave FP64 partial sums in part[p - piece0], and k_sgd_fold adds
-// them in order for each such row (fold[g]).  A cold row adds the sum of its
-// runs' deltas, a hot row their mean.
+// mode k, the sub-step's runs are numbered in walk order (slots) and listed
+// by row, each row's in walk order: position q of the list is slot
+// slots[q].  FP32 deltas are staged at their slot (a walker writes one
+// contiguous range; a 128-byte delta is a whole cache line wherever the merge
+// reads it); BF16 deltas at their position in the list (dest[slot] = q), so
+// that the merge reads a row's runs as one contiguous range and no 128-byte
+// line is shared by the runs of two rows.  Every row's list is cut into
+// pieces of at most kStagePiece runs: piece p covers positions
+// piece_start[p] .. piece_start[p + 1] (numbered over the epoch; the
+// sub-step's first is slot0).  A row that fits one piece is settled by that
+// piece (piece_row[p] = the row); the pieces of a longer row (piece_row[p] =
+// 0xFFFFFFFF) leave FP64 partial sums in part[p - piece0], and k_sgd_fold
+// adds them in order for each such row (fold[g]).  A cold row adds the sum of
+// its runs' deltas, a hot row their mean.
 constexpr uint32_t kStagePiece = 1024;
 struct FoldSeg {
   uint32_t row, piece, npiece, runs;  // piece: first piece, sub-step relative
@@ -102,7 +109,8 @@ struct SettleArgs {
   const float *stage[4];
   const uint32_t *piece_start[4];
   const uint32_t *piece_row[4];
-  const uint32_t *slots[4];
+  const uint32_t *slots[4];  // FP32: the (sub-step relative) slot at each position
+  uint64_t slot0[4];       // the sub-step's first position
   double *part[4];
   const FoldSeg *fold[4];  // the sub-step's multi-piece rows
   uint64_t piece0[4];      // the sub-step's first piece

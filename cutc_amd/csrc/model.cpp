@@ -441,6 +441,7 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->sgd_D[md]);
     dev_free(m->sgd_stage[md]);
     dev_free(m->sgd_slots[md]);
+    dev_free(m->sgd_dest[md]);
     dev_free(m->sgd_sbase[md]);
     dev_free(m->sgd_piece_start[md]);
     dev_free(m->sgd_piece_row[md]);
@@ -733,19 +734,20 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
   const DevCiss &d = m->ciss[0];
   // Modes 1.. hand in each run's delta to a staging slot.  The runs of a
   // walker's chunk (walker w, sub-step s) are the maximal runs of equal index
-  // in it, numbered in walk order from base[s][w] (the runs of the walkers
-  // before w in the same sub-step); per sub-step the slots are grouped by row
-  // in slot (= walk) order for the merge.
+  // in it, numbered in walk order (slots) from base[s][w] (the runs of the
+  // walkers before w in the same sub-step); per sub-step the slots are listed
+  // by row in slot (= walk) order for the merge: position q holds slot
+  // slots[q], and dest[slot] = q (the mixed path stages at q).
   const uint64_t nnz = d.nnz, nspans = (nnz + span - 1) / span, nsub = (span + sub - 1) / sub;
   for (uint32_t k = 1; k < m->nmodes; ++k) {
-    for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_slots[k], (void *)m->sgd_sbase[k],
+    for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_slots[k], (void *)m->sgd_dest[k], (void *)m->sgd_sbase[k],
                     (void *)m->sgd_piece_start[k], (void *)m->sgd_piece_row[k], (void *)m->sgd_part[k],
                     (void *)m->sgd_fold[k]})
       dev_free(p);
     m->sgd_stage[k] = nullptr;
     m->sgd_part[k] = nullptr;
     m->sgd_fold[k] = nullptr;
-    m->sgd_slots[k] = m->sgd_sbase[k] = m->sgd_piece_start[k] = m->sgd_piece_row[k] = nullptr;
+    m->sgd_slots[k] = m->sgd_dest[k] = m->sgd_sbase[k] = m->sgd_piece_start[k] = m->sgd_piece_row[k] = nullptr;
     std::vector<uint32_t> idx(nnz);
     CTC_HIP(hipMemcpy(idx.data(), d.oidx[k - 1], nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
     auto starts_run = [&](uint64_t p) { return (p % span) % sub == 0 || idx[p] != idx[p - 1]; };
@@ -786,6 +788,7 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
     // every row's slots of a sub-step cut into pieces of at most kStagePiece
     // slots; a row of one piece is settled by it, the others are folded
     std::vector<uint32_t> piece_start, piece_row, slots(key.size());
+    std::vector<uint32_t> dest(key.size() + 1, 0);  // + 1: the walk reads one ahead
     std::vector<FoldSeg> fold;
     std::vector<uint64_t> &poff = m->sgd_piece_off[k], &foff = m->sgd_fold_off[k];
     poff.assign(nsub + 1, 0);
@@ -805,7 +808,10 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
           piece_start.push_back((uint32_t)(q + t * kStagePiece));
           piece_row.push_back(np == 1 ? row : 0xFFFFFFFFu);
         }
-        for (uint64_t t = q; t < e; ++t) slots[t] = (uint32_t)key[t];
+        for (uint64_t t = q; t < e; ++t) {
+          slots[t] = (uint32_t)key[t];
+          dest[boff[s2] + slots[t]] = (uint32_t)(t - boff[s2]);
+        }
         q = e;
       }
       max_pieces = std::max<uint64_t>(max_pieces, piece_start.size() - poff[s2]);
@@ -817,6 +823,8 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
     CTC_TRY(upload(&m->sgd_piece_row[k], piece_row.data(), piece_row.size(), st));
     CTC_TRY(upload(&m->sgd_fold[k], fold.data(), fold.size(), st));
     CTC_TRY(upload(&m->sgd_slots[k], slots.data(), slots.size(), st));
+    CTC_TRY(upload(&m->sgd_dest[k], dest.data(), dest.size(), st));
+    m->sgd_slot_off[k] = boff;
     CTC_TRY(upload(&m->sgd_sbase[k], base.data(), base.size(), st));
     CTC_TRY(dev_alloc(&m->sgd_stage[k], (size_t)std::max<uint64_t>(1, max_runs) * m->rank));
     CTC_TRY(dev_alloc(&m->sgd_part[k], (size_t)std::max<uint64_t>(1, max_pieces) * m->rank));
@@ -897,6 +905,7 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
   const uint32_t nsub = (span + sub - 1) / sub;
   Prof pe(m->ctx, "sgd");
   for (uint32_t s = 0; s < nsub; ++s) {
+    for (uint32_t k = 1; k < N; ++k) sy.dest[k] = m->sgd_dest[k] + m->sgd_slot_off[k][s];
     {
       Prof p(m->ctx, "sgd_walk");
       CTC_HIP(launch_sgd_sync(st, c, f, sy, (int)N, (int)R, span, sub, s, span - sub + 2, step, (float)o->lambda,
@@ -907,6 +916,7 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
       for (uint32_t k = 0; k < 4; ++k) {
         const bool staged = k >= 1 && k < N;
         sa.piece0[k] = staged ? m->sgd_piece_off[k][s] : 0;
+        sa.slot0[k] = staged ? m->sgd_slot_off[k][s] : 0;
         sa.fold[k] = staged ? m->sgd_fold[k] + m->sgd_fold_off[k][s] : nullptr;
         sa.pb[k + 1] = sa.pb[k] + (staged ? m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s] : 0);
         sa.fb[k + 1] = sa.fb[k] + (staged ? m->sgd_fold_off[k][s + 1] - m->sgd_fold_off[k][s] : 0);
