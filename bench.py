@@ -64,9 +64,16 @@ def cfg2_tensor(ct, nnz, seed=0x5EED0002):
                     noise_std=0.3, test_fraction=0.01, seed=seed)
 
 
+# The SGD epochs time every SGD_SAMPLE-th walk and merge launch with HIP events
+# (a number prime to the 64 sub-steps of an epoch, so the sampled sub-step
+# moves from epoch to epoch): an event is a packet between two launches, and
+# bracketing all 192 launches of an epoch lengthened it by 6%.
+SGD_SAMPLE = 17
+
+
 def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None):
     """SGD epochs with the default schedule; -> (ms list, trajectory, walk ms per launch, walk launches per epoch,
-    options)."""
+    merge ms per epoch, options)."""
     import numpy as np  # noqa: F401
     model.set_factors(F0)
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3, lr_schedule=ct.LR_EXP_DECAY,
@@ -75,9 +82,12 @@ def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None):
     model.prepare(opt)
     model.sgd_epoch(opt, lr=0.0)  # builds the schedule's tables (lr 0: factors unchanged)
     log(f"SGD set-up {time.time() - t0:.1f}s")
+    c = opt.to_c()
+    walks = -(-c.tile_len // c.sgd_substep)  # sub-steps per epoch: one walk and one merge each
     lr, times, traj = 1e-3, [], []
     for e in range(warmup + steps):
         if e == warmup:
+            ctx.set_profiling(True, every=SGD_SAMPLE)
             ctx.reset_stats()
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -90,7 +100,8 @@ def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None):
         traj.append(model.evaluate(lam)["rmse_te"])  # untimed
     n_w, ms_w = ctx.kernel_stats("sgd_walk")
     n_m, ms_m = ctx.kernel_stats("sgd_merge")
-    return times, traj, ms_w / max(n_w, 1), n_w / max(steps, 1), ms_m / max(steps, 1), opt
+    ctx.set_profiling(True)  # every launch again, for the other legs
+    return times, traj, ms_w / max(n_w, 1), walks, ms_m / max(n_m, 1) * walks, opt
 
 
 def sgd_mixed_leg(ct, ctx, model, nnz, R, lam, F0, steps, warmup, fp32, full):

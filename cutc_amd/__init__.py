@@ -302,8 +302,10 @@ class Context:
     def synchronize(self):
         check(lib().ctc_context_synchronize(self._h))
 
-    def set_profiling(self, on=True):
-        check(lib().ctc_context_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on=True, every=1):
+        """Time kernels with HIP events: every launch, or every `every`-th
+        launch of each kernel name (a sample that barely perturbs the run)."""
+        check(lib().ctc_context_set_profiling(self._h, int(every) if on else 0))
 
     def kernel_stats(self, name):
         n = C.c_uint64()

@@ -122,6 +122,8 @@ struct ctc_context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool profiling = false;
+  uint32_t prof_every = 1;                  // time every n-th launch of each name
+  std::map<std::string, uint64_t> prof_seen;  // launches of each name since profiling was set
   std::map<std::string, KernelStat> stats;
   // pending profiled launches: name, start, stop
   struct Pending {

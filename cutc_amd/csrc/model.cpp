@@ -62,9 +62,10 @@ double median(std::vector<double> v) {
 namespace ctc {
 
 // ------------------------------------------------------------- profiling --
-void prof_begin(ctc_context *c, const char *, hipEvent_t *a, hipEvent_t *b, hipStream_t s) {
+void prof_begin(ctc_context *c, const char *name, hipEvent_t *a, hipEvent_t *b, hipStream_t s) {
   *a = *b = nullptr;
   if (!c->profiling) return;
+  if (c->prof_every > 1 && c->prof_seen[name]++ % c->prof_every != 0) return;  // not sampled
   if (c->event_pool.size() < 2) {
     // timing only: no system-scope fence (cache write-back and invalidation)
     // between the launches they bracket
@@ -297,9 +298,11 @@ extern "C" ctc_status ctc_context_synchronize(ctc_context *c) {
 }
 
 extern "C" ctc_status ctc_context_set_profiling(ctc_context *c, int enabled) {
-  if (!c) return fail(CTC_ERR_INVALID_ARG, "null context");
-  if (!enabled) CTC_TRY(prof_flush(c));
+  if (!c || enabled < 0) return fail(CTC_ERR_INVALID_ARG, "ctc_context_set_profiling");
+  CTC_TRY(prof_flush(c));
   c->profiling = enabled != 0;
+  c->prof_every = enabled > 1 ? (uint32_t)enabled : 1u;
+  c->prof_seen.clear();
   return CTC_OK;
 }
 

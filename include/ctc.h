@@ -177,7 +177,11 @@ void ctc_context_destroy(ctc_context *ctx);
 ctc_status ctc_context_stream(ctc_context *ctx, void **stream);
 /* Waits for all work queued on the context's stream. */
 ctc_status ctc_context_synchronize(ctc_context *ctx);
-/* Per-kernel timing with HIP events on the context stream (off by default). */
+/* Per-kernel timing with HIP events on the launch stream (off by default).
+ * enabled = 1 times every launch; n > 1 times every n-th launch of each
+ * kernel name only (counted from this call), so that the events, each a
+ * packet between two launches, barely perturb a run of short kernels; the
+ * statistics then hold the sampled launches.  Negative values are rejected. */
 ctc_status ctc_context_set_profiling(ctc_context *ctx, int enabled);
 ctc_status ctc_context_kernel_stats(ctc_context *ctx, const char *kernel, uint64_t *launches,
                                     double *total_ms);

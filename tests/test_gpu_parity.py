@@ -75,6 +75,25 @@ def test_mttkrp(ctx, R, nmodes):
         assert relerr(got, exp) < 1e-5, (mode, relerr(got, exp))
 
 
+def test_sampled_profiling():
+    """ctc_context_set_profiling(n > 1) times every n-th launch of each kernel
+    name from the call on; 1 every launch; negative values are errors."""
+    c = ct.Context(0)
+    tr, _ = small_problem(3, seed=2)
+    m = ct.Model(c, tr, None, None, rank=16)
+    m.set_factors(init_factors(tr.dims, 16, 1))
+    for every, want in ((3, 4), (1, 10), (4, 3)):
+        c.set_profiling(True, every=every)
+        c.reset_stats()
+        for _ in range(10):
+            m.mttkrp(0)
+        n, ms = c.kernel_stats("mttkrp")
+        assert n == want and ms > 0, (every, n, ms)
+    c.set_profiling(False)
+    with pytest.raises(ct.CtcError):
+        c.set_profiling(True, every=-2)
+
+
 @pytest.mark.parametrize("nmodes", [3, 4])
 def test_evaluate(ctx, nmodes):
     tr, te = small_problem(nmodes, seed=9)
