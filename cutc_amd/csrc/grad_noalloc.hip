@@ -76,8 +76,7 @@ extern "C" ctc_status ctc_grad_noalloc(ctc_model *m, const float *const *factors
   }
   // walkers of `span` nonzeros, span chosen so that each walk has at most
   // kGradMaxBlocks blocks of partial sums
-  const int W = R < 64 ? (int)R : 64;
-  const uint64_t per_block = (256 / 64) * (64 / W);  // walkers per 256-thread block
+  const uint64_t per_block = mttkrp_walkers_per_block((int)R, 256);
   uint64_t span = 256;
   while ((m->train_nnz + span - 1) / span > kGradMaxBlocks * per_block) span *= 2;
   const uint64_t nb = std::max<uint64_t>(mttkrp_grid(m->train_nnz, (int)R, (uint32_t)span, 256), kRegBlocks);

@@ -665,9 +665,12 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
                                                 float *out, double *loss_part, float *den) {
   constexpr bool ADJ = KIND != kWalkMttkrp;  // w is the residual
   constexpr bool SQ = KIND == kWalkJacobi;
-  constexpr int W = R < 64 ? R : 64;
-  constexpr int CPL = R / W;
-  constexpr int WPW = 64 / W;
+  // a walker is R / 4 lanes, each holding 4 consecutive entries of a row, so
+  // a gathered row is one 16-byte load per lane and a wave carries 256 / R
+  // walkers (the shape of the SGD walk)
+  constexpr int W = SyncShape<R>::W;
+  constexpr int CPL = SyncShape<R>::CPL;
+  constexpr int WPW = SyncShape<R>::WPW;
   constexpr uint32_t NONE = 0xFFFFFFFFu;
   __shared__ double red[4];
   const int lane = lane_id();
@@ -684,6 +687,14 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
   uint32_t arow = NONE, slot = 0;
 #pragma unroll
   for (int q = 0; q < CPL; ++q) acc[q] = 0.f, own[q] = 0.f, sq[q] = 0.f;
+  auto flush = [&]() {
+    float *o = out + (size_t)arow * R + wl * CPL;
+#pragma unroll
+    for (int q2 = 0; q2 < CPL; ++q2) {
+      atomic_add_f32(o + q2, KIND == kWalkAdjoint ? -acc[q2] : acc[q2]);
+      if (SQ) atomic_add_f32(den + (size_t)arow * R + wl * CPL + q2, sq[q2]);
+    }
+  };
 
   if (!wave_idle) {
     for (uint32_t ch = 0; ch < span; ch += W) {
@@ -702,6 +713,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
         for (int s = 0; s < N - 1; ++s) il[s] = 0;
       }
       const uint64_t hm = __ballot(vl && hl);
+#pragma unroll 1
       for (int q = 0; q < W; ++q) {
         const int src = sub * W + q;
         uint32_t idx[N - 1];
@@ -713,30 +725,24 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
           const bool head = (hm >> src) & 1ull;
           if (arow == NONE || head) {
             if (arow != NONE) {
-#pragma unroll
-              for (int q2 = 0; q2 < CPL; ++q2) {
-                atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, KIND == kWalkAdjoint ? -acc[q2] : acc[q2]);
-                if (SQ) atomic_add_f32(den + (size_t)arow * R + wl + q2 * W, sq[q2]);
-              }
+              flush();
               ++slot;
             } else {
               slot = c.tile_slot[s0 >> 6];
             }
             arow = c.directory[slot];
 #pragma unroll
-            for (int q2 = 0; q2 < CPL; ++q2) {
-              acc[q2] = 0.f;
-              sq[q2] = 0.f;
-              if (ADJ) own[q2] = fa.f[0][(size_t)arow * R + wl + q2 * W];
-            }
+            for (int q2 = 0; q2 < CPL; ++q2) acc[q2] = 0.f, sq[q2] = 0.f;
+            if (ADJ) load_row4<CPL>(fa.f[0] + (size_t)arow * R + wl * CPL, own);
           }
           float h[CPL];
+          load_row4<CPL>(fa.f[1] + (size_t)idx[0] * R + wl * CPL, h);
 #pragma unroll
-          for (int q2 = 0; q2 < CPL; ++q2) {
-            float v = fa.f[1][(size_t)idx[0] * R + wl + q2 * W];
+          for (int s = 1; s < N - 1; ++s) {
+            float g[CPL];
+            load_row4<CPL>(fa.f[1 + s] + (size_t)idx[s] * R + wl * CPL, g);
 #pragma unroll
-            for (int s = 1; s < N - 1; ++s) v *= fa.f[1 + s][(size_t)idx[s] * R + wl + q2 * W];
-            h[q2] = v;
+            for (int q2 = 0; q2 < CPL; ++q2) h[q2] *= g[q2];
           }
           float w = x;
           if (ADJ) {
@@ -754,13 +760,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
         }
       }
     }
-    if (arow != NONE) {
-#pragma unroll
-      for (int q2 = 0; q2 < CPL; ++q2) {
-        atomic_add_f32(out + (size_t)arow * R + wl + q2 * W, KIND == kWalkAdjoint ? -acc[q2] : acc[q2]);
-        if (SQ) atomic_add_f32(den + (size_t)arow * R + wl + q2 * W, sq[q2]);
-      }
-    }
+    if (arow != NONE) flush();
   }
   if (KIND == kWalkAdjoint) {
     double t = wave_sum_d(lsum);
@@ -1441,10 +1441,7 @@ hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R, bool mixed) {
 hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block) {
-  const uint64_t walkers = (c.nnz + span - 1) / span;
-  const int W = R < 64 ? R : 64;
-  const uint64_t waves = (walkers + (64 / W) - 1) / (64 / W);
-  const unsigned grid = grid_for(waves, block);
+  const unsigned grid = (unsigned)mttkrp_grid(c.nnz, R, span, block);
   *grid_out = grid;
   if (c.nnz == 0) return hipSuccess;
   float *none = nullptr;
@@ -1481,10 +1478,12 @@ hipError_t launch_ccd_jacobi(hipStream_t st, const CissArgs &c, const FactorArgs
   return hipGetLastError();
 }
 
+uint64_t mttkrp_walkers_per_block(int R, unsigned block) { return (uint64_t)(block / 64) * (64 / (R / 4)); }
+
 uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block) {
   const uint64_t walkers = (nnz + span - 1) / span;
-  const int W = R < 64 ? R : 64;
-  const uint64_t waves = (walkers + (64 / W) - 1) / (64 / W);
+  const uint64_t wpw = 64 / (R / 4);  // walkers per wave (SyncShape<R>)
+  const uint64_t waves = (walkers + wpw - 1) / wpw;
   return grid_for(waves, block);
 }
 

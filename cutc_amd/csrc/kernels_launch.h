@@ -22,6 +22,8 @@ hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);
 uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block);
+// walkers of k_mttkrp per workgroup of `block` threads (R / 4 lanes each)
+uint64_t mttkrp_walkers_per_block(int R, unsigned block);
 // the reference's CCD sums of one mode (SG/ccd.cu:157-195) over the CISS c:
 // num += sum of e h, den += sum of h * h per row and column, e the residual
 // of the factors as they are; num / den are zeroed dims x R arrays
