@@ -203,6 +203,35 @@ def test_cli_solves_from_five_tns_files(tmp_path):
     assert summary["rmse_te"] == pytest.approx(O.rmse(ote, F), rel=2e-5)
 
 
+def test_cli_reference_ccd_quirks(tmp_path):
+    """`ctc ... --alg ccd --ref-quirks 11` (Q1, Q2 and the reference's CCD
+    form Q4): the printed test RMSE against the oracle's restatement of the
+    reference's CCD epochs (orc_ccd_jacobi_mode, modes 0, 1, 2) from the same
+    init."""
+    full, _ = ct.synth((300, 200, 60), 20000, zipf=(1.0, 0.7, 0.0), true_rank=4, value_map=ct.VALUES_AFFINE_1_5,
+                       noise_std=0.1, test_fraction=0.0, seed=65)
+    src = tmp_path / "t.tns"
+    full.write(str(src), base=1, decimals=-1)
+    exe = os.path.join(ROOT, "cutc_amd", "ctc")
+    subprocess.run([exe, "prep", str(src), str(tmp_path / "d"), "--seed", "9", "--decimals", "-1"], check=True)
+    files = [str(tmp_path / f"d_{s}.tns") for s in ("tr", "tr2", "tr3", "v", "te")]
+    flags = ct.QUIRK_TILE_HEAD | ct.QUIRK_LAST_LANE | ct.QUIRK_NO_ADDBACK
+    r = subprocess.run([exe, *files, "--alg", "ccd", "--rank", "8", "--lambda", "5", "--max-epochs", "4",
+                        "--seed", "3", "--ref-quirks", str(flags), "--json"], check=True, capture_output=True,
+                       text=True)
+    say("[cli ccd quirks]", r.stdout.strip().splitlines()[-1])
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    tr, te = ct.Coo.read(files[0]), ct.Coo.read(files[4])
+    m = ct.Model(ct.Context(0), tr, None, None, rank=8)
+    m.init_random(3)
+    F = [f.astype(np.float64) for f in m.factors()]
+    otr = ocoo(tr)
+    ote = O.Coo(te.arrays()[0], te.arrays()[1], tr.dims)
+    for _ in range(summary["epochs"]):
+        O.ccd_jacobi_epoch(otr, F, 5.0, flags)
+    assert summary["rmse_te"] == pytest.approx(O.rmse(ote, F), rel=2e-5)
+
+
 def test_sgd_bold_driver_schedule():
     """The bold driver (SG/sgd.cu:314-323): after each epoch from the second
     on, lr x1.05 if loss + lambda * sum ||F||^2 fell, else x0.5.  Per-epoch lr
