@@ -75,7 +75,7 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
         CTC_HIP(launch_fill(st, col, m->dims[md], 0.f));  // rows with no nonzero -> 0
         {
           Prof p(m->ctx, "ccd_finish");
-          CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, o->lambda, col,
+          CTC_HIP(launch_ccd_finish(st, pieces, d.slot_pb, d.directory, d.dlength, d.npieces, o->lambda, col,
                                     (float4 *)m->ccdx[md]));
         }
       }

@@ -20,8 +20,10 @@ struct CcdColArgs {
 };
 
 hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsigned block);
+// npieces: the mode's piece count (picks one lane or one wave per slot)
 hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
-                             const uint32_t *directory, uint64_t dlength, double lam, float *col, float4 *x);
+                             const uint32_t *directory, uint64_t dlength, uint64_t npieces, double lam,
+                             float *col, float4 *x);
 hipError_t launch_ccd_pack(hipStream_t st, const float *ft, uint64_t rows, int col, int sub_col, float4 *x);
 hipError_t launch_ccd_resid(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
                             float *resid);

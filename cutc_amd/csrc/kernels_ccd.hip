@@ -149,31 +149,90 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
   }
 }
 
-// a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form).
-// One wave per slot: lane l sums pieces l, l+64, ... of the slot, then a fixed
-// shuffle tree adds the 64 partial sums (deterministic, parallel over the
-// thousands of pieces of a heavy row).
-__global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const uint32_t *slot_pb,
-                                                    const uint32_t *directory, uint64_t dlength,
-                                                    double lam, float *col, float4 *x) {
-  const uint64_t s = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (s >= dlength) return;
+// a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form),
+// written to the factor mirror and to the row's record (the current value
+// later modes' passes gather).  Each slot's pieces are summed in a fixed order
+// (deterministic).
+__device__ inline void ccd_finish_slot(uint64_t slot, double n, double d, const uint32_t *directory, double lam,
+                                       float *col, float4 *x) {
+  const uint32_t row = directory[slot];
+  const float v = (float)(n / (lam + d));
+  col[row] = v;
+  x[row].z = v;
+}
+
+// one wave per slot: lane l sums pieces l, l + 64, ... into four partial sums
+// (piece l + 64 i into partial i mod 4, so four loads are in flight), adds
+// them in order, and a fixed shuffle tree adds the lanes (parallel over the
+// thousands of pieces of a long row)
+__device__ inline void ccd_finish_wave(uint64_t slot, const double *pieces, const uint32_t *slot_pb,
+                                       const uint32_t *directory, double lam, float *col, float4 *x) {
   const int lane = lane_id();
-  double n = 0.0, d = 0.0;
-  for (uint32_t p = slot_pb[s] + lane; p < slot_pb[s + 1]; p += 64) {
-    n += pieces[2 * (size_t)p];
-    d += pieces[2 * (size_t)p + 1];
+  const double2 *pc = reinterpret_cast<const double2 *>(pieces);
+  const uint32_t p1 = slot_pb[slot + 1];
+  double n4[4] = {0.0, 0.0, 0.0, 0.0}, d4[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t p = slot_pb[slot] + lane;
+  for (; p + 3 * 64 < p1; p += 4 * 64) {
+    double2 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = pc[p + u * 64];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) n4[u] += q[u].x, d4[u] += q[u].y;
   }
+  for (int u = 0; p < p1; p += 64, ++u) n4[u] += pc[p].x, d4[u] += pc[p].y;
+  double n = (n4[0] + n4[1]) + (n4[2] + n4[3]), d = (d4[0] + d4[1]) + (d4[2] + d4[3]);
   for (int o = 32; o > 0; o >>= 1) {
     n += __shfl_xor(n, o, 64);
     d += __shfl_xor(d, o, 64);
   }
-  if (lane == 0) {
-    const uint32_t row = directory[s];
-    const float v = (float)(n / (lam + d));
-    col[row] = v;
-    x[row].z = v;  // the current value later modes' passes gather
+  if (lane == 0) ccd_finish_slot(slot, n, d, directory, lam, col, x);
+}
+
+__global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const uint32_t *slot_pb,
+                                                    const uint32_t *directory, uint64_t dlength,
+                                                    double lam, float *col, float4 *x) {
+  const uint64_t s = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s < dlength) ccd_finish_wave(s, pieces, slot_pb, directory, lam, col, x);
+}
+
+// For modes whose slots hold about one piece (mode 0 of the benchmark tensor:
+// 480K rows, each meeting a span boundary rarely): a block takes 256
+// consecutive slots; a slot of at most kFinishLight pieces is summed by its
+// own lane in piece order, the heavier ones are listed in LDS and summed by a
+// wave each as above.  Measured per pass on the benchmark tensor: mode 0
+// 119 us (129 us with a wave for every slot); the other modes, whose slots
+// hold tens to thousands of pieces, 9 us with a wave per slot and longer with
+// lanes.  What remains of mode 0's time is its long rows: the hottest rows
+// have the lowest numbers, so they fall into the first blocks and a few waves
+// sum them one after another.
+constexpr uint32_t kFinishLight = 8;
+
+__global__ void __launch_bounds__(256) k_ccd_finish_light(const double *pieces, const uint32_t *slot_pb,
+                                                          const uint32_t *directory, uint64_t dlength,
+                                                          double lam, float *col, float4 *x) {
+  __shared__ uint32_t heavy[256];
+  __shared__ uint32_t nheavy;
+  const uint64_t s0 = (uint64_t)blockIdx.x * blockDim.x;
+  const uint64_t s = s0 + threadIdx.x;
+  if (threadIdx.x == 0) nheavy = 0;
+  __syncthreads();
+  if (s < dlength) {
+    const uint32_t p0 = slot_pb[s], p1 = slot_pb[s + 1];
+    if (p1 - p0 <= kFinishLight) {
+      double n = 0.0, d = 0.0;
+      for (uint32_t p = p0; p < p1; ++p) {
+        const double2 q = reinterpret_cast<const double2 *>(pieces)[p];
+        n += q.x;
+        d += q.y;
+      }
+      ccd_finish_slot(s, n, d, directory, lam, col, x);
+    } else {
+      heavy[atomicAdd(&nheavy, 1u)] = threadIdx.x;
+    }
   }
+  __syncthreads();
+  for (uint32_t h = threadIdx.x >> 6; h < nheavy; h += blockDim.x >> 6)
+    ccd_finish_wave(s0 + heavy[h], pieces, slot_pb, directory, lam, col, x);
 }
 
 // per-row records of column `col` at the start of its update
@@ -260,10 +319,15 @@ hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsig
 }
 
 hipError_t launch_ccd_finish(hipStream_t st, const double *pieces, const uint32_t *slot_pb,
-                             const uint32_t *directory, uint64_t dlength, double lam, float *col, float4 *x) {
+                             const uint32_t *directory, uint64_t dlength, uint64_t npieces, double lam,
+                             float *col, float4 *x) {
   if (dlength == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 3) / 4)), dim3(256), 0, st, pieces,
-                     slot_pb, directory, dlength, lam, col, x);
+  if (npieces <= 2 * dlength)  // about one piece per slot
+    hipLaunchKernelGGL(k_ccd_finish_light, dim3((unsigned)((dlength + 255) / 256)), dim3(256), 0, st, pieces,
+                       slot_pb, directory, dlength, lam, col, x);
+  else
+    hipLaunchKernelGGL(k_ccd_finish, dim3((unsigned)((dlength + 3) / 4)), dim3(256), 0, st, pieces, slot_pb,
+                       directory, dlength, lam, col, x);
   return hipGetLastError();
 }
 
