@@ -88,6 +88,46 @@ def test_batch_matches_oracle_per_problem(ctx):
     assert worst < 1e-5
 
 
+def test_batch_config4_sample_ten_iterations(ctx):
+    """Sixteen problems of the 512-problem benchmark workload itself
+    (cutc_amd.workloads, its seeds: the first sixteen of at most 100K
+    nonzeros, ranks 8-64), 10 ALS iterations replayed from the hipGraph,
+    autotuned per class: every factor within 1e-5 of the FP64 oracle's 10
+    iterations from the same init, per-problem test RMSE within 1e-5."""
+    from cutc_amd import workloads as W
+    specs = W.batch_specs(512)
+    pick = [p for p, (d, n, r) in enumerate(specs) if n <= 100_000][:16]
+    probs = []
+    for p in pick:
+        dims, nnz, R = specs[p]
+        tr, te = ct.synth(dims, nnz, true_rank=max(1, R // 2), value_map=ct.VALUES_RAW, noise_std=0.1,
+                          test_fraction=0.1, seed=W.BATCH_SEED + 7919 * (p + 1))
+        F = W.init_factors(dims, R, np.random.default_rng(p))
+        probs.append((tr, te, R, F))
+    say(f"[batch_cfg4] problems {pick}, ranks {[p[2] for p in probs]}, nnz {[p[0].nnz for p in probs]}")
+    b = ct.Batch(ctx, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs])
+    opt = ct.Options(algorithm=ct.ALS, rank=16, lambda_=LAM)
+    load(b, probs)
+    b.autotune(opt)  # leaves the factors as they were
+    b.als(opt, iters=10, graph=True)
+    _, rte = b.rmse()
+    worst = 0.0
+    for p, (tr, te, R, F) in enumerate(probs):
+        ind, vals = tr.arrays()
+        oc = O.Coo(ind, vals, tr.dims)
+        F64 = [f.astype(np.float64) for f in F]
+        orders = [O.mode_order(oc, m) for m in range(3)]
+        for _ in range(10):
+            for m in range(3):
+                assert O.als_mode(oc, m, F64, LAM, order=orders[m], nthreads=8) == 0
+        for m in range(3):
+            worst = max(worst, relerr(b.factor(p, m), F64[m]))
+        ind, vals = te.arrays()
+        assert rte[p] == pytest.approx(O.rmse(O.Coo(ind, vals, te.dims), F64), rel=1e-5), p
+    say(f"  worst factor rel err {worst:.2e}")
+    assert worst < 1e-5
+
+
 def test_batch_graph_replays_match_direct(ctx):
     probs = make_problems(9, seed=2)
     say("[batch_graph]")
