@@ -1,4 +1,7 @@
-"""Full-size SGD parity: the default bulk-synchronous schedule on the
+"""Full-size parity on the 100M-nonzero benchmark tensor (SGD, CCD++, and
+sampled rows of an ALS sweep, each test's docstring).
+
+SGD: the default bulk-synchronous schedule on the
 100M-nonzero benchmark tensor (480189 x 17770 x 2182, Zipf(1.0) modes 0-1,
 integer values, 1% test; R = 32, lambda 0.05, lr 1e-3 x 0.95 per epoch,
 scale 1, factors U(-1,1) from numpy.random.default_rng(1)) against the FP64
@@ -42,13 +45,18 @@ def say(*a):
 
 
 @pytest.fixture(scope="module")
-def full_model():
+def full_tensor():
     say("[sgd_full] generating the 100M-nonzero tensor")
     tr, te = ct.synth(DIMS, 100_000_000, zipf=(1.0, 1.0, 0.0), true_rank=10, value_map=ct.VALUES_INT_1_5,
                       noise_std=0.3, test_fraction=0.01, seed=0x5EED0002)
-    ctx = ct.Context(0)
-    m = ct.Model(ctx, tr, None, te, rank=32)
-    del tr
+    yield tr, te
+    del tr, te
+
+
+@pytest.fixture(scope="module")
+def full_model(full_tensor):
+    tr, te = full_tensor
+    m = ct.Model(ct.Context(0), tr, None, te, rank=32)
     yield m
     del m
 
@@ -131,3 +139,32 @@ def test_ccd_full_size_matches_oracle(full_model):
     gap = np.max(np.abs(np.array(rm) - gold["rmse_te"]))
     say(f"  max test-RMSE gap {gap:.2e}")
     assert gap < 1e-4
+
+
+def test_als_full_size_sampled_rows_match_oracle(full_model, full_tensor):
+    """One ALS sweep (modes 0, 1, 2; the span kernels, R = 32, lambda 0.05)
+    on the full 100M-nonzero tensor from U(-1,1) factors: before each mode
+    update, the FP64 oracle solves 256 sampled rows of the mode (rows of at
+    most 100K nonzeros, so that the CPU finishes in seconds) from the same
+    FP32 factors (orc_als_rows_f64); the GPU's rows within 1e-5 relative."""
+    import oracle as O
+    tr, _ = full_tensor
+    ind, vals = tr.arrays()
+    oc = O.Coo(ind, vals, tr.dims)
+    del ind, vals
+    m = full_model
+    rng = np.random.default_rng(7)
+    m.set_factors([rng.uniform(-1, 1, (d, 32)).astype(np.float32) for d in DIMS])
+    opt = ct.Options(algorithm=ct.ALS, rank=32, lambda_=0.05)
+    for mode in range(3):
+        order = O.mode_order(oc, mode)
+        cnt = np.diff(order[1].astype(np.int64))
+        cand = np.flatnonzero((cnt > 0) & (cnt <= 100_000))
+        rows = np.sort(rng.choice(cand, size=min(256, len(cand)), replace=False)).astype(np.uint32)
+        F = m.factors()
+        ref = O.als_rows(oc, mode, F, 0.05, rows, order=order, nthreads=16)
+        m.als_mode_update(mode, opt)
+        got = m.factor(mode)[rows].astype(np.float64)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        say(f"  ALS mode {mode}: {len(rows)} rows ({int(cnt[rows].sum())} nonzeros), rel err {err:.2e}")
+        assert err < 1e-5, (mode, err)
