@@ -350,6 +350,23 @@ def als_rows(t: Coo, mode, F, lam, rows, order=None, nthreads=1):
     return out
 
 
+def grad_rows(t: Coo, mode, F, lam, rows, reg_per_occurrence=False, order=None, nthreads=1):
+    """Rows `rows` of mode `mode`'s loss gradient in FP64 from float32 factors
+    (orc_grad_rows_f64) -> (len(rows), R) float64 array."""
+    perm, rowptr = order if order is not None else mode_order(t, mode)
+    R = F[0].shape[1]
+    assert all(f.dtype == np.float32 for f in F)
+    rows = np.ascontiguousarray(rows, np.uint32)
+    out = np.zeros((len(rows), R), np.float64)
+    fn = lib().orc_grad_rows_f64
+    fn.restype = C.c_int
+    st = fn(t.c(), C.c_uint32(mode), C.c_uint32(R), _ptrs(F, "f32"), C.c_double(lam),
+            C.c_int(1 if reg_per_occurrence else 0), _u32p(perm), _u64p(rowptr), _u32p(rows),
+            C.c_uint64(len(rows)), _dp(out), C.c_int(nthreads))
+    assert st == 0, st
+    return out
+
+
 def sgd_epoch(t: Coo, F, lr, lam, scale=1.0, order=None):
     R = F[0].shape[1]
     op = _u32p(order) if order is not None else None

@@ -425,6 +425,44 @@ int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, do
 /* The FP64 ALS update of selected rows only (same rule as orc_als_mode_f64,
  * SG/als.cu:691-716), from float factors: out[r * R + f] for rows[r].  For
  * checking a full-size update on a sample of rows.  Returns 0 or 7. */
+/* Rows `rows` of mode `mode`'s loss gradient in FP64 from float factors, as
+ * orc_grad_f64 computes it: g_i = -sum over the row's nonzeros p of e_p h_p
+ * + lambda c_i a_i, e_p = x_p - <F_0(i0,:), F_1(i1,:), ...>, h_p the product
+ * of the other modes' rows, c_i the row's nonzero count (reg_per_occurrence)
+ * or 1.  perm / rowptr: orc_mode_order.  out: nrows x R.  Returns 0, or 1 on
+ * bad arguments. */
+int orc_grad_rows_f64(const orc_coo *t, uint32_t mode, uint32_t R, float *const *F, double lambda,
+                      int reg_per_occurrence, const uint32_t *perm, const uint64_t *rowptr,
+                      const uint32_t *rows, uint64_t nrows, double *out, int nthreads) {
+  if (mode >= t->nmodes || R == 0 || R > 512) return 1;
+  (void)nthreads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int64_t r = 0; r < (int64_t)nrows; ++r) {
+    const uint32_t i = rows[r];
+    double *g = out + (size_t)r * R;
+    const float *a = F[mode] + (size_t)i * R;
+    double h[512];
+    for (uint32_t f = 0; f < R; ++f) g[f] = 0.0;
+    for (uint64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const uint64_t p = perm[q];
+      for (uint32_t f = 0; f < R; ++f) h[f] = 1.0;
+      for (uint32_t m = 0; m < t->nmodes; ++m) {
+        if (m == mode) continue;
+        const float *row = F[m] + (size_t)t->ind[m][p] * R;
+        for (uint32_t f = 0; f < R; ++f) h[f] *= (double)row[f];
+      }
+      double e = t->vals[p];
+      for (uint32_t f = 0; f < R; ++f) e -= (double)a[f] * h[f];
+      for (uint32_t f = 0; f < R; ++f) g[f] -= e * h[f];
+    }
+    const double c = reg_per_occurrence ? (double)(rowptr[i + 1] - rowptr[i]) : 1.0;
+    for (uint32_t f = 0; f < R; ++f) g[f] += lambda * c * (double)a[f];
+  }
+  return 0;
+}
+
 int orc_als_rows_f64(const orc_coo *t, uint32_t mode, uint32_t R, float *const *F, double lambda,
                      const uint32_t *perm, const uint64_t *rowptr, const uint32_t *rows, uint64_t nrows,
                      double *out, int nthreads) {

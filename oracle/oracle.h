@@ -195,6 +195,13 @@ float orc_bf16_round(float x);
 int orc_als_mode_bf16(const orc_coo *t, uint32_t mode, uint32_t R, float **F, double lambda,
                       const uint32_t *perm, const uint64_t *rowptr, int nthreads);
 
+/* Rows of one mode's loss gradient in FP64 from float factors (sampled check
+ * of a full-size gradient), as orc_grad_f64: out[r * R + f] for rows[r].
+ * Returns 0, or 1 on bad arguments. */
+int orc_grad_rows_f64(const orc_coo *t, uint32_t mode, uint32_t R, float *const *F, double lambda,
+                      int reg_per_occurrence, const uint32_t *perm, const uint64_t *rowptr,
+                      const uint32_t *rows, uint64_t nrows, double *out, int nthreads);
+
 /* FP64 ALS update of selected rows from float factors (sampled check of a
  * full-size update): out[r * R + f] for rows[r].  Returns 0 or 7. */
 int orc_als_rows_f64(const orc_coo *t, uint32_t mode, uint32_t R, float *const *F, double lambda,

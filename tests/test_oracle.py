@@ -995,3 +995,20 @@ def test_ccd_jacobi_is_a_ridge_step_on_the_residual():
     want[has] = (m[has] - ga[has]) / (0.7 + np.einsum("irr->ir", G)[has])
     O.ccd_jacobi_mode(t, mode, F, 0.7)
     np.testing.assert_allclose(F[mode], want, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("per_occ", [False, True])
+def test_grad_rows_equal_the_full_gradient(per_occ):
+    """orc_grad_rows_f64 (sampled rows of a mode's gradient, for full-size
+    checks) equals those rows of orc_grad_f64 from the same float factors,
+    for every mode of a 4-way tensor, rows with and without nonzeros."""
+    rng = np.random.default_rng(21)
+    dims, n, R = (40, 20, 10, 4), 3000, 8
+    # mode-0 rows 30..39 hold no nonzero
+    t = O.Coo([rng.integers(0, min(d, 30), n) for d in dims], rng.normal(0, 1, n), dims)
+    F32 = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
+    _, G = O.grad(t, [f.astype(np.float64) for f in F32], 0.3, per_occ)
+    for mode in range(4):
+        rows = np.arange(dims[mode], dtype=np.uint32)
+        got = O.grad_rows(t, mode, F32, 0.3, rows, per_occ)
+        np.testing.assert_allclose(got, G[mode], rtol=1e-12, atol=1e-12)
