@@ -685,7 +685,7 @@ def test_autograd_finite_differences_and_sgd_direction(ctx):
         G = [f.grad.detach().cpu().numpy().astype(np.float64) for f in Ft]
         F64 = [f.astype(np.float64) for f in F]
         oloss, _ = O.grad(oc, F64, lam, per_occ)
-        assert float(loss) == pytest.approx(oloss, rel=1e-5)
+        assert float(loss.detach()) == pytest.approx(oloss, rel=1e-5)
         rng = np.random.default_rng(5)
         eps = 1e-3
         for md in range(4):
@@ -728,6 +728,6 @@ def test_autograd_training_decreases_loss(ctx):
         loss = completion_loss(m, Ft, 0.05)
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     assert all(b < a for a, b in zip(losses, losses[1:])), losses[::5]
     assert losses[-1] < 0.8 * losses[0], losses[::5]
