@@ -196,15 +196,14 @@ __global__ void __launch_bounds__(256) k_ccd_finish(const double *pieces, const 
 }
 
 // For modes whose slots hold about one piece (mode 0 of the benchmark tensor:
-// 480K rows, each meeting a span boundary rarely): a block takes 256
-// consecutive slots; a slot of at most kFinishLight pieces is summed by its
-// own lane in piece order, the heavier ones are listed in LDS and summed by a
-// wave each as above.  Measured per pass on the benchmark tensor: mode 0
-// 119 us (129 us with a wave for every slot); the other modes, whose slots
-// hold tens to thousands of pieces, 9 us with a wave per slot and longer with
-// lanes.  What remains of mode 0's time is its long rows: the hottest rows
-// have the lowest numbers, so they fall into the first blocks and a few waves
-// sum them one after another.
+// 480K rows, each meeting a span boundary rarely): a block takes 256 slots;
+// a slot of at most kFinishLight pieces is summed by its own lane in piece
+// order, the heavier ones are listed in LDS and summed by a wave each as
+// above.  Measured per pass on the benchmark tensor: mode 0 35.5 us (129 us
+// with a wave for every slot, 119 us with 256 consecutive slots per block,
+// where the hottest rows, numbered first, queued in the first blocks); the
+// other modes, whose slots hold tens to thousands of pieces, 9 us with a wave
+// per slot and longer with lanes.
 constexpr uint32_t kFinishLight = 8;
 
 __global__ void __launch_bounds__(256) k_ccd_finish_light(const double *pieces, const uint32_t *slot_pb,
@@ -212,8 +211,10 @@ __global__ void __launch_bounds__(256) k_ccd_finish_light(const double *pieces, 
                                                           double lam, float *col, float4 *x) {
   __shared__ uint32_t heavy[256];
   __shared__ uint32_t nheavy;
-  const uint64_t s0 = (uint64_t)blockIdx.x * blockDim.x;
-  const uint64_t s = s0 + threadIdx.x;
+  // slots are dealt to the blocks in a stride (block b takes b, b + G, ...):
+  // the long rows are the hottest, numbered first, and land in different
+  // blocks instead of queueing in the first ones
+  const uint64_t s = (uint64_t)blockIdx.x + (uint64_t)threadIdx.x * gridDim.x;
   if (threadIdx.x == 0) nheavy = 0;
   __syncthreads();
   if (s < dlength) {
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(256) k_ccd_finish_light(const double *pieces, 
   }
   __syncthreads();
   for (uint32_t h = threadIdx.x >> 6; h < nheavy; h += blockDim.x >> 6)
-    ccd_finish_wave(s0 + heavy[h], pieces, slot_pb, directory, lam, col, x);
+    ccd_finish_wave((uint64_t)blockIdx.x + (uint64_t)heavy[h] * gridDim.x, pieces, slot_pb, directory, lam, col, x);
 }
 
 // per-row records of column `col` at the start of its update
