@@ -71,13 +71,16 @@ def cfg2_tensor(ct, nnz, seed=0x5EED0002):
 SGD_SAMPLE = 17
 
 
-def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None):
-    """SGD epochs with the default schedule; -> (ms list, trajectory, walk ms per launch, walk launches per epoch,
-    merge ms per epoch, options)."""
+def sgd_run(ct, ctx, model, nnz, R, lam, steps, warmup, F0, precision=None, shape=None):
+    """SGD epochs with the default schedule (or `shape` = dict of tile_len /
+    sgd_substep / block); -> (ms list, trajectory, walk ms per launch, walk
+    launches per epoch, merge ms per epoch, options)."""
     import numpy as np  # noqa: F401
     model.set_factors(F0)
     opt = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam, learning_rate=1e-3, lr_schedule=ct.LR_EXP_DECAY,
                      lr_decay=0.95, sgd_scale=1.0, precision=ct.PREC_FP32 if precision is None else precision)
+    for k, v in (shape or {}).items():
+        setattr(opt, k, v)
     t0 = time.time()
     model.prepare(opt)
     model.sgd_epoch(opt, lr=0.0)  # builds the schedule's tables (lr 0: factors unchanged)
@@ -585,6 +588,19 @@ def share_leg(ct, ctx, a, nnz, R, lam):
          "walk_kernel_ms": walk_ms, "walk_launches_per_epoch": walks, "merge_ms_per_epoch": merge_ms,
          "roofline_frac": alg / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rmse_te_per_epoch": [round(x, 5) for x in traj]}
     log(f"share {nnz}: {w * 1e3:.2f} ms/epoch ({tr.nnz / w / 1e9:.2f} Gnnz/s), walk roofline {r['roofline_frac']:.2f}")
+    if nnz <= 20_000_000:
+        # the schedule's shape (walker span, sub-step, block) tuned for the share, as the reference's GA
+        # tunes its tile length: timed epochs at lr 0, then the same run with the tuned shape
+        base = ct.Options(algorithm=ct.SGD, rank=R, lambda_=lam)
+        t0 = time.time()
+        tuned = m.autotune(base)
+        shape = {"tile_len": tuned["tile_len"], "sgd_substep": tuned["sgd_substep"], "block": tuned["block"]}
+        times, traj, walk_ms, walks, merge_ms, _ = sgd_run(ct, ctx, m, tr.nnz, R, lam, 5, 1, F0, shape=shape)
+        wt = float(np.median(times))
+        r["tuned"] = dict(shape, tune_s=time.time() - t0, trials=tuned["trials"], s_per_epoch=wt,
+                          nnz_per_s=tr.nnz / wt, walk_kernel_ms=walk_ms, merge_ms_per_epoch=merge_ms,
+                          rmse_te_per_epoch=[round(x, 5) for x in traj])
+        log(f"share {nnz} tuned {shape}: {wt * 1e3:.2f} ms/epoch ({tr.nnz / wt / 1e9:.2f} Gnnz/s)")
     return r
 
 

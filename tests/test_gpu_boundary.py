@@ -299,7 +299,8 @@ def test_sgd_autotune_restores_model_and_tuned_shape_matches_oracle():
     t = m.autotune(opt)
     say(f"[sgd_autotune] {t}")
     assert t["tile_len"] in (1024, 2048, 4096, 8192) and t["sgd_substep"] in (16, 32, 64)
-    assert t["block"] in (64, 256) and t["trials"] == 24
+    assert t["tile_len"] >= 64 * t["sgd_substep"]  # at least 64 sub-steps per epoch
+    assert t["block"] in (64, 256) and t["trials"] == 18  # 9 shapes x 2 blocks
     for md in range(3):
         np.testing.assert_array_equal(m.factor(md), F1[md])
         np.testing.assert_array_equal(m.factors(best=True)[md], F0[md])
