@@ -157,7 +157,9 @@ struct ctc_model {
   // scratch
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
-  double *dstat = nullptr;  // small device buffer for reductions (256 doubles)
+  double *dstat = nullptr;  // small device buffer for reductions (4096 doubles)
+  double *eval_part = nullptr;  // per-block squared residuals of the training-loss walk
+  uint64_t eval_part_n = 0;
   double *hstat = nullptr;  // pinned host mirror
   uint32_t *dflag = nullptr;  // device status flags (not-SPD etc.)
   // CCD++ residuals per mode (in that mode's CISS order)

@@ -657,8 +657,10 @@ __global__ void __launch_bounds__(256) k_sgd_fold(SettleArgs a) {
 // the per-block sum of squared residuals.  kWalkJacobi is the reference's
 // CCD sums (SG/ccd.cu:157-195): out receives acc = sum e h and den the
 // sum of h * h, per column, all from the factors as they were at launch.
+// kWalkLoss only sums the squared residuals into loss_part (the training
+// loss of tc_loss_sq, SG/completion.c:166-189).
 // ==========================================================================
-enum : int { kWalkMttkrp = 0, kWalkAdjoint = 1, kWalkJacobi = 2 };
+enum : int { kWalkMttkrp = 0, kWalkAdjoint = 1, kWalkJacobi = 2, kWalkLoss = 3 };
 
 template <int R, int N, int KIND>
 __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint32_t span,
@@ -687,7 +689,9 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
   uint32_t arow = NONE, slot = 0;
 #pragma unroll
   for (int q = 0; q < CPL; ++q) acc[q] = 0.f, own[q] = 0.f, sq[q] = 0.f;
+  constexpr bool LOSS = KIND == kWalkAdjoint || KIND == kWalkLoss;  // the squared residuals are summed
   auto flush = [&]() {
+    if (KIND == kWalkLoss) return;
     float *o = out + (size_t)arow * R + wl * CPL;
 #pragma unroll
     for (int q2 = 0; q2 < CPL; ++q2) {
@@ -750,7 +754,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
 #pragma unroll
             for (int q2 = 0; q2 < CPL; ++q2) part += own[q2] * h[q2];
             w = x - group_sum<W>(part);
-            if (KIND == kWalkAdjoint && wl == 0) lsum += (double)w * (double)w;
+            if (LOSS && wl == 0) lsum += (double)w * (double)w;
           }
 #pragma unroll
           for (int q2 = 0; q2 < CPL; ++q2) {
@@ -762,7 +766,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(CissArgs c, FactorArgs fa, uint3
     }
     if (arow != NONE) flush();
   }
-  if (KIND == kWalkAdjoint) {
+  if (LOSS) {
     double t = wave_sum_d(lsum);
     if (lane == 0) red[threadIdx.x >> 6] = t;
     __syncthreads();
@@ -1176,9 +1180,10 @@ __global__ void __launch_bounds__(256) k_als_solve_h(const double *pieces, const
 }
 
 // ==========================================================================
-// Evaluation: sum of squared errors over a COO (validation / test) or over a
-// mode's CISS (training), FP64 accumulation, one partial per block (fixed
-// grid, fixed order -> deterministic).
+// Evaluation: sum of squared errors over a COO (validation / test) or per
+// problem of a batch, FP64 accumulation, one partial per block (fixed grid,
+// fixed order -> deterministic).  The training loss is a walk of the mode-0
+// CISS (k_mttkrp, kWalkLoss).
 // ==========================================================================
 template <int N>
 __device__ inline float predict_row(const float *const *F, const uint32_t *ix, int R) {
@@ -1225,25 +1230,6 @@ __global__ void __launch_bounds__(256) k_eval_coo(CooArgs c, FactorArgs fa, int 
     uint32_t ix[4];
 #pragma unroll
     for (int m = 0; m < N; ++m) ix[m] = c.ind[m][p];
-    const double e = (double)c.vals[p] - (double)predict_row<N>(F, ix, R);
-    acc += e * e;
-  }
-  block_store_sum(acc, part + blockIdx.x);
-}
-
-// training loss over the CISS of `mode` (factor pointers in CISS-relative order)
-template <int N>
-__global__ void __launch_bounds__(256) k_eval_ciss(CissArgs c, FactorArgs fa, int R, double *part) {
-  double acc = 0.0;
-  const float *F[4] = {fa.f[0], fa.f[1], fa.f[2], fa.f[3]};
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < c.nnz;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t t = p >> 6;
-    const uint32_t slot = tile_lane_slot(c.tile_slot[t], c.tile_mask[t], (int)(p & 63));
-    uint32_t ix[4];
-    ix[0] = c.directory[slot];
-#pragma unroll
-    for (int s = 0; s < N - 1; ++s) ix[1 + s] = c.oidx[s][p];
     const double e = (double)c.vals[p] - (double)predict_row<N>(F, ix, R);
     acc += e * e;
   }
@@ -1478,6 +1464,21 @@ hipError_t launch_ccd_jacobi(hipStream_t st, const CissArgs &c, const FactorArgs
   return hipGetLastError();
 }
 
+hipError_t launch_loss_walk(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                            uint32_t span, double *part, unsigned block) {
+  if (c.nnz == 0) return hipSuccess;
+  const unsigned grid = (unsigned)mttkrp_grid(c.nnz, R, span, block);
+  float *none = nullptr;
+  if (nmodes == 3) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 3, kWalkLoss>), dim3(grid), dim3(block), 0, st, c, f, span, none, part, none));
+  } else if (nmodes == 4) {
+    CTC_R_SWITCH(R, hipLaunchKernelGGL((k_mttkrp<RR, 4, kWalkLoss>), dim3(grid), dim3(block), 0, st, c, f, span, none, part, none));
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 uint64_t mttkrp_walkers_per_block(int R, unsigned block) { return (uint64_t)(block / 64) * (64 / (R / 4)); }
 
 uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block) {
@@ -1606,17 +1607,6 @@ hipError_t launch_eval_seg(hipStream_t st, const CissArgs *c, const CooArgs *co,
   else
     return hipErrorInvalidValue;
 #undef CTC_SEG
-  return hipGetLastError();
-}
-
-hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
-                            unsigned nblocks, double *part) {
-  if (nmodes == 3)
-    hipLaunchKernelGGL((k_eval_ciss<3>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
-  else if (nmodes == 4)
-    hipLaunchKernelGGL((k_eval_ciss<4>), dim3(nblocks), dim3(256), 0, st, c, f, R, part);
-  else
-    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

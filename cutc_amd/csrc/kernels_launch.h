@@ -22,6 +22,11 @@ hipError_t launch_mttkrp(hipStream_t st, const CissArgs &c, const FactorArgs &f,
                          uint32_t span, bool adjoint, float *out, double *loss_part,
                          unsigned *grid_out, unsigned block);
 uint64_t mttkrp_grid(uint64_t nnz, int R, uint32_t span, unsigned block);
+// the training loss as a walk of the CISS c (rows kept along their runs):
+// part[b] = the squared residuals of block b, for the mttkrp_grid(c.nnz, R,
+// span, block) blocks
+hipError_t launch_loss_walk(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
+                            uint32_t span, double *part, unsigned block);
 // walkers of k_mttkrp per workgroup of `block` threads (R / 4 lanes each)
 uint64_t mttkrp_walkers_per_block(int R, unsigned block);
 // the reference's CCD sums of one mode (SG/ccd.cu:157-195) over the CISS c:
@@ -42,8 +47,6 @@ hipError_t launch_eval_coo(hipStream_t st, const CooArgs &c, const FactorArgs &f
 // per-segment squared error: exactly one of c (mode-0 CISS order) / co (COO) is non-null
 hipError_t launch_eval_seg(hipStream_t st, const CissArgs *c, const CooArgs *co, const FactorArgs &f,
                            int nmodes, int R, const uint64_t *seg, uint32_t nseg, double *out);
-hipError_t launch_eval_ciss(hipStream_t st, const CissArgs &c, const FactorArgs &f, int nmodes, int R,
-                            unsigned nblocks, double *part);
 hipError_t launch_sumsq(hipStream_t st, const float *x, uint64_t n, unsigned nblocks, double *part);
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, uint32_t nb, uint32_t groups,
                                double *out);

@@ -461,6 +461,7 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(c->vals);
   }
   dev_free(m->scratch);
+  dev_free(m->eval_part);
   dev_free(m->dstat);
   dev_free(m->dflag);
   if (m->hstat) (void)hipHostFree(m->hstat);
@@ -1026,9 +1027,24 @@ extern "C" ctc_status ctc_evaluate(ctc_model *m, double lambda, double *loss, do
   FactorArgs f0 = rel_factors(m->F, N, 0);
   FactorArgs fa{};
   for (uint32_t md = 0; md < N; ++md) fa.f[md] = m->F[md];
+  // the training loss as a walk of the mode-0 CISS, each walker keeping its
+  // rows along their runs (kWalkLoss); its per-block sums are reduced in order
+  // into group 0's first slot, the group's other slots zeroed
+  const uint32_t span = 256;
+  const uint64_t nb = mttkrp_grid(m->ciss[0].nnz, (int)R, span, 256);
+  if (nb > m->eval_part_n) {
+    CTC_HIP(hipStreamSynchronize(st));
+    dev_free(m->eval_part);
+    m->eval_part = nullptr;
+    m->eval_part_n = 0;
+    CTC_TRY(dev_alloc(&m->eval_part, nb));
+    m->eval_part_n = nb;
+  }
   {
     Prof p(m->ctx, "eval");
-    CTC_HIP(launch_eval_ciss(st, ciss_args(m->ciss[0]), f0, (int)N, (int)R, kEvalBlocks, part));
+    CTC_HIP(launch_loss_walk(st, ciss_args(m->ciss[0]), f0, (int)N, (int)R, span, m->eval_part, 256));
+    CTC_HIP(launch_fill(st, (float *)(part + 1), 2 * (kEvalBlocks - 1), 0.f));  // 0.0 doubles
+    CTC_HIP(launch_reduce_parts(st, m->eval_part, (uint32_t)nb, 1, part));
     CooArgs cv{}, ct{};
     for (uint32_t md = 0; md < N; ++md) {
       cv.ind[md] = m->valid.ind[md];
