@@ -14,7 +14,7 @@ def pytest_configure(config):
 
 def pytest_collection_modifyitems(config, items):
     # GPU tests are skipped automatically only when no GPU is present and the
-    # run did not explicitly select them; with `-m gpu` on a GPU box they run.
+    # run did not explicitly select them; with `-m gpu` on an MI355X they run.
     try:
         import torch  # noqa: F401
         have_gpu = torch.cuda.is_available()
