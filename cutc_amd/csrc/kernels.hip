@@ -1294,7 +1294,8 @@ __global__ void __launch_bounds__(256) k_fill(float *x, uint64_t n, float v) {
 }
 
 // plain streaming copy of n 16-byte words: the bandwidth yardstick and the
-// known-size kernel that calibrates HBM traffic counters
+// known-size kernel that calibrates HBM traffic counters.  (Four loads in
+// flight per lane before the stores measured the same rate, 4.59 TB/s.)
 __global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (uint64_t)gridDim.x * blockDim.x)
