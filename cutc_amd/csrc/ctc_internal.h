@@ -198,6 +198,12 @@ struct ctc_model {
   uint32_t *sgd_piece_row[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   double *sgd_part[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
   ctc::FoldSeg *sgd_fold[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  // the pieces settled by a workgroup each (heavy) and the rows of at most
+  // kSettleLight runs settled by one walker each (light), per sub-step
+  uint32_t *sgd_heavy[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t *sgd_light[CTC_MAX_MODES] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint64_t> sgd_heavy_off[CTC_MAX_MODES];
+  std::vector<uint64_t> sgd_light_off[CTC_MAX_MODES];
   std::vector<uint64_t> sgd_slot_off[CTC_MAX_MODES];  // first staging position of each sub-step
   std::vector<uint64_t> sgd_piece_off[CTC_MAX_MODES];
   std::vector<uint64_t> sgd_fold_off[CTC_MAX_MODES];

@@ -532,6 +532,66 @@ __device__ inline void sgd_merge_block(const MergeArgs &a, uint64_t blk) {
 // walker order.  A piece that holds a whole row applies it; the others leave
 // their sums for k_sgd_fold.  The order is fixed, so the result is
 // reproducible.
+// A row of at most kSettleLight runs, settled by one walker (W lanes) of
+// light block lblk: its runs' deltas summed in position order in FP64, the
+// sum (cold row) or the mean (hot row) added to the row.
+template <int R, bool MIX>
+__device__ inline void sgd_settle_light(const SettleArgs &a, uint64_t lblk) {
+  constexpr int CPL = SyncShape<R>::CPL;
+  constexpr int W = SyncShape<R>::W;
+  constexpr int NW = 256 / W;
+  const uint64_t li = lblk * NW + threadIdx.x / W;
+  const int wl = threadIdx.x % W;
+  int k = 1;
+  while (k < 3 && li >= a.lb[k + 1]) ++k;
+  if (li >= a.lb[k + 1]) return;
+  const uint32_t gp = a.light[k][li - a.lb[k]];
+  const uint32_t b = a.piece_start[k][gp], e = a.piece_start[k][gp + 1];
+  const uint32_t row = a.piece_row[k][gp];
+  const float *stage = a.stage[k];
+  const uint32_t s0 = (uint32_t)a.slot0[k];
+  const uint32_t *slots = a.slots[k];
+  auto at = [&](uint32_t q) -> size_t { return MIX ? (size_t)(q - s0) : (size_t)slots[q]; };
+  float *fr = a.F[k] + (size_t)row * R + wl * CPL;
+  float fv[CPL];
+  load_row4<CPL>(fr, fv);
+  const uint32_t hw = a.hot[k][row >> 5];
+  double acc[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) acc[j] = 0.0;
+  uint32_t p = b;
+  for (; p + 3 < e; p += 4) {  // four slots in flight, added in position order
+    float x[4][CPL];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if constexpr (MIX)
+        load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + at(p + u) * R + wl * CPL, x[u]);
+      else
+        load_row4_nt<CPL>(stage + at(p + u) * R + wl * CPL, x[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) acc[j] += (double)x[u][j];
+  }
+  for (; p < e; ++p) {
+    float x[CPL];
+    if constexpr (MIX)
+      load_bf4_nt<CPL>(reinterpret_cast<const uint16_t *>(stage) + at(p) * R + wl * CPL, x);
+    else
+      load_row4_nt<CPL>(stage + at(p) * R + wl * CPL, x);
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] += (double)x[j];
+  }
+  const double sc = ((hw >> (row & 31)) & 1u) ? 1.0 / (double)(e - b) : 1.0;
+  float out[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) out[j] = (float)((double)fv[j] + acc[j] * sc);
+  store_row4<CPL>(fr, out);
+}
+
+// Blocks 0 .. pb[4] - 1: one heavy piece each (a workgroup sums its slots);
+// the blocks after them: NW light rows each (sgd_settle_light).
 template <int R, bool MIX>
 __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   constexpr int CPL = SyncShape<R>::CPL;  // a slot's row moves as 16-byte pieces
@@ -541,10 +601,13 @@ __global__ void __launch_bounds__(256) k_sgd_settle(SettleArgs a) {
   __shared__ double part[NW][R];
   __shared__ double red[G][R];
   const uint64_t pblk = blockIdx.x;
-  if (pblk >= a.pb[4]) return;
+  if (pblk >= a.pb[4]) {  // uniform per block: the light path has no barrier
+    sgd_settle_light<R, MIX>(a, pblk - a.pb[4]);
+    return;
+  }
   int k = 1;
   while (k < 3 && pblk >= a.pb[k + 1]) ++k;
-  const uint64_t gp = a.piece0[k] + (pblk - a.pb[k]);
+  const uint64_t gp = a.heavy[k][pblk - a.pb[k]];
   const int wid = threadIdx.x / W;
   const int wl = threadIdx.x % W;
   const uint32_t b = a.piece_start[k][gp], e = a.piece_start[k][gp + 1];
@@ -1412,12 +1475,14 @@ hipError_t launch_sgd_settle(hipStream_t st, SettleArgs a, int R, bool mixed) {
   const int W = R < 64 ? R : 64;
   const uint64_t items = a.mg.off[a.mg.nseg];
   a.nb_merge = grid_for((items + (64 / W) - 1) / (64 / W), 256);
-  if (a.pb[4] > 0x7FFFFFFFull) return hipErrorInvalidValue;
-  if (a.pb[4] > 0) {
+  const uint64_t nw = 1024 / (uint64_t)R;  // light rows per block: 256 / (R / 4) walkers
+  const uint64_t sb = a.pb[4] + (a.lb[4] + nw - 1) / nw;
+  if (sb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  if (sb > 0) {
     if (mixed) {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, true>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, true>), dim3((unsigned)sb), dim3(256), 0, st, a));
     } else {
-      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, false>), dim3((unsigned)a.pb[4]), dim3(256), 0, st, a));
+      CTC_R_SWITCH(R, hipLaunchKernelGGL((k_sgd_settle<RR, false>), dim3((unsigned)sb), dim3(256), 0, st, a));
     }
   }
   const uint64_t fg = a.nb_merge + grid_for((a.fb[4] + (64 / W) - 1) / (64 / W), 256);

@@ -81,6 +81,9 @@ struct SyncArgs {
 // adds them in order for each such row (fold[g]).  A cold row adds the sum of
 // its runs' deltas, a hot row their mean.
 constexpr uint32_t kStagePiece = 1024;
+// rows of at most this many runs in a sub-step are settled by one walker each
+// (most rows of a skewed mode); longer ones by a workgroup per piece
+constexpr uint32_t kSettleLight = 32;
 struct FoldSeg {
   uint32_t row, piece, npiece, runs;  // piece: first piece, sub-step relative
 };
@@ -114,7 +117,10 @@ struct SettleArgs {
   double *part[4];
   const FoldSeg *fold[4];  // the sub-step's multi-piece rows
   uint64_t piece0[4];      // the sub-step's first piece
-  uint64_t pb[5];          // mode k's pieces: k_sgd_settle's blocks pb[k] .. pb[k + 1]
+  const uint32_t *heavy[4];  // the sub-step's pieces settled by a workgroup each (epoch-numbered)
+  const uint32_t *light[4];  // the sub-step's rows of at most kSettleLight runs, a walker each
+  uint64_t pb[5];          // mode k's heavy pieces: k_sgd_settle's blocks pb[k] .. pb[k + 1]
+  uint64_t lb[5];          // mode k's light rows: walkers lb[k] .. lb[k + 1] of the blocks after pb[4]
   uint64_t fb[5];          // mode k's multi-piece rows: k_sgd_fold's walkers fb[k] .. fb[k + 1] after the merge
 };
 

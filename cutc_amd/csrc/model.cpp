@@ -450,6 +450,8 @@ extern "C" void ctc_model_destroy(ctc_model *m) {
     dev_free(m->sgd_piece_row[md]);
     dev_free(m->sgd_part[md]);
     dev_free(m->sgd_fold[md]);
+    dev_free(m->sgd_heavy[md]);
+    dev_free(m->sgd_light[md]);
     dev_free(m->sgd_hidx[md]);
     dev_free(m->sgd_hrow[md]);
     dev_free(m->sgd_Dh[md]);
@@ -746,8 +748,9 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
   for (uint32_t k = 1; k < m->nmodes; ++k) {
     for (void *p : {(void *)m->sgd_stage[k], (void *)m->sgd_slots[k], (void *)m->sgd_dest[k], (void *)m->sgd_sbase[k],
                     (void *)m->sgd_piece_start[k], (void *)m->sgd_piece_row[k], (void *)m->sgd_part[k],
-                    (void *)m->sgd_fold[k]})
+                    (void *)m->sgd_fold[k], (void *)m->sgd_heavy[k], (void *)m->sgd_light[k]})
       dev_free(p);
+    m->sgd_heavy[k] = m->sgd_light[k] = nullptr;
     m->sgd_stage[k] = nullptr;
     m->sgd_part[k] = nullptr;
     m->sgd_fold[k] = nullptr;
@@ -823,6 +826,24 @@ static ctc_status ensure_sync(ctc_model *m, uint32_t span, uint32_t sub) {
     poff[nsub] = piece_start.size();
     foff[nsub] = fold.size();
     piece_start.push_back((uint32_t)key.size());
+    // per sub-step: rows of at most kSettleLight runs (one piece each) go to
+    // the light list, every other piece to the heavy one
+    std::vector<uint32_t> heavy, light;
+    std::vector<uint64_t> &hoff = m->sgd_heavy_off[k], &loff = m->sgd_light_off[k];
+    hoff.assign(nsub + 1, 0);
+    loff.assign(nsub + 1, 0);
+    for (uint64_t s2 = 0; s2 < nsub; ++s2) {
+      hoff[s2] = heavy.size();
+      loff[s2] = light.size();
+      for (uint64_t gp = poff[s2]; gp < poff[s2 + 1]; ++gp) {
+        const bool lt = piece_row[gp] != 0xFFFFFFFFu && piece_start[gp + 1] - piece_start[gp] <= ctc::kSettleLight;
+        (lt ? light : heavy).push_back((uint32_t)gp);
+      }
+    }
+    hoff[nsub] = heavy.size();
+    loff[nsub] = light.size();
+    CTC_TRY(upload(&m->sgd_heavy[k], heavy.data(), heavy.size(), st));
+    CTC_TRY(upload(&m->sgd_light[k], light.data(), light.size(), st));
     CTC_TRY(upload(&m->sgd_piece_start[k], piece_start.data(), piece_start.size(), st));
     CTC_TRY(upload(&m->sgd_piece_row[k], piece_row.data(), piece_row.size(), st));
     CTC_TRY(upload(&m->sgd_fold[k], fold.data(), fold.size(), st));
@@ -922,7 +943,10 @@ static ctc_status sgd_epoch_sync(ctc_model *m, const ctc_options *o, double lr) 
         sa.piece0[k] = staged ? m->sgd_piece_off[k][s] : 0;
         sa.slot0[k] = staged ? m->sgd_slot_off[k][s] : 0;
         sa.fold[k] = staged ? m->sgd_fold[k] + m->sgd_fold_off[k][s] : nullptr;
-        sa.pb[k + 1] = sa.pb[k] + (staged ? m->sgd_piece_off[k][s + 1] - m->sgd_piece_off[k][s] : 0);
+        sa.heavy[k] = staged ? m->sgd_heavy[k] + m->sgd_heavy_off[k][s] : nullptr;
+        sa.light[k] = staged ? m->sgd_light[k] + m->sgd_light_off[k][s] : nullptr;
+        sa.pb[k + 1] = sa.pb[k] + (staged ? m->sgd_heavy_off[k][s + 1] - m->sgd_heavy_off[k][s] : 0);
+        sa.lb[k + 1] = sa.lb[k] + (staged ? m->sgd_light_off[k][s + 1] - m->sgd_light_off[k][s] : 0);
         sa.fb[k + 1] = sa.fb[k] + (staged ? m->sgd_fold_off[k][s + 1] - m->sgd_fold_off[k][s] : 0);
       }
       CTC_HIP(launch_sgd_settle(st, sa, (int)R, mixed));
