@@ -67,6 +67,14 @@ ctc_status ccd_epoch_impl(ctc_model *m, const ctc_options *o) {
         a.nspans = d.nspans;
         a.span_off = d.span_off;
         a.pieces = pieces;
+        a.lds_s = -1;  // a small other mode's records are read from LDS (3-way tensors)
+        if (N == 3)
+          for (int s = 0; s < 2; ++s)
+            if (m->dims[(md + 1 + s) % N] <= kCcdLdsRows) {
+              a.lds_s = s;
+              a.lds_rows = (uint32_t)m->dims[(md + 1 + s) % N];
+              break;
+            }
         float *col = m->FT[md] + (size_t)f * m->dims[md];
         {
           Prof p(m->ctx, "ccd_col");

@@ -17,7 +17,10 @@ struct CcdColArgs {
   uint64_t nspans;
   const uint32_t *span_off;
   double *pieces;             // (num, den) per piece
+  int lds_s;                  // other mode (CISS-relative 1 + lds_s) read from LDS, or -1
+  uint32_t lds_rows;          // its rows (at most kCcdLdsRows)
 };
+constexpr uint32_t kCcdLdsRows = 2304;  // 36 KB of 16-byte records
 
 hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsigned block);
 // npieces: the mode's piece count (picks one lane or one wave per slot)

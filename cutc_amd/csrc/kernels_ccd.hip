@@ -51,11 +51,12 @@ __device__ inline void seg_scan(float &a, float &b, bool f) {
   }
 }
 
-template <int N>
-__global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
+// The pass of wave w over its span.  LS >= 0: the records of other mode LS
+// (a mode of at most kCcdLdsRows rows) are read from the workgroup's copy xs
+// in LDS instead of gathered from memory.
+template <int N, int LS>
+__device__ __forceinline__ void ccd_col_wave(const CcdColArgs &a, uint64_t w, const float4 *xs) {
   const int lane = lane_id();
-  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= a.nspans) return;
   const CissArgs &c = a.c;
   const uint64_t t0 = w * a.span_tiles;
   const uint64_t t1 = t0 + a.span_tiles < c.ntiles ? t0 + a.span_tiles : c.ntiles;
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
       myslot = tile_lane_slot(first_slot, mask, lane);
       float4 q[N];
 #pragma unroll
-      for (int s = 0; s < N - 1; ++s) q[1 + s] = a.x[1 + s][c.oidx[s][p]];
+      for (int s = 0; s < N - 1; ++s) q[1 + s] = s == LS ? xs[c.oidx[s][p]] : a.x[1 + s][c.oidx[s][p]];
       float e = a.resid[p];
       if (fix) {
         q[0] = a.x[0][c.directory[myslot]];  // the own mode's row: pending and saved values
@@ -147,6 +148,26 @@ __global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
     a.pieces[(size_t)(carry_slot + poff) * 2] = carry_n;
     a.pieces[(size_t)(carry_slot + poff) * 2 + 1] = carry_d;
   }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_ccd_col(CcdColArgs a) {
+  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= a.nspans) return;
+  ccd_col_wave<N, -1>(a, w, nullptr);
+}
+
+// 1024-thread workgroups (16 spans) sharing one LDS copy of the records of a
+// small other mode: its gathers no longer go through the texture addresser.
+template <int N, int LS>
+__global__ void __launch_bounds__(1024) k_ccd_col_lds(CcdColArgs a) {
+  __shared__ float4 xs[kCcdLdsRows];
+  const float4 *src = a.x[1 + LS];
+  for (uint32_t i = threadIdx.x; i < a.lds_rows; i += blockDim.x) xs[i] = src[i];
+  __syncthreads();
+  const uint64_t w = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= a.nspans) return;
+  ccd_col_wave<N, LS>(a, w, xs);
 }
 
 // a_f(row) = num / (lambda + den) per directory slot (SG/ccd.cu:345 form),
@@ -308,6 +329,16 @@ __global__ void __launch_bounds__(256) k_transpose(const float *in, float *out, 
 
 hipError_t launch_ccd_col(hipStream_t st, const CcdColArgs &a, int nmodes, unsigned block) {
   if (a.c.nnz == 0) return hipSuccess;
+  if (a.lds_s >= 0 && a.lds_rows <= kCcdLdsRows) {
+    const unsigned g = (unsigned)((a.nspans + 15) / 16);
+    if (nmodes == 3 && a.lds_s == 0)
+      hipLaunchKernelGGL((k_ccd_col_lds<3, 0>), dim3(g), dim3(1024), 0, st, a);
+    else if (nmodes == 3 && a.lds_s == 1)
+      hipLaunchKernelGGL((k_ccd_col_lds<3, 1>), dim3(g), dim3(1024), 0, st, a);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   const uint64_t wpb = block / 64;
   const unsigned grid = (unsigned)((a.nspans + wpb - 1) / wpb);
   if (nmodes == 3)
